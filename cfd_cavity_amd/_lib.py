@@ -1,0 +1,107 @@
+"""ctypes loader for libcavlbm.so (the C-ABI declared in include/cavlbm.h).
+
+The library is the only compute path: if it is missing or fails to load,
+``lib()`` raises; nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcavlbm.so")
+
+E_PARAM, E_SIZE, E_HIP, E_UNSUPPORTED, E_NODEVICE = -1, -2, -3, -4, -5
+STORAGE = {"fp64": 0, "fp32": 1, "fp16_offset": 2}
+FIELD = {"rho": 0, "ux": 1, "uy": 2, "temperature": 3, "nu_t": 4, "fy_node": 5}
+PHASE = {"collide": 0, "stream": 1, "thermal_bc": 2, "update_macroscopic": 3,
+         "enforce_wall_velocity": 4, "bounce_back": 5}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint64), ("ny", C.c_uint64),
+        ("ra", C.c_double), ("pr", C.c_double), ("cs", C.c_double),
+        ("pr_t", C.c_double), ("utarget", C.c_double),
+        ("seed", C.c_uint64), ("jobs", C.c_uint64),
+        ("alpha_mult", C.c_double), ("nu_start_mult", C.c_double),
+        ("has_nu_override", C.c_int32), ("storage", C.c_int32),
+        ("nu_override", C.c_double),
+        ("geq_order", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class Diag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "nusselt_hot", "nusselt_cold", "nusselt_hot_wall_link", "nusselt_volume",
+        "total_mass", "total_f_mass", "max_uy", "max_u", "nu_t_max", "y_plus_hot")] + [
+        ("all_finite", C.c_int32), ("reserved", C.c_int32), ("time_step", C.c_uint64)]
+
+
+class Derived(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "lz", "lx", "nu", "alpha", "alpha_phys", "beta_eff", "tau", "tau_t", "cs",
+        "pr_t", "nu_start_mult", "t_ref")] + [("spin_up_steps", C.c_uint64)]
+
+
+# name -> (restype, argtypes); the exported set of include/cavlbm.h
+_vp, _u64, _i32, _d = C.c_void_p, C.c_uint64, C.c_int32, C.c_double
+_dp = C.POINTER(C.c_double)
+SIGNATURES = {
+    "cavlbm_abi_version": (C.c_int, []),
+    "cavlbm_params_default": (C.c_int, [C.POINTER(Params)]),
+    "cavlbm_create": (C.c_int, [C.POINTER(Params), C.POINTER(_vp)]),
+    "cavlbm_destroy": (C.c_int, [_vp]),
+    "cavlbm_step": (C.c_int, [_vp]),
+    "cavlbm_run": (C.c_int, [_vp, _u64]),
+    "cavlbm_synchronize": (C.c_int, [_vp]),
+    "cavlbm_init_from_fields": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _u64]),
+    "cavlbm_get_field": (C.c_int, [_vp, _i32, _dp, _u64]),
+    "cavlbm_set_field": (C.c_int, [_vp, _i32, _dp, _u64]),
+    "cavlbm_get_pops": (C.c_int, [_vp, _i32, _dp, _u64]),
+    "cavlbm_set_pops": (C.c_int, [_vp, _i32, _dp, _u64]),
+    "cavlbm_diagnostics": (C.c_int, [_vp, C.POINTER(Diag)]),
+    "cavlbm_max_uy_on_row": (C.c_int, [_vp, _u64, _dp, _dp]),
+    "cavlbm_derived_params": (C.c_int, [_vp, C.POINTER(Derived)]),
+    "cavlbm_time_step": (C.c_int, [_vp, C.POINTER(_u64)]),
+    "cavlbm_set_time_step": (C.c_int, [_vp, _u64]),
+    "cavlbm_set_hot_wall": (C.c_int, [_vp, _dp, _u64]),
+    "cavlbm_set_dynamic_sgs": (C.c_int, [_vp, _i32, _d, _d, _d]),
+    "cavlbm_set_jobs": (C.c_int, [_u64]),
+    "cavlbm_phase": (C.c_int, [_vp, _i32, _u64]),
+    "cavlbm_time_run": (C.c_int, [_vp, _u64, _dp]),
+    "cavlbm_time_step_kernel": (C.c_int, [_vp, _u64, _dp]),
+    "cavlbm_last_error": (C.c_char_p, [_vp]),
+}
+
+_lib = None
+
+
+class CavlbmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cavlbm error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libcavlbm.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != 0:
+        msg = lib().cavlbm_last_error(handle)
+        raise CavlbmError(rc, msg.decode() if msg else "")

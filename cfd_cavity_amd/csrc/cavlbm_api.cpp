@@ -1,0 +1,708 @@
+// cavlbm_api.cpp — the C-ABI (include/cavlbm.h) over the HIP kernels.
+//
+// State representation.  The reference keeps the post-stream populations and
+// the moments after each step (src/uniform/solver.rs:279-293).  On the device
+// the solver holds that state in one or both of two forms:
+//   M-form: post-stream f, g (FP64, 18 planes) and the fields rho, ux, uy, T
+//           exactly as the reference stores them;
+//   C-form: the post-collision populations of the last collision (storage
+//           type, ping-pong buffers).  Streaming, the thermal BCs, the moments
+//           with the Guo correction and the wall velocity are recomputed from
+//           it by the next fused step kernel, or by a materialize pass when
+//           the caller reads the state.
+// fy_node and nu_t belong to both forms (the last collision's values).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cavlbm.h"
+#include "lbm_launch.hpp"
+
+using namespace cavlbm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kGraphSteps = 64;  // steps per captured graph (even: keeps buffer parity)
+
+}  // namespace
+
+struct cavlbm_solver {
+  cavlbm_params p{};
+  cavlbm_derived d{};
+  Geom geo{};
+  int storage = 0, geq = 2;
+  size_t st_bytes = 8, r_bytes = 8;
+  hipStream_t st = nullptr;
+  void* cpop[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* mpops = nullptr;
+  double* mtmp = nullptr;
+  double* fld[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, T
+  double* scratch_field = nullptr;                        // FP64 conversion buffer
+  void* fy = nullptr;
+  void* nut = nullptr;
+  double* theta = nullptr;  // ny values, always present (0.5 when no profile)
+  double* red_part = nullptr;
+  double* red_out = nullptr;
+  bool c_valid = false, m_valid = false;
+  uint64_t time_step = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer parity
+  bool graph_failed = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int fail(cavlbm_solver* s, int code, const std::string& msg) {
+  if (s)
+    s->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(s, expr)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail((s), CAVLBM_E_HIP,                                                  \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+// Per-step scalars of collide(step): beta ramp, ramp_nu_alpha, tau_mol
+// (src/uniform/solver.rs:390-410, :433-442).
+StepCtx make_ctx(const cavlbm_solver* s, uint64_t step) {
+  StepCtx c;
+  const cavlbm_derived& d = s->d;
+  double beta = d.beta_eff, nu_eff = d.nu, alpha_eff = d.alpha;
+  if (step < d.spin_up_steps) {
+    const double x = (double)step / (double)d.spin_up_steps;
+    beta = d.beta_eff * x;
+    const double nu_start = d.nu * d.nu_start_mult;
+    nu_eff = nu_start * std::pow(d.nu / nu_start, x);
+    const double alpha_start = d.alpha * d.nu_start_mult;
+    alpha_eff = alpha_start * std::pow(d.alpha / alpha_start, x);
+  }
+  c.beta = beta;
+  c.nu = nu_eff;
+  c.alpha = alpha_eff;
+  c.tau = 0.5 + 3.0 * std::fmax(nu_eff, 0.002 / 3.0);
+  c.t_ref = d.t_ref;
+  c.cs = d.cs;
+  c.pr_t = d.pr_t;
+  return c;
+}
+
+StepBufs bufs(cavlbm_solver* s) {
+  StepBufs b;
+  b.src = s->cpop[s->cur];
+  b.dst = s->cpop[s->cur ^ 1];
+  b.fy = s->fy;
+  b.nut = s->nut;
+  b.mpops = s->mpops;
+  b.rho = s->fld[0];
+  b.ux = s->fld[1];
+  b.uy = s->fld[2];
+  b.temp = s->fld[3];
+  return b;
+}
+
+hipError_t launch(cavlbm_solver* s, int mode, const StepCtx& ctx, int write_aux) {
+  const Walls w{s->theta};
+  const StepBufs b = bufs(s);
+  if (s->storage == ST_FP64)
+    return launch_step_fp64(s->geq, mode, s->geo, ctx, w, b, write_aux, s->st);
+  return launch_step_lowp(s->storage, s->geq, mode, s->geo, ctx, w, b, write_aux, s->st);
+}
+
+int materialize(cavlbm_solver* s) {
+  if (s->m_valid) return 0;
+  if (!s->c_valid) return fail(s, CAVLBM_E_PARAM, "internal: no valid state");
+  HIP_TRY(s, launch(s, MODE_MATERIALIZE, make_ctx(s, s->time_step), 0));
+  s->m_valid = true;
+  return 0;
+}
+
+void drop_graphs(cavlbm_solver* s) {
+  for (auto& g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// One step: collide(time_step) on the current state, stream/BC/moments/walls
+// deferred to the next kernel.  Advances the C-form buffers.
+int one_step(cavlbm_solver* s, int write_aux) {
+  const StepCtx ctx = make_ctx(s, s->time_step);
+  HIP_TRY(s, launch(s, s->c_valid ? MODE_FUSED : MODE_FIRST, ctx, write_aux));
+  s->cur ^= 1;
+  s->c_valid = true;
+  s->m_valid = false;
+  s->time_step += 1;
+  return 0;
+}
+
+// Captures kGraphSteps fused steps starting from buffer parity `par`.
+hipGraphExec_t capture(cavlbm_solver* s, int par) {
+  const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);  // constant after spin-up
+  const int saved = s->cur;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  s->cur = par;
+  bool ok = true;
+  for (int i = 0; i < kGraphSteps && ok; ++i) {
+    ok = launch(s, MODE_FUSED, ctx, 0) == hipSuccess;
+    s->cur ^= 1;
+  }
+  s->cur = saved;
+  if (hipStreamEndCapture(s->st, &g) != hipSuccess || !ok) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  return ex;
+}
+
+int run_steps(cavlbm_solver* s, uint64_t n) {
+  uint64_t left = n;
+  while (left > 0) {
+    // Steady part: whole graphs of kGraphSteps fused steps (constant per-step
+    // scalars after spin-up), keeping the final step direct so that it writes
+    // nu_t.
+    if (s->c_valid && !s->graph_failed && s->time_step >= s->d.spin_up_steps &&
+        left > (uint64_t)kGraphSteps) {
+      hipGraphExec_t& g = s->graph[s->cur];
+      if (!g) g = capture(s, s->cur);
+      if (!g) {
+        s->graph_failed = true;
+        continue;
+      }
+      HIP_TRY(s, hipGraphLaunch(g, s->st));
+      s->time_step += kGraphSteps;
+      s->m_valid = false;
+      left -= kGraphSteps;
+      continue;
+    }
+    int rc = one_step(s, left == 1 ? 1 : 0);
+    if (rc) return rc;
+    --left;
+  }
+  return 0;
+}
+
+void free_all(cavlbm_solver* s) {
+  drop_graphs(s);
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(s->cpop[0]);
+  f(s->cpop[1]);
+  f(s->mpops);
+  f(s->mtmp);
+  for (auto* p : s->fld) f(p);
+  f(s->scratch_field);
+  f(s->fy);
+  f(s->nut);
+  f(s->theta);
+  f(s->red_part);
+  f(s->red_out);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->st) (void)hipStreamDestroy(s->st);
+}
+
+// Host <-> device copies of one pitched per-node FP64 field.
+int field_down(cavlbm_solver* s, const double* dev, double* host) {
+  HIP_TRY(s, hipMemcpy2DAsync(host, s->p.nx * sizeof(double), dev,
+                              (size_t)s->geo.pitch * sizeof(double), s->p.nx * sizeof(double),
+                              s->p.ny, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int field_up(cavlbm_solver* s, const double* host, double* dev) {
+  HIP_TRY(s, hipMemcpy2DAsync(dev, (size_t)s->geo.pitch * sizeof(double), host,
+                              s->p.nx * sizeof(double), s->p.nx * sizeof(double), s->p.ny,
+                              hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int fill_theta(cavlbm_solver* s, const double* theta) {
+  std::vector<double> h(s->p.ny, 0.5);
+  if (theta) std::copy(theta, theta + s->p.ny, h.begin());
+  HIP_TRY(s, hipMemcpyAsync(s->theta, h.data(), h.size() * sizeof(double),
+                            hipMemcpyHostToDevice, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+// Before any host-side modification: make the M-form current, and mark the
+// C-form stale afterwards.
+int begin_modify(cavlbm_solver* s) {
+  int rc = materialize(s);
+  if (rc) return rc;
+  s->c_valid = false;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cavlbm_abi_version(void) { return CAVLBM_ABI_VERSION; }
+
+int cavlbm_params_default(cavlbm_params* p) {
+  if (!p) return CAVLBM_E_PARAM;
+  std::memset(p, 0, sizeof(*p));
+  p->nx = 64;
+  p->ny = 256;
+  p->ra = 1.0e6;
+  p->pr = 0.71;
+  p->cs = 0.10;
+  p->pr_t = 0.9;
+  p->utarget = 0.02;
+  p->seed = 42;
+  p->jobs = 1;
+  p->alpha_mult = 1.0;
+  p->nu_start_mult = 1.0;
+  p->has_nu_override = 0;
+  p->nu_override = 0.0;
+  p->storage = CAVLBM_STORAGE_FP64;
+  p->geq_order = 2;
+  p->device = 0;
+  return 0;
+}
+
+int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
+  if (!pin || !out) return fail(nullptr, CAVLBM_E_PARAM, "null argument");
+  *out = nullptr;
+  const cavlbm_params p = *pin;
+  if (p.nx < 3 || p.ny < 3 || p.nx > (1u << 20) || p.ny > (1u << 20))
+    return fail(nullptr, CAVLBM_E_PARAM, "nx and ny must be in [3, 2^20]");
+  if (p.storage < 0 || p.storage > 2) return fail(nullptr, CAVLBM_E_PARAM, "bad storage");
+  if (p.geq_order != 1 && p.geq_order != 2)
+    return fail(nullptr, CAVLBM_E_PARAM, "geq_order must be 1 or 2");
+
+  // Lattice-unit calibration (src/uniform/solver.rs:123-154).
+  cavlbm_derived d{};
+  d.lz = (double)(p.ny - 1);
+  d.lx = (double)(p.nx - 1);
+  const double c_scale = 1.7;
+  const double nu_cal =
+      std::pow(p.utarget * std::sqrt(p.pr) * d.lz / (c_scale * std::sqrt(p.ra)), 2.0 / 3.0);
+  if (p.has_nu_override) {
+    if (!(p.nu_override > 0.0 && std::isfinite(p.nu_override)))
+      return fail(nullptr, CAVLBM_E_PARAM, "--nu must be > 0 and finite");
+    if (!(0.5 + 3.0 * p.nu_override > 0.5005))
+      return fail(nullptr, CAVLBM_E_PARAM, "--nu too low: tau+ close to 0.5");
+    d.nu = p.nu_override;
+  } else {
+    d.nu = std::fmax(nu_cal, 0.002);
+  }
+  d.alpha_phys = d.nu / p.pr;
+  d.alpha = d.alpha_phys * p.alpha_mult;
+  d.beta_eff = p.ra * d.nu * d.alpha_phys / (d.lz * d.lz * d.lz);
+  d.tau = 0.5 + 3.0 * d.nu;
+  d.tau_t = 0.5 + 3.0 * d.alpha;
+  if (!p.has_nu_override && !(d.tau > 0.505))
+    return fail(nullptr, CAVLBM_E_PARAM,
+                "tau=" + std::to_string(d.tau) + " too close to 0.5 (raise utarget)");
+  if (!(d.tau_t > 0.505))
+    return fail(nullptr, CAVLBM_E_PARAM,
+                "tau_t=" + std::to_string(d.tau_t) + " too close to 0.5 (adjust alpha_mult)");
+  d.cs = p.cs;
+  d.pr_t = p.pr_t;
+  d.nu_start_mult = std::fmax(p.nu_start_mult, 1.0);
+  d.t_ref = 0.0;
+  d.spin_up_steps = std::min<uint64_t>(std::max<uint64_t>(p.ny / 10, 200), 2000);
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(nullptr, CAVLBM_E_NODEVICE, "no HIP device visible");
+  if (p.device < 0 || p.device >= ndev) return fail(nullptr, CAVLBM_E_PARAM, "bad device ordinal");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p.device) != hipSuccess)
+    return fail(nullptr, CAVLBM_E_NODEVICE, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, CAVLBM_E_NODEVICE,
+                std::string("libcavlbm is built for gfx950, device is ") + prop.gcnArchName);
+  if (hipSetDevice(p.device) != hipSuccess)
+    return fail(nullptr, CAVLBM_E_HIP, "hipSetDevice failed");
+
+  auto* s = new cavlbm_solver();
+  s->p = p;
+  s->d = d;
+  s->storage = p.storage;
+  s->geq = p.geq_order;
+  s->st_bytes = p.storage == ST_FP64 ? 8 : (p.storage == ST_FP32 ? 4 : 2);
+  s->r_bytes = p.storage == ST_FP64 ? 8 : 4;
+  Geom& g = s->geo;
+  g.nx = (int)p.nx;
+  g.ny = (int)p.ny;
+  g.pitch = (int)((p.nx + 63) / 64 * 64);
+  g.nbx = (g.nx + kTileX - 1) / kTileX;
+  g.nby = (g.ny + kTileY - 1) / kTileY;
+  g.plane = (long long)g.pitch * g.ny;
+
+  auto fail_free = [&](const std::string& m) {
+    free_all(s);
+    delete s;
+    return fail(nullptr, CAVLBM_E_HIP, m);
+  };
+  const size_t P = (size_t)g.plane;
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+    return fail_free("hipStreamCreate failed");
+  bool ok = true;
+  ok &= hipMalloc(&s->cpop[0], 18 * P * s->st_bytes) == hipSuccess;
+  ok &= hipMalloc(&s->cpop[1], 18 * P * s->st_bytes) == hipSuccess;
+  ok &= hipMalloc((void**)&s->mpops, 18 * P * sizeof(double)) == hipSuccess;
+  for (auto& f : s->fld) ok &= hipMalloc((void**)&f, P * sizeof(double)) == hipSuccess;
+  ok &= hipMalloc((void**)&s->scratch_field, P * sizeof(double)) == hipSuccess;
+  ok &= hipMalloc(&s->fy, P * s->r_bytes) == hipSuccess;
+  ok &= hipMalloc(&s->nut, P * s->r_bytes) == hipSuccess;
+  ok &= hipMalloc((void**)&s->theta, p.ny * sizeof(double)) == hipSuccess;
+  ok &= hipMalloc((void**)&s->red_part, (size_t)kDiagSlots * diag_blocks(g) * sizeof(double)) ==
+        hipSuccess;
+  ok &= hipMalloc((void**)&s->red_out, kDiagSlots * sizeof(double)) == hipSuccess;
+  ok &= hipEventCreate(&s->ev0) == hipSuccess;
+  ok &= hipEventCreate(&s->ev1) == hipSuccess;
+  if (!ok) return fail_free("device allocation failed");
+  ok &= hipMemsetAsync(s->fy, 0, P * s->r_bytes, s->st) == hipSuccess;
+  ok &= hipMemsetAsync(s->nut, 0, P * s->r_bytes, s->st) == hipSuccess;
+  ok &= hipMemsetAsync(s->mpops, 0, 18 * P * sizeof(double), s->st) == hipSuccess;
+  for (auto& f : s->fld) ok &= hipMemsetAsync(f, 0, P * sizeof(double), s->st) == hipSuccess;
+  if (!ok) return fail_free("device memset failed");
+  if (fill_theta(s, nullptr)) return fail_free(s->err);
+
+  // Initial fields (src/uniform/solver.rs:160-176): T linear in x plus the
+  // xorshift64* noise drawn in (iy outer, ix inner) order, rho = 1, u = 0;
+  // the device then sets f = feq(1, 0, 0) and g = geq(T, 0, 0).
+  std::vector<double> t(p.nx * p.ny), one(p.nx * p.ny, 1.0), zero(p.nx * p.ny, 0.0);
+  uint64_t rng = p.seed | 1ULL;
+  for (uint64_t iy = 0; iy < p.ny; ++iy)
+    for (uint64_t ix = 0; ix < p.nx; ++ix) {
+      uint64_t x = rng;
+      x ^= x >> 12;
+      x ^= x << 25;
+      x ^= x >> 27;
+      rng = x;
+      const uint64_t r = x * 0x2545F4914F6CDD1DULL;
+      const double u01 = (double)(r >> 11) / (double)(1ULL << 53);
+      const double xf = (double)ix / (double)(p.nx - 1);
+      const double t_lin = 0.5 + (-0.5 - 0.5) * xf;
+      const double pert = 1.0e-3 * (2.0 * u01 - 1.0);
+      t[iy * p.nx + ix] = t_lin + pert;
+    }
+  if (field_up(s, one.data(), s->fld[0]) || field_up(s, zero.data(), s->fld[1]) ||
+      field_up(s, zero.data(), s->fld[2]) || field_up(s, t.data(), s->fld[3]))
+    return fail_free(s->err);
+  if (launch_init_eq(g, s->geq, s->mpops, s->fld[0], s->fld[1], s->fld[2], s->fld[3], s->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return fail_free("init kernel failed");
+  s->m_valid = true;
+  s->c_valid = false;
+  s->time_step = 0;
+  *out = s;
+  return 0;
+}
+
+int cavlbm_destroy(cavlbm_solver* s) {
+  if (!s) return 0;
+  (void)hipSetDevice(s->p.device);
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  free_all(s);
+  delete s;
+  return 0;
+}
+
+int cavlbm_step(cavlbm_solver* s) { return cavlbm_run(s, 1); }
+
+int cavlbm_run(cavlbm_solver* s, uint64_t nsteps) {
+  if (!s) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  return run_steps(s, nsteps);
+}
+
+int cavlbm_synchronize(cavlbm_solver* s) {
+  if (!s) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int cavlbm_init_from_fields(cavlbm_solver* s, const double* rho, const double* ux,
+                            const double* uy, const double* t, uint64_t n) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!rho || !ux || !uy || !t) return fail(s, CAVLBM_E_PARAM, "null field");
+  if (n != s->p.nx * s->p.ny)
+    return fail(s, CAVLBM_E_SIZE, "init_from_fields: expected " +
+                                      std::to_string(s->p.nx * s->p.ny) + " values, got " +
+                                      std::to_string(n));
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  if (field_up(s, rho, s->fld[0]) || field_up(s, ux, s->fld[1]) ||
+      field_up(s, uy, s->fld[2]) || field_up(s, t, s->fld[3]))
+    return CAVLBM_E_HIP;
+  HIP_TRY(s, launch_init_eq(s->geo, s->geq, s->mpops, s->fld[0], s->fld[1], s->fld[2],
+                            s->fld[3], s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  s->m_valid = true;
+  s->c_valid = false;
+  s->time_step = 0;
+  return 0;
+}
+
+int cavlbm_get_field(cavlbm_solver* s, int32_t field, double* out, uint64_t n) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!out) return fail(s, CAVLBM_E_PARAM, "null output");
+  if (n != s->p.nx * s->p.ny) return fail(s, CAVLBM_E_SIZE, "get_field: size mismatch");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  if (field >= CAVLBM_FIELD_RHO && field <= CAVLBM_FIELD_T) {
+    int rc = materialize(s);
+    if (rc) return rc;
+    return field_down(s, s->fld[field], out);
+  }
+  if (field == CAVLBM_FIELD_NU_T || field == CAVLBM_FIELD_FY_NODE) {
+    const void* src = field == CAVLBM_FIELD_NU_T ? s->nut : s->fy;
+    HIP_TRY(s, launch_field_convert(s->geo, s->storage, src, s->scratch_field, 1, nullptr,
+                                    nullptr, s->st));
+    return field_down(s, s->scratch_field, out);
+  }
+  return fail(s, CAVLBM_E_PARAM, "unknown field");
+}
+
+int cavlbm_set_field(cavlbm_solver* s, int32_t field, const double* in, uint64_t n) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!in) return fail(s, CAVLBM_E_PARAM, "null input");
+  if (n != s->p.nx * s->p.ny) return fail(s, CAVLBM_E_SIZE, "set_field: size mismatch");
+  if (field < CAVLBM_FIELD_RHO || field > CAVLBM_FIELD_FY_NODE)
+    return fail(s, CAVLBM_E_PARAM, "unknown field");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = begin_modify(s);
+  if (rc) return rc;
+  if (field <= CAVLBM_FIELD_T) return field_up(s, in, s->fld[field]);
+  if (field_up(s, in, s->scratch_field)) return CAVLBM_E_HIP;
+  void* dst = field == CAVLBM_FIELD_NU_T ? s->nut : s->fy;
+  HIP_TRY(s, launch_field_convert(s->geo, s->storage, nullptr, nullptr, 0, dst,
+                                  s->scratch_field, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int cavlbm_get_pops(cavlbm_solver* s, int32_t which, double* out, uint64_t n9) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!out) return fail(s, CAVLBM_E_PARAM, "null output");
+  if (which != CAVLBM_POPS_F && which != CAVLBM_POPS_G)
+    return fail(s, CAVLBM_E_PARAM, "which must be F or G");
+  const uint64_t n = s->p.nx * s->p.ny;
+  if (n9 != 9 * n) return fail(s, CAVLBM_E_SIZE, "get_pops: expected 9*nx*ny values");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = materialize(s);
+  if (rc) return rc;
+  std::vector<double> plane(n);
+  for (int i = 0; i < 9; ++i) {
+    rc = field_down(s, s->mpops + (size_t)(9 * which + i) * s->geo.plane, plane.data());
+    if (rc) return rc;
+    for (uint64_t k = 0; k < n; ++k) out[k * 9 + i] = plane[k];
+  }
+  return 0;
+}
+
+int cavlbm_set_pops(cavlbm_solver* s, int32_t which, const double* in, uint64_t n9) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!in) return fail(s, CAVLBM_E_PARAM, "null input");
+  if (which != CAVLBM_POPS_F && which != CAVLBM_POPS_G)
+    return fail(s, CAVLBM_E_PARAM, "which must be F or G");
+  const uint64_t n = s->p.nx * s->p.ny;
+  if (n9 != 9 * n) return fail(s, CAVLBM_E_SIZE, "set_pops: expected 9*nx*ny values");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = begin_modify(s);
+  if (rc) return rc;
+  std::vector<double> plane(n);
+  for (int i = 0; i < 9; ++i) {
+    for (uint64_t k = 0; k < n; ++k) plane[k] = in[k * 9 + i];
+    rc = field_up(s, plane.data(), s->mpops + (size_t)(9 * which + i) * s->geo.plane);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int cavlbm_diagnostics(cavlbm_solver* s, cavlbm_diag* out) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!out) return fail(s, CAVLBM_E_PARAM, "null output");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = materialize(s);
+  if (rc) return rc;
+  HIP_TRY(s, launch_diagnostics(s->geo, s->storage, s->mpops, s->fld[0], s->fld[1], s->fld[2],
+                                s->fld[3], s->nut, s->d.nu, s->red_part, s->red_out, s->st));
+  double r[kDiagSlots];
+  HIP_TRY(s, hipMemcpyAsync(r, s->red_out, sizeof(r), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  const double ny = (double)s->p.ny, lz = s->d.lz, dT = 1.0;
+  out->nusselt_hot = r[D_NU_HOT] / ny * lz / dT;
+  out->nusselt_cold = r[D_NU_COLD] / ny * lz / dT;
+  out->nusselt_hot_wall_link = r[D_NU_LINK] / ny * lz / dT;
+  const double conv = r[D_NU_VOL] / (double)((s->p.nx - 2) * (s->p.ny - 2));
+  out->nusselt_volume = 1.0 + conv * lz / dT;
+  out->total_mass = r[D_MASS];
+  out->total_f_mass = r[D_FMASS];
+  out->max_uy = r[D_MAX_UY];
+  out->max_u = r[D_MAX_U];
+  out->nu_t_max = r[D_NUT_MAX];
+  out->y_plus_hot = r[D_YPLUS] / (double)(s->p.ny - 2);
+  out->all_finite = r[D_NONFINITE] == 0.0 ? 1 : 0;
+  out->reserved = 0;
+  out->time_step = s->time_step;
+  return 0;
+}
+
+int cavlbm_max_uy_on_row(cavlbm_solver* s, uint64_t iy, double* w_max, double* x_norm) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (iy >= s->p.ny) return fail(s, CAVLBM_E_PARAM, "row out of range");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = materialize(s);
+  if (rc) return rc;
+  std::vector<double> row(s->p.nx);
+  HIP_TRY(s, hipMemcpyAsync(row.data(), s->fld[2] + (size_t)iy * s->geo.pitch,
+                            s->p.nx * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  double best = 0.0;
+  uint64_t best_ix = 1;
+  for (uint64_t ix = 1; ix + 1 < s->p.nx; ++ix) {
+    const double v = std::fabs(row[ix]);
+    if (v > best) {
+      best = v;
+      best_ix = ix;
+    }
+  }
+  if (w_max) *w_max = best;
+  if (x_norm) *x_norm = (double)best_ix / (double)(s->p.nx - 1);
+  return 0;
+}
+
+int cavlbm_derived_params(const cavlbm_solver* s, cavlbm_derived* out) {
+  if (!s || !out) return CAVLBM_E_PARAM;
+  *out = s->d;
+  return 0;
+}
+
+int cavlbm_time_step(const cavlbm_solver* s, uint64_t* out) {
+  if (!s || !out) return CAVLBM_E_PARAM;
+  *out = s->time_step;
+  return 0;
+}
+
+int cavlbm_set_time_step(cavlbm_solver* s, uint64_t t) {
+  if (!s) return CAVLBM_E_PARAM;
+  s->time_step = t;
+  return 0;
+}
+
+int cavlbm_set_hot_wall(cavlbm_solver* s, const double* theta_hot, uint64_t ny) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (theta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "theta_hot must have ny values");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = begin_modify(s);
+  if (rc) return rc;
+  return fill_theta(s, theta_hot);
+}
+
+int cavlbm_set_dynamic_sgs(cavlbm_solver* s, int32_t enable, double, double, double) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (enable)
+    return fail(s, CAVLBM_E_UNSUPPORTED,
+                "dynamic mixed-scale SGS (--dynamic-smag) is not on the GPU path");
+  return 0;
+}
+
+int cavlbm_set_jobs(uint64_t) { return 0; }
+
+int cavlbm_phase(cavlbm_solver* s, int32_t phase, uint64_t step) {
+  if (!s) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = begin_modify(s);
+  if (rc) return rc;
+  const Walls w{s->theta};
+  switch (phase) {
+    case CAVLBM_PHASE_COLLIDE:
+      HIP_TRY(s, launch(s, MODE_COLLIDE_M, make_ctx(s, step), 1));
+      break;
+    case CAVLBM_PHASE_STREAM:
+      if (!s->mtmp) HIP_TRY(s, hipMalloc((void**)&s->mtmp, 18 * (size_t)s->geo.plane * 8));
+      HIP_TRY(s, launch_phase_stream(s->geo, s->mpops, s->mtmp, s->st));
+      std::swap(s->mpops, s->mtmp);
+      break;
+    case CAVLBM_PHASE_THERMAL_BC:
+      HIP_TRY(s, launch_phase_thermal_bc(s->geo, s->geq, w, s->mpops, s->st));
+      break;
+    case CAVLBM_PHASE_UPDATE_MACROSCOPIC:
+      HIP_TRY(s, launch_phase_macro(s->geo, s->storage, s->mpops, s->fy, s->fld[0], s->fld[1],
+                                    s->fld[2], s->fld[3], s->st));
+      break;
+    case CAVLBM_PHASE_ENFORCE_WALL_VELOCITY:
+      HIP_TRY(s, launch_phase_walls(s->geo, s->fld[1], s->fld[2], s->st));
+      break;
+    case CAVLBM_PHASE_BOUNCE_BACK:
+      HIP_TRY(s, launch_phase_bounce_back(s->geo, s->mpops, s->st));
+      break;
+    default:
+      return fail(s, CAVLBM_E_PARAM, "unknown phase");
+  }
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int cavlbm_time_run(cavlbm_solver* s, uint64_t nsteps, double* ms) {
+  if (!s || !ms) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  HIP_TRY(s, hipEventRecord(s->ev0, s->st));
+  int rc = run_steps(s, nsteps);
+  if (rc) return rc;
+  HIP_TRY(s, hipEventRecord(s->ev1, s->st));
+  HIP_TRY(s, hipEventSynchronize(s->ev1));
+  float t = 0.f;
+  HIP_TRY(s, hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms = t;
+  return 0;
+}
+
+int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms) {
+  if (!s || !mean_ms || launches == 0) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  if (!s->c_valid) {
+    int rc = run_steps(s, 1);
+    if (rc) return rc;
+  }
+  double total = 0.0;
+  for (uint64_t i = 0; i < launches; ++i) {
+    HIP_TRY(s, hipEventRecord(s->ev0, s->st));
+    int rc = one_step(s, i + 1 == launches ? 1 : 0);
+    if (rc) return rc;
+    HIP_TRY(s, hipEventRecord(s->ev1, s->st));
+    HIP_TRY(s, hipEventSynchronize(s->ev1));
+    float t = 0.f;
+    HIP_TRY(s, hipEventElapsedTime(&t, s->ev0, s->ev1));
+    total += t;
+  }
+  *mean_ms = total / (double)launches;
+  return 0;
+}
+
+const char* cavlbm_last_error(const cavlbm_solver* s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+}  // extern "C"

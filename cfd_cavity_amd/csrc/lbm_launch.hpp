@@ -1,0 +1,53 @@
+// lbm_launch.hpp — host-side launch entry points of the cavity kernels
+// (internal to libcavlbm; the public boundary is include/cavlbm.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_device.hpp"
+
+namespace cavlbm {
+
+// One step-kernel launch: storage S, g-equilibrium order geq (1 or 2), mode.
+hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& ctx,
+                            const Walls& walls, const StepBufs& b, int write_aux,
+                            hipStream_t st);
+hipError_t launch_step_lowp(int storage, int geq, int mode, const Geom& geo,
+                            const StepCtx& ctx, const Walls& walls, const StepBufs& b,
+                            int write_aux, hipStream_t st);
+
+// f = feq / g = geq from the M-form fields, then u = 0 on the walls.
+hipError_t launch_init_eq(const Geom& geo, int geq, double* mpops, const double* rho,
+                          double* ux, double* uy, const double* temp, hipStream_t st);
+
+// Phase kernels on the M-form state (FP64), see lbm_aux.hip.
+hipError_t launch_phase_stream(const Geom& geo, const double* src, double* dst,
+                               hipStream_t st);
+hipError_t launch_phase_thermal_bc(const Geom& geo, int geq, const Walls& walls,
+                                   double* mpops, hipStream_t st);
+hipError_t launch_phase_macro(const Geom& geo, int storage, const double* mpops,
+                              const void* fy, double* rho, double* ux, double* uy,
+                              double* temp, hipStream_t st);
+hipError_t launch_phase_walls(const Geom& geo, double* ux, double* uy, hipStream_t st);
+hipError_t launch_phase_bounce_back(const Geom& geo, double* mpops, hipStream_t st);
+
+// Diagnostics: FP64 reductions over the M-form fields.  `partials` holds
+// kDiagSlots*nblocks doubles, `out` kDiagSlots doubles.
+constexpr int kDiagSlots = 12;
+enum DiagSlot {
+  D_MASS = 0, D_FMASS, D_NU_HOT, D_NU_COLD, D_NU_LINK, D_NU_VOL, D_YPLUS,
+  D_MAX_UY, D_MAX_U, D_NUT_MAX, D_NONFINITE, D_UNUSED
+};
+int diag_blocks(const Geom& geo);
+hipError_t launch_diagnostics(const Geom& geo, int storage, const double* mpops,
+                              const double* rho, const double* ux, const double* uy,
+                              const double* temp, const void* nut, double nu,
+                              double* partials, double* out, hipStream_t st);
+// Type conversion of an R-typed per-node field (fy_node / nu_t) to/from FP64.
+hipError_t launch_field_convert(const Geom& geo, int storage, const void* src_r,
+                                double* dst_d, int to_double, void* dst_r,
+                                const double* src_d, hipStream_t st);
+
+inline int nblocks_of(const Geom& g) { return g.nbx * g.nby; }
+
+}  // namespace cavlbm

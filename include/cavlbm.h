@@ -1,0 +1,186 @@
+/*
+ * cavlbm.h — C-ABI of the MI355X (gfx950) thermal LBM cavity time-stepper.
+ *
+ * Drop-in boundary for the reference's uniform-grid solver type
+ * `crate::uniform::solver::CavitySolver` (bbrangeo/cfd_cavity,
+ * src/uniform/solver.rs:35-104) and its diagnostics (src/uniform/post.rs).
+ * The reference has no FFI of its own; each entry point below names the Rust
+ * item it replaces.  Conventions:
+ *   - every call returns 0 on success or a negative CAVLBM_E_* code; the
+ *     message is available from cavlbm_last_error(handle) (or
+ *     cavlbm_last_error(NULL) for a failed cavlbm_create);
+ *   - host arrays are copied in or out, never retained;
+ *   - a handle is not re-entrant (like `&mut self`); separate handles are
+ *     independent and each owns its own HIP stream;
+ *   - per-node arrays use k = iy*nx + ix (solver.rs:162); population arrays
+ *     are k-major [k][9] in the direction order given below;
+ *   - non-finite values are reported by cavlbm_diagnostics, never an error.
+ *
+ * Direction order: i = 0..8 with c_i = (0,0) (1,0) (1,1) (0,1) (-1,1) (-1,0)
+ * (-1,-1) (0,-1) (1,-1); opp(i) = i+4 for i = 1..4.
+ */
+#ifndef CAVLBM_H
+#define CAVLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAVLBM_ABI_VERSION 1
+
+enum {
+  CAVLBM_OK = 0,
+  CAVLBM_E_PARAM = -1,       /* bad argument; the reference assert!s (solver.rs:130-154) */
+  CAVLBM_E_SIZE = -2,        /* array length mismatch (solver.rs:236-240) */
+  CAVLBM_E_HIP = -3,         /* HIP runtime failure */
+  CAVLBM_E_UNSUPPORTED = -4, /* feature not on this path (dynamic SGS) */
+  CAVLBM_E_NODEVICE = -5     /* no usable gfx950 device */
+};
+
+/* Population storage in HBM.  Arithmetic is FP64 for FP64 storage and FP32
+ * otherwise; reductions are always FP64. */
+enum {
+  CAVLBM_STORAGE_FP64 = 0,
+  CAVLBM_STORAGE_FP32 = 1,
+  CAVLBM_STORAGE_FP16_OFFSET = 2 /* f stored as half(f - w_i), g as half(g) */
+};
+
+enum { CAVLBM_FIELD_RHO = 0, CAVLBM_FIELD_UX = 1, CAVLBM_FIELD_UY = 2,
+       CAVLBM_FIELD_T = 3, CAVLBM_FIELD_NU_T = 4, CAVLBM_FIELD_FY_NODE = 5 };
+enum { CAVLBM_POPS_F = 0, CAVLBM_POPS_G = 1 };
+
+/* Phase-split entry points, the crate-private phases of step()
+ * (solver.rs:287-291) and the test-only apply_bounce_back (solver.rs:920). */
+enum {
+  CAVLBM_PHASE_COLLIDE = 0,
+  CAVLBM_PHASE_STREAM = 1,
+  CAVLBM_PHASE_THERMAL_BC = 2,
+  CAVLBM_PHASE_UPDATE_MACROSCOPIC = 3,
+  CAVLBM_PHASE_ENFORCE_WALL_VELOCITY = 4,
+  CAVLBM_PHASE_BOUNCE_BACK = 5
+};
+
+typedef struct cavlbm_solver cavlbm_solver;
+
+/* The twelve arguments of CavitySolver::new (solver.rs:108) plus the device
+ * options.  Fill with cavlbm_params_default() first. */
+typedef struct {
+  uint64_t nx, ny;
+  double ra, pr, cs, pr_t, utarget;
+  uint64_t seed;
+  uint64_t jobs; /* recorded for parity with the reference; the GPU ignores it */
+  double alpha_mult, nu_start_mult;
+  int32_t has_nu_override; /* Option<f64> nu_override */
+  int32_t storage;         /* CAVLBM_STORAGE_* */
+  double nu_override;
+  int32_t geq_order;       /* 2: quadratic g equilibrium (default), 1: linear */
+  int32_t device;          /* HIP device ordinal */
+} cavlbm_params;
+
+/* Values of post::* and main.rs's progress diagnostics at the current state. */
+typedef struct {
+  double nusselt_hot;           /* post.rs:68-78 */
+  double nusselt_cold;          /* post.rs:100-111 */
+  double nusselt_hot_wall_link; /* post.rs:86-96 */
+  double nusselt_volume;        /* post.rs:116-132 */
+  double total_mass;            /* solver.rs:1012-1014 */
+  double total_f_mass;          /* solver.rs:1018-1020 */
+  double max_uy;                /* post.rs:153-155 */
+  double max_u;                 /* main.rs:192-196 */
+  double nu_t_max;              /* main.rs:170-173 */
+  double y_plus_hot;            /* post.rs:190-199 */
+  int32_t all_finite;           /* post.rs:406-411 */
+  int32_t reserved;
+  uint64_t time_step;           /* completed steps */
+} cavlbm_diag;
+
+/* Lattice-unit calibration of CavitySolver::new (solver.rs:123-156, :194). */
+typedef struct {
+  double lz, lx, nu, alpha, alpha_phys, beta_eff, tau, tau_t, cs, pr_t,
+      nu_start_mult, t_ref;
+  uint64_t spin_up_steps;
+} cavlbm_derived;
+
+/* Defaults of src/cli.rs:94-130 (64x256, Ra 1e6, Pr 0.71, Cs 0.10, Pr_t 0.9,
+ * utarget 0.02, seed 42, jobs 1) with FP64 storage on device 0. */
+int cavlbm_params_default(cavlbm_params *p);
+
+/* CavitySolver::new (solver.rs:108-210): calibration, asserts (as
+ * CAVLBM_E_PARAM), linear-T + xorshift64* noise init, f = feq, g = geq. */
+int cavlbm_create(const cavlbm_params *p, cavlbm_solver **out);
+/* Drop of CavitySolver. */
+int cavlbm_destroy(cavlbm_solver *s);
+
+/* CavitySolver::step (solver.rs:279-293). */
+int cavlbm_step(cavlbm_solver *s);
+/* n consecutive step() calls (the loop of main.rs:164-165 without I/O). */
+int cavlbm_run(cavlbm_solver *s, uint64_t nsteps);
+/* Blocks until queued work on the handle's stream has finished. */
+int cavlbm_synchronize(cavlbm_solver *s);
+
+/* CavitySolver::init_from_fields (solver.rs:232-262): rho, ux, uy, T of n
+ * = nx*ny values each; f = feq, g = geq, wall u = 0, time_step = 0. */
+int cavlbm_init_from_fields(cavlbm_solver *s, const double *rho,
+                            const double *ux, const double *uy,
+                            const double *t, uint64_t n);
+
+/* Read or overwrite one per-node field (the public fields lbm.rho, lbm.ux,
+ * lbm.uy, lbm.temperature, nu_t and the crate-private fy_node). */
+int cavlbm_get_field(cavlbm_solver *s, int32_t field, double *out, uint64_t n);
+int cavlbm_set_field(cavlbm_solver *s, int32_t field, const double *in,
+                     uint64_t n);
+/* Read or overwrite lbm.f / lbm.g as k-major [k][9] doubles (n9 = 9*nx*ny). */
+int cavlbm_get_pops(cavlbm_solver *s, int32_t which, double *out, uint64_t n9);
+int cavlbm_set_pops(cavlbm_solver *s, int32_t which, const double *in,
+                    uint64_t n9);
+
+/* post::nusselt_*, total_mass, total_f_mass, max_uy, y_plus_hot,
+ * check_finite and the progress-line maxima, computed on the device in FP64. */
+int cavlbm_diagnostics(cavlbm_solver *s, cavlbm_diag *out);
+/* post::max_uy_on_row (post.rs:137-150). */
+int cavlbm_max_uy_on_row(cavlbm_solver *s, uint64_t iy, double *w_max,
+                         double *x_norm);
+/* Public scalars of CavitySolver (lz, lx, beta_eff, nu, alpha, ...). */
+int cavlbm_derived_params(const cavlbm_solver *s, cavlbm_derived *out);
+/* CavitySolver::time_step (solver.rs:67). */
+int cavlbm_time_step(const cavlbm_solver *s, uint64_t *out);
+int cavlbm_set_time_step(cavlbm_solver *s, uint64_t t);
+
+/* Hot-wall temperature profile theta_hot[iy] (ny values) used by the
+ * isothermal condition at ix = 0 (solver.rs:957-963) in place of T_HOT;
+ * NULL restores the constant +0.5. */
+int cavlbm_set_hot_wall(cavlbm_solver *s, const double *theta_hot, uint64_t ny);
+
+/* CavitySolver::set_dynamic_sgs (solver.rs:215-220): enable = 0 is accepted,
+ * enable != 0 returns CAVLBM_E_UNSUPPORTED (dynamic SGS is not on this path). */
+int cavlbm_set_dynamic_sgs(cavlbm_solver *s, int32_t enable, double smag_alpha,
+                           double nu_sm_min, double eta_test);
+/* CavitySolver::set_jobs (solver.rs:373-383): accepted and ignored. */
+int cavlbm_set_jobs(uint64_t jobs);
+
+/* One crate-private phase: collide(step) (solver.rs:421), stream (:838),
+ * apply_thermal_boundary_conditions (:951), update_macroscopic (:298),
+ * enforce_wall_velocity (:992), apply_bounce_back (:920).  `step` is used by
+ * CAVLBM_PHASE_COLLIDE only. */
+int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
+
+/* Timing helpers used by the benchmark (no reference counterpart): run
+ * nsteps on the handle's stream between HIP events and return the elapsed
+ * milliseconds; cavlbm_time_step_kernel brackets each of `launches` fused
+ * step kernels with its own event pair and returns the mean kernel time. */
+int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
+int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
+
+/* Message of the last failure on this handle (or of cavlbm_create when s is
+ * NULL).  Valid until the next call on the same handle / thread. */
+const char *cavlbm_last_error(const cavlbm_solver *s);
+/* CAVLBM_ABI_VERSION of the loaded library. */
+int cavlbm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

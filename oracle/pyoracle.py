@@ -1,0 +1,260 @@
+"""ctypes binding of the CPU oracle (oracle/libcavity_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+``bench.py``'s cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product package (``cfd_cavity_amd``) never imports it.
+
+The class mirrors the reference's ``CavitySolver`` (src/uniform/solver.rs:35-104,
+:108-293) with numpy views onto the oracle's own buffers: ``f``/``g`` are
+``(ny*nx, 9)`` AoS arrays, the fields are ``(ny*nx,)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcavity_oracle.so")
+_lib = None
+
+RHO, UX, UY, T, NU_T, FY = range(6)
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc -O3 -ffp-contract=off)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
+    L = C.CDLL(_LIB_PATH)
+    vp, sz, u64, d, i = C.c_void_p, C.c_size_t, C.c_uint64, C.c_double, C.c_int
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "orc_new": (vp, [sz, sz, d, d, d, d, d, u64, sz, d, d, i, d, i, C.c_char_p, sz]),
+        "orc_free": (None, [vp]),
+        "orc_set_jobs": (None, [vp, sz]),
+        "orc_step": (None, [vp]),
+        "orc_run": (None, [vp, u64]),
+        "orc_collide": (None, [vp, u64]),
+        "orc_stream": (None, [vp]),
+        "orc_apply_bounce_back": (None, [vp]),
+        "orc_thermal_bc": (None, [vp]),
+        "orc_update_macroscopic": (None, [vp]),
+        "orc_enforce_wall_velocity": (None, [vp]),
+        "orc_init_from_fields": (i, [vp, dp, dp, dp, dp, sz]),
+        "orc_set_hot_wall": (None, [vp, dp]),
+        "orc_field": (dp, [vp, i]),
+        "orc_pops": (dp, [vp, i]),
+        "orc_nx": (sz, [vp]),
+        "orc_ny": (sz, [vp]),
+        "orc_time_step": (u64, [vp]),
+        "orc_set_time_step": (None, [vp, u64]),
+        "orc_derived": (None, [vp, dp]),
+        "orc_step_scalars": (None, [vp, u64, dp]),
+        "orc_total_mass": (d, [vp]),
+        "orc_total_f_mass": (d, [vp]),
+        "orc_nusselt_hot": (d, [vp]),
+        "orc_nusselt_hot_wall_link": (d, [vp]),
+        "orc_nusselt_cold": (d, [vp]),
+        "orc_nusselt_volume": (d, [vp]),
+        "orc_max_uy": (d, [vp]),
+        "orc_max_uy_on_row": (d, [vp, sz, dp]),
+        "orc_max_u": (d, [vp]),
+        "orc_nu_t_max": (d, [vp]),
+        "orc_wall_shear_hot": (d, [vp, sz]),
+        "orc_y_plus_hot": (d, [vp]),
+        "orc_check_finite": (i, [vp]),
+        "orc_feq": (d, [i, d, d, d]),
+        "orc_geq": (d, [i, d, d, d, i]),
+        "orc_lattice": (None, [C.POINTER(C.c_int)] * 3 + [dp]),
+        "orc_collide_node": (None, [dp, dp, dp, dp] + [d] * 11 + [i]),
+        "orc_poiseuille_channel": (d, [sz, sz, d, d, u64]),
+        "orc_rng_next_u64": (u64, [C.POINTER(C.c_uint64)]),
+        "orc_rng_next_f64": (d, [C.POINTER(C.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lattice():
+    L = lib()
+    cx = (C.c_int * 9)()
+    cy = (C.c_int * 9)()
+    opp = (C.c_int * 9)()
+    w = np.zeros(9)
+    L.orc_lattice(cx, cy, opp, _dp(w))
+    return np.array(cx[:]), np.array(cy[:]), np.array(opp[:]), w
+
+
+def feq(i, rho, ux, uy):
+    return lib().orc_feq(i, rho, ux, uy)
+
+
+def geq(i, t, ux, uy, order=2):
+    return lib().orc_geq(i, t, ux, uy, order)
+
+
+class OracleSolver:
+    """The reference ``CavitySolver`` restated on the CPU (see module doc)."""
+
+    def __init__(self, nx, ny, ra, pr, cs, pr_t, utarget, seed, jobs=1,
+                 alpha_mult=1.0, nu_start_mult=1.0, nu_override=None, geq_order=2):
+        L = lib()
+        err = C.create_string_buffer(512)
+        has = 0 if nu_override is None else 1
+        h = L.orc_new(nx, ny, ra, pr, cs, pr_t, utarget, seed, jobs, alpha_mult,
+                      nu_start_mult, has, 0.0 if nu_override is None else nu_override,
+                      geq_order, err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h = h
+        self._L = L
+        self.nx, self.ny = nx, ny
+        n = nx * ny
+        self._n = n
+
+        def view(ptr, shape):
+            return np.ctypeslib.as_array(ptr, shape=shape)
+
+        self.rho = view(L.orc_field(h, RHO), (n,))
+        self.ux = view(L.orc_field(h, UX), (n,))
+        self.uy = view(L.orc_field(h, UY), (n,))
+        self.temperature = view(L.orc_field(h, T), (n,))
+        self.nu_t = view(L.orc_field(h, NU_T), (n,))
+        self.fy_node = view(L.orc_field(h, FY), (n,))
+        self.f = view(L.orc_pops(h, 0), (n, 9))
+        self.g = view(L.orc_pops(h, 1), (n, 9))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.orc_free(h)
+            self._h = None
+
+    # reference API (src/uniform/solver.rs)
+    def idx(self, ix, iy):
+        return iy * self.nx + ix
+
+    def step(self):
+        self._L.orc_step(self._h)
+
+    def run(self, n):
+        self._L.orc_run(self._h, n)
+
+    def set_jobs(self, jobs):
+        self._L.orc_set_jobs(self._h, jobs)
+
+    def collide(self, step):
+        self._L.orc_collide(self._h, step)
+
+    def stream(self):
+        self._L.orc_stream(self._h)
+
+    def apply_bounce_back(self):
+        self._L.orc_apply_bounce_back(self._h)
+
+    def apply_thermal_boundary_conditions(self):
+        self._L.orc_thermal_bc(self._h)
+
+    def update_macroscopic(self):
+        self._L.orc_update_macroscopic(self._h)
+
+    def enforce_wall_velocity(self):
+        self._L.orc_enforce_wall_velocity(self._h)
+
+    def init_from_fields(self, rho, ux, uy, t):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, t)]
+        if any(a.size != self._n for a in arrs):
+            raise ValueError("init_from_fields: field size mismatch")
+        self._L.orc_init_from_fields(self._h, *[_dp(a) for a in arrs], self._n)
+
+    def set_hot_wall(self, theta):
+        if theta is None:
+            self._L.orc_set_hot_wall(self._h, None)
+        else:
+            a = np.ascontiguousarray(theta, dtype=np.float64)
+            assert a.size == self.ny
+            self._L.orc_set_hot_wall(self._h, _dp(a))
+
+    @property
+    def time_step(self):
+        return self._L.orc_time_step(self._h)
+
+    @time_step.setter
+    def time_step(self, v):
+        self._L.orc_set_time_step(self._h, v)
+
+    def derived(self):
+        out = np.zeros(13)
+        self._L.orc_derived(self._h, _dp(out))
+        keys = ["lz", "lx", "nu", "alpha", "alpha_phys", "beta_eff", "tau", "tau_t",
+                "spin_up_steps", "cs", "pr_t", "nu_start_mult", "t_ref"]
+        return dict(zip(keys, out.tolist()))
+
+    def step_scalars(self, step):
+        out = np.zeros(4)
+        self._L.orc_step_scalars(self._h, step, _dp(out))
+        return dict(zip(["beta", "nu_eff", "alpha_eff", "tau"], out.tolist()))
+
+    def total_mass(self):
+        return self._L.orc_total_mass(self._h)
+
+    def total_f_mass(self):
+        return self._L.orc_total_f_mass(self._h)
+
+    def diagnostics(self):
+        L, h = self._L, self._h
+        return {
+            "nusselt_hot": L.orc_nusselt_hot(h),
+            "nusselt_cold": L.orc_nusselt_cold(h),
+            "nusselt_hot_wall_link": L.orc_nusselt_hot_wall_link(h),
+            "nusselt_volume": L.orc_nusselt_volume(h),
+            "total_mass": L.orc_total_mass(h),
+            "total_f_mass": L.orc_total_f_mass(h),
+            "max_uy": L.orc_max_uy(h),
+            "max_u": L.orc_max_u(h),
+            "nu_t_max": L.orc_nu_t_max(h),
+            "y_plus_hot": L.orc_y_plus_hot(h),
+            "all_finite": bool(L.orc_check_finite(h)),
+            "time_step": L.orc_time_step(h),
+        }
+
+    def max_uy_on_row(self, iy):
+        x = C.c_double(0.0)
+        v = self._L.orc_max_uy_on_row(self._h, iy, C.byref(x))
+        return v, x.value
+
+    def wall_shear_hot(self, iy):
+        return self._L.orc_wall_shear_hot(self._h, iy)
+
+
+def collide_node(f, g, rho, ux, uy, temp, tau_mol, beta, t_ref, cs, pr_t, nu, alpha,
+                 geq_order=2):
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    g = np.ascontiguousarray(g, dtype=np.float64).copy()
+    nut = np.zeros(1)
+    fy = np.zeros(1)
+    lib().orc_collide_node(_dp(f), _dp(g), _dp(nut), _dp(fy), rho, ux, uy, temp, tau_mol,
+                           beta, t_ref, cs, pr_t, nu, alpha, geq_order)
+    return f, g, nut[0], fy[0]
+
+
+def poiseuille_channel(nx, ny, nu, fy, nsteps):
+    return lib().orc_poiseuille_channel(nx, ny, nu, fy, nsteps)
