@@ -1,0 +1,75 @@
+"""Golden fixtures (tests/golden/, written by make_golden.py from the oracle):
+the oracle must reproduce them bit for bit (CPU), and the HIP path must match
+them bit for bit (GPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden.make_golden import CASES, FIELDS, state_digest
+from tests.helpers import bits_equal
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_small():
+    return np.load(os.path.join(HERE, "oracle_16x40_210.npz"), allow_pickle=False)
+
+
+def load_digests():
+    with open(os.path.join(HERE, "digests.json")) as fh:
+        return json.load(fh)
+
+
+def test_oracle_reproduces_small_fixture(oracle):
+    z = load_small()
+    s = oracle.OracleSolver(16, 40, 1.0e5, 0.71, 0.1, 0.9, 0.02, 42, 1)
+    s.run(210)
+    assert bits_equal(s.f, z["f"]) and bits_equal(s.g, z["g"])
+    for n in FIELDS:
+        assert bits_equal(getattr(s, n), z[n]), n
+
+
+def test_oracle_reproduces_digests(oracle):
+    dig = load_digests()
+    name = "64x256_ra1e6_1000"
+    args, steps = CASES[name]
+    o = oracle.OracleSolver(*args, 8)
+    o.run(steps)
+    assert state_digest(o) == dig[name]["sha256"]
+
+
+@pytest.mark.gpu
+def test_gpu_matches_small_fixture():
+    from cfd_cavity_amd import CavitySolver
+
+    z = load_small()
+    s = CavitySolver(16, 40, 1.0e5, 0.71, 0.1, 0.9, 0.02, 42, 1)
+    s.run(210)
+    assert bits_equal(s.get_pops("f"), z["f"]) and bits_equal(s.get_pops("g"), z["g"])
+    for n in FIELDS:
+        assert bits_equal(s.get_field(n), z[n]), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_matches_digests(name):
+    from cfd_cavity_amd import CavitySolver
+
+    dig = load_digests()[name]
+    args, steps = CASES[name]
+    s = CavitySolver(*args, 1)
+    s.run(steps)
+
+    class View:
+        pass
+
+    v = View()
+    v.f, v.g = s.get_pops("f"), s.get_pops("g")
+    for n in FIELDS:
+        setattr(v, n, s.get_field(n))
+    assert state_digest(v) == dig["sha256"]
+    d = s.diagnostics()
+    for k in ("nusselt_hot", "nusselt_cold", "total_mass", "y_plus_hot"):
+        assert abs(d[k] - dig["diagnostics"][k]) <= 1e-12 * abs(dig["diagnostics"][k])
