@@ -98,6 +98,7 @@ StepCtx make_ctx(const cavlbm_solver* s, uint64_t step) {
   c.t_ref = d.t_ref;
   c.cs = d.cs;
   c.pr_t = d.pr_t;
+  c.inv_pr_t = 1.0 / d.pr_t;
   return c;
 }
 
@@ -357,6 +358,10 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   g.nbx = (g.nx + kTileX - 1) / kTileX;
   g.nby = (g.ny + kTileY - 1) / kTileY;
   g.plane = (long long)g.pitch * g.ny;
+  if (18 * g.plane >= (1LL << 31)) {
+    delete s;
+    return fail(nullptr, CAVLBM_E_PARAM, "lattice too large: 18*pitch*ny must be < 2^31");
+  }
 
   auto fail_free = [&](const std::string& m) {
     free_all(s);

@@ -30,6 +30,18 @@ constexpr double kTHot = 0.5;
 constexpr double kTCold = -0.5;
 constexpr double kF64Eps = 2.220446049250313e-16;
 
+// Divisors that appear with a constant denominator, evaluated in the same
+// order as the reference's expressions, and their correctly rounded
+// reciprocals (compile-time IEEE division).
+constexpr double kCs2 = 1.0 / 3.0;
+constexpr double kDenEu2 = 2.0 * kCs2 * kCs2;  // eu*eu / (2 cs2 cs2)
+constexpr double kDenU2 = 2.0 * kCs2;          // u2 / (2 cs2)
+constexpr double kDenCs4 = kCs2 * kCs2;        // ei_u*cy / (cs2*cs2)
+constexpr double kInvCs2 = 1.0 / kCs2;
+constexpr double kInvDenEu2 = 1.0 / kDenEu2;
+constexpr double kInvDenU2 = 1.0 / kDenU2;
+constexpr double kInvDenCs4 = 1.0 / kDenCs4;
+
 enum Storage { ST_FP64 = 0, ST_FP32 = 1, ST_FP16_OFFSET = 2 };
 enum Mode {
   MODE_FUSED = 0,        // C-form -> stream+BC+moments+walls -> collide -> C-form
@@ -41,13 +53,14 @@ enum Mode {
 struct Geom {
   int nx, ny, pitch;
   int nbx, nby;       // tile grid: tiles of kTileX x kTileY nodes
-  long long plane;    // elements per population plane (pitch*ny)
+  long long plane;    // elements per population plane (18*plane < 2^31)
 };
 
 // Per-step scalars computed on the host exactly as collide() does
-// (solver.rs:433-454); t_ref/cs/pr_t are constants of the solver.
+// (solver.rs:433-454); t_ref/cs/pr_t are constants of the solver and
+// inv_pr_t = 1.0 / pr_t (IEEE, host).
 struct StepCtx {
-  double beta, nu, alpha, tau, t_ref, cs, pr_t;
+  double beta, nu, alpha, tau, t_ref, cs, pr_t, inv_pr_t;
 };
 
 struct Walls {
@@ -71,61 +84,131 @@ constexpr int kTileY = 4;
 constexpr int kBlock = kTileX * kTileY;
 
 // ---------------------------------------------------------------------------
-// storage traits
+// Arithmetic policy.
+//
+// FP64: a / b with a known y = RN(1/b) is evaluated as q0 = a*y followed by
+// two Markstein corrections r = fma(-b, q, a), q = fma(r, y, q).  After the
+// first correction q is a faithful quotient, and Markstein's theorem (y the
+// correctly rounded reciprocal, q within one ulp) makes the second one the
+// correctly rounded a/b; the result equals IEEE division for all normal
+// operands (a zero dividend may return +0 where division gives -0; every use
+// below adds that value to a nonzero term or to +0).  Five FMA-class
+// instructions replace the generic division sequence.
+//
+// FP32 (precision forms): reciprocal multiplication and fast division.
+template <class R> struct Arith;
+template <> struct Arith<double> {
+  __device__ static __forceinline__ double divr(double a, double b, double y) {
+    double q = a * y;
+    double r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, y, q);
+    r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+  }
+  __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+  __device__ static __forceinline__ double rcp(double b) { return 1.0 / b; }
+};
+template <> struct Arith<float> {
+  __device__ static __forceinline__ float divr(float a, float, float y) { return a * y; }
+  __device__ static __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
+  __device__ static __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+};
+
+// ---------------------------------------------------------------------------
+// storage traits; offsets are element offsets within one plane.
 template <int S> struct StorageT;
 template <> struct StorageT<ST_FP64> {
   using T = double;
   using R = double;
-  __device__ static __forceinline__ R load_f(const T* p, long long o, int) { return p[o]; }
-  __device__ static __forceinline__ R load_g(const T* p, long long o) { return p[o]; }
-  __device__ static __forceinline__ void store_f(T* p, long long o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, long long o, R v) { p[o] = v; }
+  __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return p[o]; }
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = v; }
 };
 template <> struct StorageT<ST_FP32> {
   using T = float;
   using R = float;
-  __device__ static __forceinline__ R load_f(const T* p, long long o, int) { return p[o]; }
-  __device__ static __forceinline__ R load_g(const T* p, long long o) { return p[o]; }
-  __device__ static __forceinline__ void store_f(T* p, long long o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, long long o, R v) { p[o] = v; }
+  __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return p[o]; }
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = v; }
 };
 // FP16 storage of f - w_i (the rest-state offset keeps ~11 significant bits
 // on the O(1e-2) deviations) and of g itself (its rest value w_i*0 is zero).
 template <> struct StorageT<ST_FP16_OFFSET> {
   using T = __half;
   using R = float;
-  __device__ static __forceinline__ R load_f(const T* p, long long o, int i) {
+  __device__ static __forceinline__ R load_f(const T* p, unsigned o, int i) {
     return __half2float(p[o]) + (float)kW[i];
   }
-  __device__ static __forceinline__ R load_g(const T* p, long long o) { return __half2float(p[o]); }
-  __device__ static __forceinline__ void store_f(T* p, long long o, int i, R v) {
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return __half2float(p[o]); }
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int i, R v) {
     p[o] = __float2half_rn(v - (float)kW[i]);
   }
-  __device__ static __forceinline__ void store_g(T* p, long long o, R v) { p[o] = __float2half_rn(v); }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = __float2half_rn(v); }
 };
 
 // ---------------------------------------------------------------------------
-// equilibria (the crate's feq/geq; feq form as written in
-// src/uniform/tests.rs:314-316)
+// Equilibria (the crate's feq/geq; the feq form is the one written out in
+// src/uniform/tests.rs:314-316):
+//   w * s * (1 + eu/cs2 + eu*eu/(2 cs2 cs2) - u2/(2 cs2)),  eu = cx*ux + cy*uy.
+// EqTerms holds the velocity-only parts shared by f, g and all directions.
+template <class R>
+struct EqTerms {
+  R q1[9];  // eu/cs2
+  R q2[9];  // eu*eu/(2 cs2 cs2)
+  R t;      // u2/(2 cs2)
+};
+
+// The direction products c_i.u are formed exactly as the reference's
+// cx*ux + cy*uy up to the sign of zero: multiplying by +-1 is exact, and a
+// zero product only changes the sign of an exact zero, which every use below
+// absorbs (1 + q1, q2 = eu*eu, or an added nonzero term).  Opposite
+// directions give exactly negated dividends, and RN is sign-symmetric, so
+// their quotients are shared: q1[opp] = -q1[i], q2[opp] = q2[i].
+template <class R>
+__device__ __forceinline__ void eq_terms(R ux, R uy, EqTerms<R>& e) {
+  using A = Arith<R>;
+  const R u2 = ux * ux + uy * uy;
+  e.t = A::divr(u2, R(kDenU2), R(kInvDenU2));
+  const R eu[5] = {R(0.0), ux, ux + uy, uy, -ux + uy};
+  e.q1[0] = R(0.0);
+  e.q2[0] = R(0.0);
+#pragma unroll
+  for (int i = 1; i < 5; ++i) {
+    const R q1 = A::divr(eu[i], R(kCs2), R(kInvCs2));
+    const R q2 = A::divr(eu[i] * eu[i], R(kDenEu2), R(kInvDenEu2));
+    e.q1[i] = q1;
+    e.q1[i + 4] = -q1;
+    e.q2[i] = q2;
+    e.q2[i + 4] = q2;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ R eq_quad(int i, R s, const EqTerms<R>& e) {
+  return R(kW[i]) * s * (R(1.0) + e.q1[i] + e.q2[i] - e.t);
+}
+
+template <class R, int GEQ>
+__device__ __forceinline__ R eq_g(int i, R t, const EqTerms<R>& e) {
+  if (GEQ == 1) return R(kW[i]) * t * (R(1.0) + e.q1[i]);
+  return eq_quad<R>(i, t, e);
+}
+
+// Stand-alone forms used by the initialisation and wall kernels.
 template <class R>
 __device__ __forceinline__ R feq(int i, R rho, R ux, R uy) {
-  const R cs2 = R(1.0) / R(3.0);
-  const R cx = R(kCx[i]), cy = R(kCy[i]);
-  const R eu = cx * ux + cy * uy;
-  const R u2 = ux * ux + uy * uy;
-  return R(kW[i]) * rho *
-         (R(1.0) + eu / cs2 + eu * eu / (R(2.0) * cs2 * cs2) - u2 / (R(2.0) * cs2));
+  EqTerms<R> e;
+  eq_terms<R>(ux, uy, e);
+  return eq_quad<R>(i, rho, e);
 }
 
 template <class R, int GEQ>
 __device__ __forceinline__ R geq(int i, R t, R ux, R uy) {
-  const R cs2 = R(1.0) / R(3.0);
-  const R cx = R(kCx[i]), cy = R(kCy[i]);
-  const R eu = cx * ux + cy * uy;
-  if (GEQ == 1) return R(kW[i]) * t * (R(1.0) + eu / cs2);
-  const R u2 = ux * ux + uy * uy;
-  return R(kW[i]) * t *
-         (R(1.0) + eu / cs2 + eu * eu / (R(2.0) * cs2 * cs2) - u2 / (R(2.0) * cs2));
+  EqTerms<R> e;
+  eq_terms<R>(ux, uy, e);
+  return eq_g<R, GEQ>(i, t, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -135,49 +218,62 @@ template <class R, int GEQ>
 __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, R uy,
                                              R temp, const StepCtx& c, R& fy_out,
                                              R& nut_out) {
-  const R cs2 = R(1.0) / R(3.0);
+  using A = Arith<R>;
   const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
   fy_out = fy;
 
+  EqTerms<R> e;
+  eq_terms<R>(ux, uy, e);
   R d_f[9], d_g[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    d_f[i] = f[i] - feq<R>(i, rho, ux, uy);
-    d_g[i] = g[i] - geq<R, GEQ>(i, temp, ux, uy);
+    d_f[i] = f[i] - eq_quad<R>(i, rho, e);
+    d_g[i] = g[i] - eq_g<R, GEQ>(i, temp, e);
   }
-  R pi_xx = R(0.0), pi_xy = R(0.0), pi_yy = R(0.0);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    const R cx = R(kCx[i]), cy = R(kCy[i]);
-    const R fneq = d_f[i];
-    pi_xx += fneq * cx * cx;
-    pi_xy += fneq * cx * cy;
-    pi_yy += fneq * cy * cy;
-  }
+  // pi_neq = sum c c (f - feq) (solver.rs:1096-1106).  With c in {-1,0,1} the
+  // products are exact; zero products are dropped because the accumulators
+  // start at +0 and never hold -0, so adding a signed zero leaves them as is.
+  const R pi_xx = R(0.0) + d_f[1] + d_f[2] + d_f[4] + d_f[5] + d_f[6] + d_f[8];
+  const R pi_xy = R(0.0) + d_f[2] - d_f[4] + d_f[6] - d_f[8];
+  const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
   const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
-  const R s_mag = sqrt(pi2) / (R(2.0) * cs2 * R(c.tau) * fmax(rho, R(1e-12)));
+  const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
   const R nu_t = R(c.cs) * R(c.cs) * s_mag;
   nut_out = nu_t;
 
   const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
-  const R tau_minus = R(0.5) + R(3.0 / 16.0) / (tau_plus - R(0.5));
-  const R alpha_t = nu_t / R(c.pr_t);
+  const R tau_minus = R(0.5) + A::div(R(3.0 / 16.0), tau_plus - R(0.5));
+  const R alpha_t = A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
   const R tau_t_plus = R(0.5) + R(3.0) * (R(c.alpha) + alpha_t);
-  const R tau_t_minus = tau_t_plus;
-  const R a_f = R(0.5) * (R(1.0) / tau_plus + R(1.0) / tau_minus);
-  const R b_f = R(0.5) * (R(1.0) / tau_plus - R(1.0) / tau_minus);
-  const R a_g = R(0.5) * (R(1.0) / tau_t_plus + R(1.0) / tau_t_minus);
-  const R b_g = R(0.5) * (R(1.0) / tau_t_plus - R(1.0) / tau_t_minus);
-  const R k_minus = R(1.0) - R(0.5) / tau_minus;
-  const R k_plus = R(1.0) - R(0.5) / tau_plus;
+  // 1/tau (IEEE); 0.5/tau = 0.5*(1/tau) exactly (power-of-two scaling).
+  const R itp = A::rcp(tau_plus);
+  const R itm = A::rcp(tau_minus);
+  const R itt = A::rcp(tau_t_plus);  // tau_t_minus = tau_t_plus (BGK for g)
+  const R a_f = R(0.5) * (itp + itm);
+  const R b_f = R(0.5) * (itp - itm);
+  const R a_g = R(0.5) * (itt + itt);
+  const R b_g = R(0.5) * (itt - itt);
+  const R k_minus = R(1.0) - R(0.5) * itm;
+  const R k_plus = R(1.0) - R(0.5) * itp;
 
+  // Guo source S_i = w_i [(cy - uy)/cs2 + (c.u) cy / cs2^2] fy; its odd part
+  // s_minus = w_i (cy/cs2) fy carries the momentum (solver.rs:1168-1179).
+  // (cy - uy)/cs2 takes three values; (c.u) cy is +-eu of the directions
+  // with cy = 1 (exact negation for cy = -1), and a signed zero for cy = 0,
+  // where (0 - uy)/cs2 is never -0, so that term adds nothing.
+  const R a_p = A::divr(R(1.0) - uy, R(kCs2), R(kInvCs2));
+  const R a_0 = A::divr(R(0.0) - uy, R(kCs2), R(kInvCs2));
+  const R a_m = A::divr(R(-1.0) - uy, R(kCs2), R(kInvCs2));
+  const R b2 = A::divr(ux + uy, R(kDenCs4), R(kInvDenCs4));
+  const R b3 = A::divr(uy, R(kDenCs4), R(kInvDenCs4));
+  const R b4 = A::divr(-ux + uy, R(kDenCs4), R(kInvDenCs4));
+  const R bracket[9] = {a_0, a_0, a_p + b2, a_p + b3, a_p + b4, a_0, a_m + b2, a_m + b3, a_m + b4};
   R fo[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    const R cx = R(kCx[i]), cy = R(kCy[i]);
-    const R ei_u = cx * ux + cy * uy;
-    const R s_i = R(kW[i]) * ((cy - uy) / cs2 + ei_u * cy / (cs2 * cs2)) * fy;
-    const R s_minus = R(kW[i]) * (cy / cs2) * fy;
+    const R cy = R(kCy[i]);
+    const R s_i = R(kW[i]) * bracket[i] * fy;
+    const R s_minus = R(kW[i]) * (cy / R(kCs2)) * fy;
     const R s_plus = s_i - s_minus;
     fo[i] = f[i] + (-a_f * d_f[i] - b_f * d_f[kOpp[i]] + k_minus * s_minus +
                     k_plus * s_plus);
@@ -211,6 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
   using ST = StorageT<S>;
   using T = typename ST::T;
   using R = typename ST::R;
+  using A = Arith<R>;
   int bx, by;
   tile_of_block(geo, bx, by);
   const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
@@ -218,8 +315,9 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
   const int nx = geo.nx, ny = geo.ny;
   if (ix >= nx || iy >= ny) return;
   const long long P = geo.plane;
+  const unsigned P32 = (unsigned)geo.plane;  // 18*plane < 2^31 (checked on the host)
   const int pitch = geo.pitch;
-  const long long k = (long long)iy * pitch + ix;
+  const int k = iy * pitch + ix;
   const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
 
   R f[9], g[9];
@@ -227,13 +325,14 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
 
   if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE) {
     const T* src = static_cast<const T*>(b.src);
-    // pull streaming with full-way bounce-back for f (solver.rs:890-912)
+    // pull streaming with full-way bounce-back for f (solver.rs:890-912);
+    // element offsets from the start of the population set fit in 32 bits.
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       const int sx = ix - kCx[i], sy = iy - kCy[i];
       const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
-      const long long o = in ? (long long)i * P + (long long)sy * pitch + sx
-                             : (long long)kOpp[i] * P + k;
+      const unsigned o = in ? (unsigned)i * P32 + (unsigned)(sy * pitch + sx)
+                            : (unsigned)kOpp[i] * P32 + (unsigned)k;
       f[i] = ST::load_f(src, o, in ? i : kOpp[i]);
     }
     // g: pull, then the thermal boundary conditions (solver.rs:951-988):
@@ -246,30 +345,30 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
       } else {
         th = R(kTCold);
       }
+      EqTerms<R> e0;
+      eq_terms<R>(R(0.0), R(0.0), e0);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) g[i] = geq<R, GEQ>(i, th, R(0.0), R(0.0));
+      for (int i = 0; i < 9; ++i) g[i] = eq_g<R, GEQ>(i, th, e0);
     } else {
       const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const long long o = (long long)(9 + i) * P + (long long)(gy - kCy[i]) * pitch + (ix - kCx[i]);
-        g[i] = ST::load_g(src, o);
-      }
+      for (int i = 0; i < 9; ++i)
+        g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 +
+                                   (unsigned)((gy - kCy[i]) * pitch + (ix - kCx[i])));
     }
     // update_macroscopic with the Guo correction (solver.rs:319-345)
-    R r = R(0.0), un = R(0.0), vn = R(0.0), tn = R(0.0);
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      r += f[j];
-      un += R(kCx[j]) * f[j];
-      vn += R(kCy[j]) * f[j];
-      tn += g[j];
-    }
+    // sums in the reference's order j = 0..8; zero products dropped and
+    // products by -1 turned into subtractions (exact, accumulators from +0)
+    const R r = R(0.0) + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    R un = R(0.0) + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
+    R vn = R(0.0) + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
+    const R tn = R(0.0) + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
     const R fyp = static_cast<const R*>(b.fy)[k];
     if (r > R(kF64Eps)) {
-      un /= r;
-      vn /= r;
-      vn += R(0.5) * fyp / r;
+      const R ir = A::rcp(r);
+      un = A::divr(un, r, ir);
+      vn = A::divr(vn, r, ir);
+      vn += A::divr(R(0.5) * fyp, r, ir);
     }
     // enforce_wall_velocity (solver.rs:992-1009)
     if (wall) {
@@ -283,8 +382,8 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
     if (MODE == MODE_MATERIALIZE) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        b.mpops[(long long)i * P + k] = (double)f[i];
-        b.mpops[(long long)(9 + i) * P + k] = (double)g[i];
+        b.mpops[i * P + k] = (double)f[i];
+        b.mpops[(9 + i) * P + k] = (double)g[i];
       }
       b.rho[k] = (double)rho;
       b.ux[k] = (double)ux;
@@ -295,8 +394,8 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
   } else {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      f[i] = R(b.mpops[(long long)i * P + k]);
-      g[i] = R(b.mpops[(long long)(9 + i) * P + k]);
+      f[i] = R(b.mpops[i * P + k]);
+      g[i] = R(b.mpops[(9 + i) * P + k]);
     }
     rho = R(b.rho[k]);
     ux = R(b.ux[k]);
@@ -312,15 +411,15 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
   if (MODE == MODE_COLLIDE_M) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      b.mpops[(long long)i * P + k] = (double)f[i];
-      b.mpops[(long long)(9 + i) * P + k] = (double)g[i];
+      b.mpops[i * P + k] = (double)f[i];
+      b.mpops[(9 + i) * P + k] = (double)g[i];
     }
   } else {
     T* dst = static_cast<T*>(b.dst);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      ST::store_f(dst, (long long)i * P + k, i, f[i]);
-      ST::store_g(dst, (long long)(9 + i) * P + k, g[i]);
+      ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
+      ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, g[i]);
     }
   }
 }
