@@ -372,8 +372,10 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
     return fail_free("hipStreamCreate failed");
   bool ok = true;
-  ok &= hipMalloc(&s->cpop[0], 18 * P * s->st_bytes) == hipSuccess;
-  ok &= hipMalloc(&s->cpop[1], 18 * P * s->st_bytes) == hipSuccess;
+  // FP16 sets carry one FP32 temperature plane after the 18 population planes.
+  const size_t set_bytes = 18 * P * s->st_bytes + (p.storage == ST_FP16_OFFSET ? P * 4 : 0);
+  ok &= hipMalloc(&s->cpop[0], set_bytes) == hipSuccess;
+  ok &= hipMalloc(&s->cpop[1], set_bytes) == hipSuccess;
   ok &= hipMalloc((void**)&s->mpops, 18 * P * sizeof(double)) == hipSuccess;
   for (auto& f : s->fld) ok &= hipMalloc((void**)&f, P * sizeof(double)) == hipSuccess;
   ok &= hipMalloc((void**)&s->scratch_field, P * sizeof(double)) == hipSuccess;

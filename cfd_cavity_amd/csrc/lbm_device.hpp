@@ -96,6 +96,8 @@ constexpr int kBlock = kTileX * kTileY;
 // instructions replace the generic division sequence.
 //
 // FP32 (precision forms): reciprocal multiplication and fast division.
+template <class R> inline constexpr bool kLowp = sizeof(R) == 4;  // FP32 arithmetic
+
 template <class R> struct Arith;
 template <> struct Arith<double> {
   __device__ static __forceinline__ double divr(double a, double b, double y) {
@@ -120,32 +122,42 @@ template <int S> struct StorageT;
 template <> struct StorageT<ST_FP64> {
   using T = double;
   using R = double;
+  static constexpr bool kTPlane = false;
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
-  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return p[o]; }
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o, int, R) { return p[o]; }
   __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = v; }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R) { p[o] = v; }
 };
 template <> struct StorageT<ST_FP32> {
   using T = float;
   using R = float;
+  static constexpr bool kTPlane = false;
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
-  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return p[o]; }
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o, int, R) { return p[o]; }
   __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = v; }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R) { p[o] = v; }
 };
-// FP16 storage of f - w_i (the rest-state offset keeps ~11 significant bits
-// on the O(1e-2) deviations) and of g itself (its rest value w_i*0 is zero).
+// FP16 storage of the deviations from each population's local rest state:
+// f_i - w_i, and g_i - w_i*T_j with T_j the temperature of the node j that
+// wrote the value (kept in one FP32 plane after the 18 FP16 planes).  The
+// stored deviations are O(1e-2) of the populations, so the ~11 significant
+// bits of FP16 act on them rather than on the populations themselves.
 template <> struct StorageT<ST_FP16_OFFSET> {
   using T = __half;
   using R = float;
+  static constexpr bool kTPlane = true;
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int i) {
     return __half2float(p[o]) + (float)kW[i];
   }
-  __device__ static __forceinline__ R load_g(const T* p, unsigned o) { return __half2float(p[o]); }
+  __device__ static __forceinline__ R load_g(const T* p, unsigned o, int i, R t_src) {
+    return __half2float(p[o]) + (float)kW[i] * t_src;
+  }
   __device__ static __forceinline__ void store_f(T* p, unsigned o, int i, R v) {
     p[o] = __float2half_rn(v - (float)kW[i]);
   }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, R v) { p[o] = __float2half_rn(v); }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int i, R v, R t_node) {
+    p[o] = __float2half_rn(v - (float)kW[i] * t_node);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -224,11 +236,24 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
 
   EqTerms<R> e;
   eq_terms<R>(ux, uy, e);
+  R feq_[9], geq_[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    feq_[i] = eq_quad<R>(i, rho, e);
+    geq_[i] = eq_g<R, GEQ>(i, temp, e);
+  }
+  if (kLowp<R>) {
+    // FP32 arithmetic only: close the zeroth moments exactly (sum feq = rho,
+    // sum geq = T) through the rest direction, so rounding of the weights and
+    // of the quadratic terms cannot leak mass or heat at every collision.
+    feq_[0] = rho - (feq_[1] + feq_[2] + feq_[3] + feq_[4] + feq_[5] + feq_[6] + feq_[7] + feq_[8]);
+    geq_[0] = temp - (geq_[1] + geq_[2] + geq_[3] + geq_[4] + geq_[5] + geq_[6] + geq_[7] + geq_[8]);
+  }
   R d_f[9], d_g[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    d_f[i] = f[i] - eq_quad<R>(i, rho, e);
-    d_g[i] = g[i] - eq_g<R, GEQ>(i, temp, e);
+    d_f[i] = f[i] - feq_[i];
+    d_g[i] = g[i] - geq_[i];
   }
   // pi_neq = sum c c (f - feq) (solver.rs:1096-1106).  With c in {-1,0,1} the
   // products are exact; zero products are dropped because the accumulators
@@ -268,13 +293,17 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R b3 = A::divr(uy, R(kDenCs4), R(kInvDenCs4));
   const R b4 = A::divr(-ux + uy, R(kDenCs4), R(kInvDenCs4));
   const R bracket[9] = {a_0, a_0, a_p + b2, a_p + b3, a_p + b4, a_0, a_m + b2, a_m + b3, a_m + b4};
+  R s_i[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s_i[i] = R(kW[i]) * bracket[i] * fy;
+  if (kLowp<R>)  // FP32 arithmetic only: sum_i S_i = 0 exactly
+    s_i[0] = -(s_i[1] + s_i[2] + s_i[3] + s_i[4] + s_i[5] + s_i[6] + s_i[7] + s_i[8]);
   R fo[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     const R cy = R(kCy[i]);
-    const R s_i = R(kW[i]) * bracket[i] * fy;
     const R s_minus = R(kW[i]) * (cy / R(kCs2)) * fy;
-    const R s_plus = s_i - s_minus;
+    const R s_plus = s_i[i] - s_minus;
     fo[i] = f[i] + (-a_f * d_f[i] - b_f * d_f[kOpp[i]] + k_minus * s_minus +
                     k_plus * s_plus);
   }
@@ -351,10 +380,14 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
       for (int i = 0; i < 9; ++i) g[i] = eq_g<R, GEQ>(i, th, e0);
     } else {
       const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+      const float* tplane =
+          reinterpret_cast<const float*>(src + (unsigned)18 * P32);  // used by FP16 only
 #pragma unroll
-      for (int i = 0; i < 9; ++i)
-        g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 +
-                                   (unsigned)((gy - kCy[i]) * pitch + (ix - kCx[i])));
+      for (int i = 0; i < 9; ++i) {
+        const unsigned nj = (unsigned)((gy - kCy[i]) * pitch + (ix - kCx[i]));
+        const R tj = ST::kTPlane ? R(tplane[nj]) : R(0.0);
+        g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 + nj, i, tj);
+      }
     }
     // update_macroscopic with the Guo correction (solver.rs:319-345)
     // sums in the reference's order j = 0..8; zero products dropped and
@@ -419,8 +452,9 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
-      ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, g[i]);
+      ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp);
     }
+    if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)temp;
   }
 }
 

@@ -1,0 +1,46 @@
+"""Precision forms (FP32 storage, FP16-offset storage; FP32 arithmetic, FP64
+reductions) against the FP64 path from the same initial state.  The gate is
+rel |dNu| <= 1e-4 for the FP32 form on the steady Ra=1e6 cavity; the FP16
+form's error is bounded and reported.  Errors are printed (pytest -s)."""
+import numpy as np
+import pytest
+
+from cfd_cavity_amd import CavitySolver
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "64x256_ra1e6": ((64, 256, 1.0e6, 0.71, 0.10, 0.9, 0.02, 42), 20_000),
+    "96x384_ra1e7": ((96, 384, 1.0e7, 0.71, 0.10, 0.9, 0.02, 42), 20_000),
+}
+# (rel dNu bound, rel d(w_max mid-height) bound, rel mass drift bound)
+BOUNDS = {"fp32": (1e-4, 1e-3, 2e-5), "fp16_offset": (5e-3, 5e-2, 1e-3)}
+
+
+def metrics(s):
+    d = s.diagnostics()
+    w, _ = s.max_uy_on_row(s.ny() // 2)
+    return d["nusselt_hot"], d["nusselt_cold"], w, d["total_mass"]
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("storage", ["fp32", "fp16_offset"])
+def test_precision_form_against_fp64(case, storage):
+    args, steps = CASES[case]
+    ref = CavitySolver(*args, 1, storage="fp64")
+    low = CavitySolver(*args, 1, storage=storage)
+    m0 = ref.total_mass()
+    ref.run(steps)
+    low.run(steps)
+    nh, nc, w, m = metrics(ref)
+    lh, lc, lw, lm = metrics(low)
+    e_nu = max(abs(lh - nh) / nh, abs(lc - nc) / nc)
+    e_w = abs(lw - w) / w
+    e_m = abs(lm - m0) / m0
+    print(f"\n[precision] {case} {storage}: Nu_hot {lh:.8f} vs {nh:.8f}  rel dNu {e_nu:.2e}  "
+          f"rel dw_max {e_w:.2e}  mass drift {e_m:.2e} (fp64 drift {abs(m - m0) / m0:.2e})")
+    b_nu, b_w, b_m = BOUNDS[storage]
+    assert np.isfinite(lh) and low.diagnostics()["all_finite"]
+    assert e_nu <= b_nu
+    assert e_w <= b_w
+    assert e_m <= b_m
