@@ -25,8 +25,10 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes per lattice update: read + write 9 f and 9 g populations
-# and the per-node Guo force fy_node.
-BYTES_PER_LUP = {"fp64": 2 * (18 * 8 + 8), "fp32": 2 * (18 * 4 + 4), "fp16_offset": 2 * (18 * 2 + 4)}
+# and the per-node Guo force fy_node (FP16-offset: plus the FP32 per-node
+# temperature plane its g offsets refer to).
+BYTES_PER_LUP = {"fp64": 2 * (18 * 8 + 8), "fp32": 2 * (18 * 4 + 4),
+                 "fp16_offset": 2 * (18 * 2 + 4 + 4)}
 DTYPE = {"fp64": "f64", "fp32": "f32", "fp16_offset": "f16-offset storage, f32 arithmetic"}
 REFERENCE_CPU_MLUPS_256x1024 = 49.8  # README/PLAN_GPU_BACKEND: 2M steps in ~2.9 h, jobs 8, M1 Max
 METRIC = "MLUPS & %HBM peak at 256x1024; mixed vs FP64 at rel dNu<=1e-4"
@@ -56,15 +58,19 @@ def gpu_form(storage: str, steps: int, warmup: int, kernel_launches: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     d = s.diagnostics()
-    kern_ms = s.time_step_kernel(kernel_launches)
+    kern_ms = s.time_step_kernel(kernel_launches) if kernel_launches > 0 else float("nan")
     mlups = nodes * steps / wall / 1e6
     b = BYTES_PER_LUP[storage]
-    achieved = nodes * b / (kern_ms * 1e-3) / 1e9
+    # average fused-kernel launch duration over the timed region, from HIP
+    # events recorded on the solver's stream (one launch per step)
+    launch_ms = ev_ms / steps
+    achieved = nodes * b / (launch_ms * 1e-3) / 1e9
     res = {
         "mlups": mlups,
         "ms_per_step": wall * 1e3 / steps,
         "ms_per_step_events": ev_ms / steps,
-        "kernel_ms": kern_ms,
+        "kernel_ms_isolated": kern_ms,
+        "launch_ms": launch_ms,
         "bytes_per_lup": b,
         "hbm_frac_whole_job": mlups * 1e6 * b / (HBM_PEAK_GBS * 1e9),
         "kernel_gbs": achieved,
@@ -78,6 +84,82 @@ def gpu_form(storage: str, steps: int, warmup: int, kernel_launches: int):
     }
     s.close()
     return res
+
+
+BATCH_SIZES = [(64, 256), (96, 384), (128, 512), (192, 768), (256, 1024)]
+
+
+def batch_specs(n=64, seed=42):
+    """64 cavities: sizes drawn (seeded) from BATCH_SIZES, Ra log-uniform in
+    [1e6, 1e9], Cs uniform in [0.08, 0.12], Pr 0.71, utarget 0.02."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        nx, ny = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        ra = float(10 ** rng.uniform(6.0, 9.0))
+        cs = float(rng.uniform(0.08, 0.12))
+        specs.append(dict(nx=nx, ny=ny, ra=ra, cs=cs, pr=0.71, pr_t=0.9, utarget=0.02,
+                          seed=seed + i))
+    return specs
+
+
+def batched_form(steps: int, warmup: int, storage: str = "fp32"):
+    """One launch per step for all cavities (graph replays), autotuned, against
+    the same cavities stepped one after another with the single-cavity path."""
+    from cfd_cavity_amd import CavityBatch, CavitySolver
+
+    specs = batch_specs()
+    b = CavityBatch(specs, storage=storage)
+    nodes = b.nodes()
+    b.run(warmup)
+    tune = b.autotune(trial_steps=max(130, steps // 10))
+    ms = b.time_run(steps)
+    mlups = nodes * steps / (ms * 1e-3) / 1e6
+    finite = sum(1 for c in b if c.diagnostics()["all_finite"])
+    b.close()
+    seq_ms = 0.0
+    for sp in specs:
+        s = CavitySolver(sp["nx"], sp["ny"], sp["ra"], sp["pr"], sp["cs"], sp["pr_t"],
+                         sp["utarget"], sp["seed"], 1, storage=storage)
+        s.run(warmup)
+        seq_ms += s.time_run(steps)
+        s.close()
+    seq_mlups = nodes * steps / (seq_ms * 1e-3) / 1e6
+    return {"cavities": len(specs), "nodes": nodes, "storage": storage, "steps": steps,
+            "mlups": mlups, "ms_per_step": ms / steps, "sequential_mlups": seq_mlups,
+            "speedup_vs_sequential": mlups / seq_mlups, "tuned": tune,
+            "finite_cavities": finite,
+            "hbm_frac": mlups * 1e6 * BYTES_PER_LUP[storage] / (HBM_PEAK_GBS * 1e9)}
+
+
+def measured_traffic():
+    """HBM bytes per fused FP64 launch at 256x1024 from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy
+    kernel), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_step_fused_256x1024.md")
+    try:
+        for line in open(path):
+            if line.startswith("TRAFFIC_FP64_BYTES_PER_LAUNCH"):
+                return int(line.split("=")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib: int, reps: int = 5):
+    """A plain device copy of known size (torch's copy kernel), for
+    calibrating FETCH_SIZE / WRITE_SIZE: run it under rocprofv3 --pmc."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
 
 
 def cpu_baseline(budget_s: float):
@@ -112,7 +194,14 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32,fp16_offset")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the multi-GPU "
               "layout is a design note, not a measured path.", file=sys.stderr)
@@ -129,6 +218,11 @@ def main():
             v["rel_dNu_vs_fp64"] = abs(v["nusselt_hot"] - ref_nu) / abs(ref_nu)
             v["speedup_vs_fp64"] = v["mlups"] / forms["fp64"]["mlups"]
             v["rel_dmass_vs_fp64"] = abs(v["mass"] - forms["fp64"]["mass"]) / forms["fp64"]["mass"]
+
+    batched = None
+    if not a.no_batch:
+        print(f"[bench] batched: 64 cavities, {a.batch_steps} steps", file=sys.stderr, flush=True)
+        batched = batched_form(a.batch_steps, a.warmup)
 
     cpu = None
     if not a.no_cpu:
@@ -157,11 +251,14 @@ def main():
                    "storage": "fp64"},
         "roofline": {"bound": "hbm", "achieved": round(main_form["kernel_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(main_form["kernel_hbm_frac"], 4), "traffic": None,
-                     "bytes_per_launch": nodes * b, "kernel_ms": main_form["kernel_ms"]},
+                     "frac": round(main_form["kernel_hbm_frac"], 4),
+                     "traffic": measured_traffic(),
+                     "bytes_per_launch": nodes * b, "launch_ms": main_form["launch_ms"],
+                     "timing": "HIP events around the timed steps on the solver stream"},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -38,6 +38,12 @@ class Diag(C.Structure):
         ("all_finite", C.c_int32), ("reserved", C.c_int32), ("time_step", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_x", C.c_int32), ("tile_y", C.c_int32), ("xcd_remap", C.c_int32),
+                ("candidates", C.c_int32), ("best_ms_per_step", C.c_double),
+                ("first_ms_per_step", C.c_double)]
+
+
 class Derived(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "lz", "lx", "nu", "alpha", "alpha_phys", "beta_eff", "tau", "tau_t", "cs",
@@ -72,6 +78,15 @@ SIGNATURES = {
     "cavlbm_time_run": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_time_step_kernel": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_last_error": (C.c_char_p, [_vp]),
+    "cavlbm_batch_create": (C.c_int, [C.POINTER(Params), _u64, C.POINTER(_vp)]),
+    "cavlbm_batch_destroy": (C.c_int, [_vp]),
+    "cavlbm_batch_run": (C.c_int, [_vp, _u64]),
+    "cavlbm_batch_synchronize": (C.c_int, [_vp]),
+    "cavlbm_batch_solver": (C.c_int, [_vp, _u64, C.POINTER(_vp)]),
+    "cavlbm_batch_autotune": (C.c_int, [_vp, _u64, C.POINTER(TuneResult)]),
+    "cavlbm_batch_nodes": (C.c_int, [_vp, C.POINTER(_u64)]),
+    "cavlbm_batch_time_run": (C.c_int, [_vp, _u64, _dp]),
+    "cavlbm_batch_last_error": (C.c_char_p, [_vp]),
 }
 
 _lib = None

@@ -29,14 +29,22 @@ UNITS = {
     "lbm_aux.hip": ["-ffp-contract=off"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
+# one source, two objects: the batched kernel in FP64 (no contraction) and in
+# the FP32-arithmetic forms (contraction allowed)
+VARIANTS = {
+    "lbm_batch_fp64": ("lbm_batch.hip", ["-ffp-contract=off", "-DCAVLBM_BATCH_PRECISION=64"]),
+    "lbm_batch_lowp": ("lbm_batch.hip", ["-ffp-contract=fast", "-DCAVLBM_BATCH_PRECISION=32"]),
+}
 
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     procs = []
     objs = []
-    for src, extra in UNITS.items():
-        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    jobs = [(os.path.splitext(src)[0], src, extra) for src, extra in UNITS.items()]
+    jobs += [(name, src, extra) for name, (src, extra) in VARIANTS.items()]
+    for name, src, extra in jobs:
+        obj = os.path.join(OBJ, name + ".o")
         cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -713,3 +713,405 @@ const char* cavlbm_last_error(const cavlbm_solver* s) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched form
+// ===========================================================================
+struct cavlbm_batch {
+  std::vector<cavlbm_solver*> cav;
+  int storage = 0, geq = 2, device = 0;
+  hipStream_t st = nullptr;
+  CavDesc* d_desc = nullptr;
+  int* d_tile_cav = nullptr;
+  int total_tiles = 0;
+  int shape = 0, xcd = 1;
+  uint64_t max_spin = 0;
+  uint64_t nodes = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool graph_failed = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int bfail(cavlbm_batch* b, int code, const std::string& m) {
+  if (b)
+    b->err = m;
+  else
+    g_create_error = m;
+  return code;
+}
+
+#define BHIP_TRY(b, expr)                                                             \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return bfail((b), CAVLBM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void batch_drop_graphs(cavlbm_batch* b) {
+  for (auto& g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// (Re)builds the cavity descriptors and the tile table for the current tile
+// shape; buffer pointers come from each solver's two C-form sets.
+int batch_layout(cavlbm_batch* b) {
+  const TileShape sh = kBatchShapes[b->shape];
+  std::vector<CavDesc> desc(b->cav.size());
+  std::vector<int> tile_cav;
+  int begin = 0;
+  for (size_t c = 0; c < b->cav.size(); ++c) {
+    cavlbm_solver* s = b->cav[c];
+    CavDesc& d = desc[c];
+    d.geo = s->geo;
+    d.geo.nbx = (s->geo.nx + sh.tx - 1) / sh.tx;
+    d.geo.nby = (s->geo.ny + sh.ty - 1) / sh.ty;
+    d.ctx = make_ctx(s, s->d.spin_up_steps);
+    d.beta_eff = s->d.beta_eff;
+    d.spin_up = (int)s->d.spin_up_steps;
+    d.tile_begin = begin;
+    d.walls = Walls{s->theta};
+    for (int par = 0; par < 2; ++par) {
+      const int saved = s->cur;
+      s->cur = par;
+      d.bufs[par] = bufs(s);
+      s->cur = saved;
+    }
+    const int nt = d.geo.nbx * d.geo.nby;
+    tile_cav.insert(tile_cav.end(), nt, (int)c);
+    begin += nt;
+  }
+  b->total_tiles = begin;
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_tile_cav) (void)hipFree(b->d_tile_cav);
+  b->d_desc = nullptr;
+  b->d_tile_cav = nullptr;
+  BHIP_TRY(b, hipMalloc((void**)&b->d_desc, desc.size() * sizeof(CavDesc)));
+  BHIP_TRY(b, hipMalloc((void**)&b->d_tile_cav, tile_cav.size() * sizeof(int)));
+  BHIP_TRY(b, hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(CavDesc),
+                             hipMemcpyHostToDevice, b->st));
+  BHIP_TRY(b, hipMemcpyAsync(b->d_tile_cav, tile_cav.data(), tile_cav.size() * sizeof(int),
+                             hipMemcpyHostToDevice, b->st));
+  BHIP_TRY(b, hipStreamSynchronize(b->st));
+  batch_drop_graphs(b);
+  b->graph_failed = false;
+  return 0;
+}
+
+hipError_t batch_launch(cavlbm_batch* b, int parity, uint64_t step, int write_aux) {
+  const int stepi = step > (uint64_t)INT32_MAX ? INT32_MAX : (int)step;
+  if (b->storage == ST_FP64)
+    return launch_step_batch_fp64(b->geq, b->shape, b->d_desc, b->d_tile_cav, b->total_tiles,
+                                  parity, stepi, write_aux, b->xcd, b->st);
+  return launch_step_batch_lowp(b->storage, b->geq, b->shape, b->d_desc, b->d_tile_cav,
+                                b->total_tiles, parity, stepi, write_aux, b->xcd, b->st);
+}
+
+// Every cavity in C-form, with a common buffer parity and step count.  A
+// cavity still in M-form (never stepped, or modified through a setter) takes
+// its next step alone (MODE_FIRST); *consumed reports whether that happened.
+int batch_prepare(cavlbm_batch* b, int write_aux, uint64_t* consumed) {
+  *consumed = 0;
+  for (cavlbm_solver* s : b->cav) BHIP_TRY(b, hipStreamSynchronize(s->st));
+  bool any = false;
+  for (cavlbm_solver* s : b->cav) any |= !s->c_valid;
+  if (any) {
+    // keep the cavities in lockstep: the ones already in C-form take the
+    // same step through the fused kernel
+    for (cavlbm_solver* s : b->cav)
+      if (one_step(s, write_aux)) return bfail(b, CAVLBM_E_HIP, s->err);
+    *consumed = 1;
+  }
+  for (cavlbm_solver* s : b->cav) BHIP_TRY(b, hipStreamSynchronize(s->st));
+  const cavlbm_solver* s0 = b->cav[0];
+  for (cavlbm_solver* s : b->cav)
+    if (s->cur != s0->cur || s->time_step != s0->time_step)
+      return bfail(b, CAVLBM_E_PARAM, "batch cavities out of lockstep (stepped individually?)");
+  return 0;
+}
+
+bool batch_ready(const cavlbm_batch* b) {
+  for (const cavlbm_solver* s : b->cav)
+    if (!s->c_valid) return false;
+  return true;
+}
+
+void batch_advance(cavlbm_batch* b, uint64_t k) {
+  for (cavlbm_solver* s : b->cav) {
+    s->time_step += k;
+    if (k & 1) s->cur ^= 1;
+    s->m_valid = false;
+  }
+}
+
+hipGraphExec_t batch_capture(cavlbm_batch* b, int par) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  bool ok = true;
+  int p = par;
+  for (int i = 0; i < kGraphSteps && ok; ++i) {
+    ok = batch_launch(b, p, b->max_spin, 0) == hipSuccess;
+    p ^= 1;
+  }
+  if (hipStreamEndCapture(b->st, &g) != hipSuccess || !ok) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  return ex;
+}
+
+int batch_run_steps(cavlbm_batch* b, uint64_t n) {
+  if (n == 0) return 0;
+  uint64_t consumed = 0;
+  int rc = batch_prepare(b, n == 1 ? 1 : 0, &consumed);
+  if (rc) return rc;
+  uint64_t left = n - consumed;
+  cavlbm_solver* s0 = b->cav[0];
+  while (left > 0) {
+    if (!b->graph_failed && s0->time_step >= b->max_spin && left > (uint64_t)kGraphSteps) {
+      hipGraphExec_t& g = b->graph[s0->cur];
+      if (!g) g = batch_capture(b, s0->cur);
+      if (!g) {
+        b->graph_failed = true;
+        continue;
+      }
+      BHIP_TRY(b, hipGraphLaunch(g, b->st));
+      batch_advance(b, kGraphSteps);
+      left -= kGraphSteps;
+      continue;
+    }
+    BHIP_TRY(b, batch_launch(b, s0->cur, s0->time_step, left == 1 ? 1 : 0));
+    batch_advance(b, 1);
+    --left;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cavlbm_batch_create(const cavlbm_params* cavities, uint64_t n, cavlbm_batch** out) {
+  if (!cavities || !out || n == 0) return bfail(nullptr, CAVLBM_E_PARAM, "empty batch");
+  *out = nullptr;
+  const cavlbm_params& p0 = cavities[0];
+  for (uint64_t i = 0; i < n; ++i) {
+    const cavlbm_params& p = cavities[i];
+    if (p.storage != p0.storage || p.geq_order != p0.geq_order || p.device != p0.device)
+      return bfail(nullptr, CAVLBM_E_PARAM, "batch cavities must share storage, geq_order, device");
+    if (p.nu_start_mult > 1.0)
+      return bfail(nullptr, CAVLBM_E_UNSUPPORTED, "batch requires nu_start_mult = 1");
+  }
+  auto* b = new cavlbm_batch();
+  b->storage = p0.storage;
+  b->geq = p0.geq_order;
+  b->device = p0.device;
+  for (uint64_t i = 0; i < n; ++i) {
+    cavlbm_solver* s = nullptr;
+    int rc = cavlbm_create(&cavities[i], &s);
+    if (rc) {
+      cavlbm_batch_destroy(b);
+      return rc;  // message already in cavlbm_last_error(NULL)
+    }
+    b->cav.push_back(s);
+    b->max_spin = std::max<uint64_t>(b->max_spin, s->d.spin_up_steps);
+    b->nodes += s->p.nx * s->p.ny;
+  }
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    cavlbm_batch_destroy(b);
+    return bfail(nullptr, CAVLBM_E_HIP, "batch stream/event creation failed");
+  }
+  if (batch_layout(b)) {
+    const std::string m = b->err;
+    cavlbm_batch_destroy(b);
+    return bfail(nullptr, CAVLBM_E_HIP, m);
+  }
+  *out = b;
+  return 0;
+}
+
+int cavlbm_batch_destroy(cavlbm_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_drop_graphs(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_tile_cav) (void)hipFree(b->d_tile_cav);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  for (cavlbm_solver* s : b->cav) cavlbm_destroy(s);
+  delete b;
+  return 0;
+}
+
+int cavlbm_batch_run(cavlbm_batch* b, uint64_t nsteps) {
+  if (!b) return CAVLBM_E_PARAM;
+  BHIP_TRY(b, hipSetDevice(b->device));
+  int rc = batch_run_steps(b, nsteps);
+  if (rc) return rc;
+  BHIP_TRY(b, hipStreamSynchronize(b->st));
+  return 0;
+}
+
+int cavlbm_batch_synchronize(cavlbm_batch* b) {
+  if (!b) return CAVLBM_E_PARAM;
+  BHIP_TRY(b, hipStreamSynchronize(b->st));
+  return 0;
+}
+
+int cavlbm_batch_solver(cavlbm_batch* b, uint64_t i, cavlbm_solver** out) {
+  if (!b || !out) return CAVLBM_E_PARAM;
+  if (i >= b->cav.size()) return bfail(b, CAVLBM_E_PARAM, "cavity index out of range");
+  *out = b->cav[i];
+  return 0;
+}
+
+int cavlbm_batch_nodes(const cavlbm_batch* b, uint64_t* out) {
+  if (!b || !out) return CAVLBM_E_PARAM;
+  *out = b->nodes;
+  return 0;
+}
+
+int cavlbm_batch_time_run(cavlbm_batch* b, uint64_t nsteps, double* ms) {
+  if (!b || !ms) return CAVLBM_E_PARAM;
+  BHIP_TRY(b, hipSetDevice(b->device));
+  BHIP_TRY(b, hipEventRecord(b->ev0, b->st));
+  int rc = batch_run_steps(b, nsteps);
+  if (rc) return rc;
+  BHIP_TRY(b, hipEventRecord(b->ev1, b->st));
+  BHIP_TRY(b, hipEventSynchronize(b->ev1));
+  float t = 0.f;
+  BHIP_TRY(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+  *ms = t;
+  return 0;
+}
+
+int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_result* out) {
+  if (!b || !out || trial_steps == 0) return CAVLBM_E_PARAM;
+  BHIP_TRY(b, hipSetDevice(b->device));
+  if (!batch_ready(b))
+    return bfail(b, CAVLBM_E_PARAM, "run the batch at least one step before autotuning");
+  uint64_t consumed = 0;
+  int rc = batch_prepare(b, 0, &consumed);
+  if (rc) return rc;
+  // Snapshot the state the trials will overwrite: the current C-form set,
+  // fy_node and nu_t of every cavity, and the step counters.
+  struct Snap {
+    void* pops;
+    void* fy;
+    void* nut;
+    size_t pops_bytes, field_bytes;
+    int cur;
+    uint64_t t;
+  };
+  std::vector<Snap> snap;
+  auto free_snap = [&]() {
+    for (auto& z : snap) {
+      if (z.pops) (void)hipFree(z.pops);
+      if (z.fy) (void)hipFree(z.fy);
+      if (z.nut) (void)hipFree(z.nut);
+    }
+  };
+  for (cavlbm_solver* s : b->cav) {
+    Snap z{};
+    const size_t P = (size_t)s->geo.plane;
+    z.pops_bytes = 18 * P * s->st_bytes + (s->storage == ST_FP16_OFFSET ? P * 4 : 0);
+    z.field_bytes = P * s->r_bytes;
+    z.cur = s->cur;
+    z.t = s->time_step;
+    bool ok = hipMalloc(&z.pops, z.pops_bytes) == hipSuccess &&
+              hipMalloc(&z.fy, z.field_bytes) == hipSuccess &&
+              hipMalloc(&z.nut, z.field_bytes) == hipSuccess;
+    snap.push_back(z);
+    if (!ok) {
+      free_snap();
+      return bfail(b, CAVLBM_E_HIP, "autotune snapshot allocation failed");
+    }
+    ok = hipMemcpyAsync(z.pops, s->cpop[s->cur], z.pops_bytes, hipMemcpyDeviceToDevice, b->st) ==
+             hipSuccess &&
+         hipMemcpyAsync(z.fy, s->fy, z.field_bytes, hipMemcpyDeviceToDevice, b->st) == hipSuccess &&
+         hipMemcpyAsync(z.nut, s->nut, z.field_bytes, hipMemcpyDeviceToDevice, b->st) == hipSuccess;
+    if (!ok) {
+      free_snap();
+      return bfail(b, CAVLBM_E_HIP, "autotune snapshot copy failed");
+    }
+  }
+  auto restore = [&]() -> bool {
+    bool ok = true;
+    for (size_t c = 0; c < b->cav.size(); ++c) {
+      cavlbm_solver* s = b->cav[c];
+      const Snap& z = snap[c];
+      s->cur = z.cur;
+      s->time_step = z.t;
+      s->c_valid = true;
+      s->m_valid = false;
+      ok &= hipMemcpyAsync(s->cpop[s->cur], z.pops, z.pops_bytes, hipMemcpyDeviceToDevice,
+                           b->st) == hipSuccess;
+      ok &= hipMemcpyAsync(s->fy, z.fy, z.field_bytes, hipMemcpyDeviceToDevice, b->st) ==
+            hipSuccess;
+      ok &= hipMemcpyAsync(s->nut, z.nut, z.field_bytes, hipMemcpyDeviceToDevice, b->st) ==
+            hipSuccess;
+    }
+    ok &= hipStreamSynchronize(b->st) == hipSuccess;
+    return ok;
+  };
+  double best = 1e300, first = 0.0;
+  int best_shape = 0, best_xcd = 1, count = 0;
+  for (int shape = 0; shape < kNumBatchShapes; ++shape) {
+    for (int xcd = 1; xcd >= 0; --xcd) {
+      b->shape = shape;
+      b->xcd = xcd;
+      if (batch_layout(b) || !restore()) {
+        free_snap();
+        return CAVLBM_E_HIP;
+      }
+      // one untimed pass (graph capture, caches), then the timed trial
+      double ms = 0.0;
+      if (batch_run_steps(b, trial_steps) || hipStreamSynchronize(b->st) != hipSuccess ||
+          !restore() || cavlbm_batch_time_run(b, trial_steps, &ms)) {
+        free_snap();
+        return CAVLBM_E_HIP;
+      }
+      const double per = ms / (double)trial_steps;
+      if (count == 0) first = per;
+      if (per < best) {
+        best = per;
+        best_shape = shape;
+        best_xcd = xcd;
+      }
+      ++count;
+    }
+  }
+  b->shape = best_shape;
+  b->xcd = best_xcd;
+  const int lrc = batch_layout(b);
+  const bool rok = restore();
+  free_snap();
+  if (lrc || !rok) return bfail(b, CAVLBM_E_HIP, "autotune restore failed");
+  out->tile_x = kBatchShapes[best_shape].tx;
+  out->tile_y = kBatchShapes[best_shape].ty;
+  out->xcd_remap = best_xcd;
+  out->candidates = count;
+  out->best_ms_per_step = best;
+  out->first_ms_per_step = first;
+  return 0;
+}
+
+const char* cavlbm_batch_last_error(const cavlbm_batch* b) {
+  return b ? b->err.c_str() : g_create_error.c_str();
+}
+
+}  // extern "C"

@@ -330,19 +330,17 @@ __device__ __forceinline__ void tile_of_block(const Geom& geo, int& bx, int& by)
 
 // ---------------------------------------------------------------------------
 // The step kernel: one thread per lattice node.
+// The work of one lattice node (ix, iy) inside [0, nx) x [0, ny) for one
+// step kernel mode.
 template <int S, int GEQ, int MODE>
-__global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls walls,
-                                                  StepBufs b, int write_aux) {
+__device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
+                                          const Walls& walls, const StepBufs& b,
+                                          int write_aux, int ix, int iy) {
   using ST = StorageT<S>;
   using T = typename ST::T;
   using R = typename ST::R;
   using A = Arith<R>;
-  int bx, by;
-  tile_of_block(geo, bx, by);
-  const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
-  const int iy = by * kTileY + (int)(threadIdx.x / kTileX);
   const int nx = geo.nx, ny = geo.ny;
-  if (ix >= nx || iy >= ny) return;
   const long long P = geo.plane;
   const unsigned P32 = (unsigned)geo.plane;  // 18*plane < 2^31 (checked on the host)
   const int pitch = geo.pitch;
@@ -456,6 +454,17 @@ __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls wa
     }
     if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)temp;
   }
+}
+
+template <int S, int GEQ, int MODE>
+__global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls walls,
+                                                  StepBufs b, int write_aux) {
+  int bx, by;
+  tile_of_block(geo, bx, by);
+  const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
+  const int iy = by * kTileY + (int)(threadIdx.x / kTileX);
+  if (ix >= geo.nx || iy >= geo.ny) return;
+  step_node<S, GEQ, MODE>(geo, ctx, walls, b, write_aux, ix, iy);
 }
 
 }  // namespace cavlbm

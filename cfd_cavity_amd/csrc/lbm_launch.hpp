@@ -50,4 +50,31 @@ hipError_t launch_field_convert(const Geom& geo, int storage, const void* src_r,
 
 inline int nblocks_of(const Geom& g) { return g.nbx * g.nby; }
 
+// ---------------------------------------------------------------------------
+// Batched step: many cavities advanced by one launch.  One descriptor per
+// cavity (device memory); `tile_cav` maps every global tile to its cavity.
+struct CavDesc {
+  Geom geo;          // nbx / nby for the tile shape in use
+  StepCtx ctx;       // scalars after spin-up (beta = beta_eff)
+  double beta_eff;   // beta during spin-up: beta_eff * (step / spin_up)
+  int spin_up;
+  int tile_begin;    // first global tile of this cavity
+  Walls walls;
+  StepBufs bufs[2];  // by buffer parity (src/dst exchanged)
+};
+
+// Tile shapes the batched kernel is instantiated for (autotuner candidates).
+struct TileShape {
+  int tx, ty;
+};
+constexpr int kNumBatchShapes = 4;
+constexpr TileShape kBatchShapes[kNumBatchShapes] = {{64, 4}, {64, 2}, {64, 8}, {128, 4}};
+
+hipError_t launch_step_batch_fp64(int geq, int shape, const CavDesc* descs, const int* tile_cav,
+                                  int total_tiles, int parity, int step, int write_aux,
+                                  int xcd_remap, hipStream_t st);
+hipError_t launch_step_batch_lowp(int storage, int geq, int shape, const CavDesc* descs,
+                                  const int* tile_cav, int total_tiles, int parity, int step,
+                                  int write_aux, int xcd_remap, hipStream_t st);
+
 }  // namespace cavlbm

@@ -27,6 +27,23 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def make_params(nx, ny, ra, pr, cs, pr_t, utarget, seed, jobs=1, alpha_mult=1.0,
+                nu_start_mult=1.0, nu_override=None, *, storage="fp64", geq_order=2, device=0):
+    """The C-ABI parameter struct for CavitySolver::new's twelve arguments."""
+    p = _lib.Params()
+    check(lib().cavlbm_params_default(C.byref(p)))
+    p.nx, p.ny = int(nx), int(ny)
+    p.ra, p.pr, p.cs, p.pr_t, p.utarget = ra, pr, cs, pr_t, utarget
+    p.seed, p.jobs = int(seed), int(jobs)
+    p.alpha_mult, p.nu_start_mult = alpha_mult, nu_start_mult
+    p.has_nu_override = 0 if nu_override is None else 1
+    p.nu_override = 0.0 if nu_override is None else float(nu_override)
+    p.storage = _lib.STORAGE[storage]
+    p.geq_order = int(geq_order)
+    p.device = int(device)
+    return p
+
+
 class _LbmView:
     """Read-only view mirroring the reference's ``s.lbm`` (ThermalD2Q9) fields."""
 
@@ -90,35 +107,40 @@ class CavitySolver:
                  alpha_mult=1.0, nu_start_mult=1.0, nu_override=None, *,
                  storage="fp64", geq_order=2, device=0):
         L = lib()
-        p = _lib.Params()
-        check(L.cavlbm_params_default(C.byref(p)))
-        p.nx, p.ny = int(nx), int(ny)
-        p.ra, p.pr, p.cs, p.pr_t, p.utarget = ra, pr, cs, pr_t, utarget
-        p.seed, p.jobs = int(seed), int(jobs)
-        p.alpha_mult, p.nu_start_mult = alpha_mult, nu_start_mult
-        p.has_nu_override = 0 if nu_override is None else 1
-        p.nu_override = 0.0 if nu_override is None else float(nu_override)
-        p.storage = _lib.STORAGE[storage]
-        p.geq_order = int(geq_order)
-        p.device = int(device)
+        p = make_params(nx, ny, ra, pr, cs, pr_t, utarget, seed, jobs, alpha_mult,
+                        nu_start_mult, nu_override, storage=storage, geq_order=geq_order,
+                        device=device)
         h = C.c_void_p()
         rc = L.cavlbm_create(C.byref(p), C.byref(h))
         if rc == _lib.E_PARAM:
             raise ValueError(L.cavlbm_last_error(None).decode())
         check(rc, None)
-        self._h = h
-        self._L = L
-        self._nx, self._ny = int(nx), int(ny)
+        self._adopt(h, int(nx), int(ny), storage, owned=True)
         self.jobs = max(int(jobs), 1)
+
+    def _adopt(self, h, nx, ny, storage, owned):
+        self._h = h
+        self._L = lib()
+        self._nx, self._ny = nx, ny
         self.storage = storage
+        self._owned = owned
         self.lbm = _LbmView(self)
+
+    @classmethod
+    def _borrow(cls, handle, nx, ny, storage, keepalive):
+        """Wrap a handle owned elsewhere (a batch cavity); never destroyed here."""
+        obj = cls.__new__(cls)
+        obj._adopt(handle, nx, ny, storage, owned=False)
+        obj.jobs = 1
+        obj._keepalive = keepalive
+        return obj
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owned", False):
             self._L.cavlbm_destroy(h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

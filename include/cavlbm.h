@@ -174,6 +174,43 @@ int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
 int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
 int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
 
+/* ---- batched form -------------------------------------------------------
+ * A batch advances many independent cavities (sizes, Ra, Cs may differ) with
+ * one kernel launch per step, captured in HIP graphs after the buoyancy
+ * spin-up.  No counterpart in the reference, which runs one cavity per
+ * process (src/main.rs:42-443); each cavity behaves exactly like a
+ * cavlbm_solver created from the same parameters (bitwise equal results). */
+typedef struct cavlbm_batch cavlbm_batch;
+
+typedef struct {
+  int32_t tile_x, tile_y;   /* chosen workgroup tile (nodes) */
+  int32_t xcd_remap;        /* 1: tiles remapped so each XCD owns contiguous rows */
+  int32_t candidates;       /* configurations timed */
+  double best_ms_per_step;  /* of the chosen configuration */
+  double first_ms_per_step; /* of the first candidate (64x4, remapped) */
+} cavlbm_tune_result;
+
+/* n parameter sets (same storage, geq_order and device; nu_start_mult 1). */
+int cavlbm_batch_create(const cavlbm_params *cavities, uint64_t n,
+                        cavlbm_batch **out);
+int cavlbm_batch_destroy(cavlbm_batch *b);
+/* nsteps steps of every cavity. */
+int cavlbm_batch_run(cavlbm_batch *b, uint64_t nsteps);
+int cavlbm_batch_synchronize(cavlbm_batch *b);
+/* Borrowed handle of cavity i for reads, diagnostics and setters (valid
+ * until cavlbm_batch_destroy; do not destroy it; do not step it alone). */
+int cavlbm_batch_solver(cavlbm_batch *b, uint64_t i, cavlbm_solver **out);
+/* Times every tile shape x block-order candidate for trial_steps steps and
+ * keeps the fastest; the cavities' state is restored afterwards.  The batch
+ * must have been run at least one step (CAVLBM_E_PARAM otherwise); tune after
+ * the spin-up so that the trials time the graph-replayed steady steps. */
+int cavlbm_batch_autotune(cavlbm_batch *b, uint64_t trial_steps,
+                          cavlbm_tune_result *out);
+/* Total lattice nodes of the batch. */
+int cavlbm_batch_nodes(const cavlbm_batch *b, uint64_t *out);
+int cavlbm_batch_time_run(cavlbm_batch *b, uint64_t nsteps, double *ms);
+const char *cavlbm_batch_last_error(const cavlbm_batch *b);
+
 /* Message of the last failure on this handle (or of cavlbm_create when s is
  * NULL).  Valid until the next call on the same handle / thread. */
 const char *cavlbm_last_error(const cavlbm_solver *s);

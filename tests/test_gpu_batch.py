@@ -1,0 +1,68 @@
+"""Batched form: every cavity of a batch must equal, bit for bit, the same
+cavity stepped alone (same per-node code), across graph replays and after
+the autotuner (which must leave the state unchanged)."""
+import numpy as np
+import pytest
+
+from cfd_cavity_amd import CavityBatch, CavitySolver
+from tests.helpers import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [
+    dict(nx=16, ny=40, ra=1e5, cs=0.10),
+    dict(nx=64, ny=256, ra=1e6, cs=0.08),
+    dict(nx=96, ny=384, ra=3e7, cs=0.12),
+    dict(nx=24, ny=100, ra=2e6, cs=0.11, seed=7),
+]
+
+
+def single(sp, storage):
+    return CavitySolver(sp["nx"], sp["ny"], sp["ra"], 0.71, sp["cs"], 0.9, 0.02,
+                        sp.get("seed", 42), 1, storage=storage)
+
+
+def same_state(a, b):
+    for name in ("rho", "ux", "uy", "temperature", "fy_node", "nu_t"):
+        if not bits_equal(a.get_field(name), b.get_field(name)):
+            return name
+    for w in ("f", "g"):
+        if not bits_equal(a.get_pops(w), b.get_pops(w)):
+            return w
+    return None
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16_offset"])
+def test_batch_equals_single_cavities(storage):
+    batch = CavityBatch(SPECS, storage=storage)
+    singles = [single(sp, storage) for sp in SPECS]
+    for chunk in (1, 150, 300):   # spin-up direct launches, then graph replays
+        batch.run(chunk)
+        for s in singles:
+            s.run(chunk)
+        for i, (c, s) in enumerate(zip(batch, singles)):
+            assert c.time_step == s.time_step
+            bad = same_state(c, s)
+            assert bad is None, f"cavity {i} differs in {bad} after {c.time_step} steps"
+
+
+def test_autotune_keeps_state_and_picks_a_candidate():
+    batch = CavityBatch(SPECS, storage="fp32")
+    ref = [single(sp, "fp32") for sp in SPECS]
+    batch.run(250)
+    for s in ref:
+        s.run(250)
+    res = batch.autotune(trial_steps=130)
+    assert res["candidates"] == 8
+    assert (res["tile_x"], res["tile_y"]) in {(64, 4), (64, 2), (64, 8), (128, 4)}
+    assert res["best_ms_per_step"] <= res["first_ms_per_step"]
+    for i, (c, s) in enumerate(zip(batch, ref)):
+        assert c.time_step == s.time_step
+        assert same_state(c, s) is None, f"autotune changed cavity {i}"
+    batch.run(200)
+    for s in ref:
+        s.run(200)
+    for i, (c, s) in enumerate(zip(batch, ref)):
+        assert same_state(c, s) is None, f"cavity {i} after tuned run"
+    d = batch[1].diagnostics()
+    assert d["all_finite"] and d["time_step"] == 450
