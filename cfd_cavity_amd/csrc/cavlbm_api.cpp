@@ -1115,3 +1115,220 @@ const char* cavlbm_batch_last_error(const cavlbm_batch* b) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Window objective and its adjoint (FP64)
+// ===========================================================================
+namespace {
+
+// Device work area of one window evaluation; freed on destruction.
+struct WindowWork {
+  std::vector<void*> allocs;
+  ~WindowWork() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  template <class T>
+  T* alloc(size_t n, hipStream_t st) {
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    if (hipMemsetAsync(p, 0, n * sizeof(T), st) != hipSuccess) return nullptr;
+    return static_cast<T*>(p);
+  }
+};
+
+StepCtx window_ctx(const cavlbm_solver* s, uint64_t step, double cs) {
+  StepCtx c = make_ctx(s, step);
+  c.cs = cs;
+  return c;
+}
+
+// One forward collision of the window into (dst, fy); src == nullptr: the
+// first collision, from the solver's M-form state S_0.
+hipError_t window_step(cavlbm_solver* s, const double* src, double* dst, double* fy,
+                       double* nut, const double* theta, const StepCtx& ctx) {
+  StepBufs b;
+  b.src = src;
+  b.dst = dst;
+  b.fy = fy;
+  b.nut = nut;
+  b.mpops = s->mpops;
+  b.rho = s->fld[0];
+  b.ux = s->fld[1];
+  b.uy = s->fld[2];
+  b.temp = s->fld[3];
+  return launch_step_fp64(s->geq, src ? MODE_FUSED : MODE_FIRST, s->geo, ctx, Walls{theta}, b, 0,
+                          s->st);
+}
+
+int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K,
+                 WindowWork& w, double*& theta) {
+  if (s->storage != ST_FP64)
+    return fail(s, CAVLBM_E_UNSUPPORTED, "window objective and adjoint need FP64 storage");
+  if (K == 0) return fail(s, CAVLBM_E_PARAM, "window_steps must be >= 1");
+  if (theta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "theta_hot must have ny values");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = materialize(s);  // S_0 in M-form (the logical state is unchanged)
+  if (rc) return rc;
+  theta = w.alloc<double>(s->p.ny, s->st);
+  if (!theta) return fail(s, CAVLBM_E_HIP, "window allocation failed");
+  std::vector<double> h(s->p.ny, 0.5);
+  if (theta_hot) std::copy(theta_hot, theta_hot + s->p.ny, h.begin());
+  HIP_TRY(s, hipMemcpyAsync(theta, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                            s->st));
+  return 0;
+}
+
+// Forward window: J, and optionally the checkpoints (X_{qC}, fy_{qC}).
+int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K, uint64_t C,
+                   WindowWork& w, std::vector<double*>* ckpt, double* j_out) {
+  const size_t P = (size_t)s->geo.plane;
+  double* X[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
+  double* fy = w.alloc<double>(P, s->st);
+  double* nut = w.alloc<double>(P, s->st);
+  double* acc = w.alloc<double>(s->p.ny, s->st);
+  if (!X[0] || !X[1] || !fy || !nut || !acc) return fail(s, CAVLBM_E_HIP, "window allocation failed");
+  const uint64_t t0 = s->time_step;
+  int cur = 0;
+  for (uint64_t m = 0; m < K; ++m) {
+    // X_m = Collide(S_m): S_0 from the M-form, then the fused step from X_{m-1}
+    const double* src = m == 0 ? nullptr : X[cur];
+    double* dst = m == 0 ? X[cur] : X[cur ^ 1];
+    HIP_TRY(s, window_step(s, src, dst, fy, nut, theta, window_ctx(s, t0 + m, cs)));
+    if (m > 0) cur ^= 1;
+    if (ckpt && m % C == 0) {
+      double* slot = w.alloc<double>(19 * P, s->st);
+      if (!slot) return fail(s, CAVLBM_E_HIP, "checkpoint allocation failed");
+      HIP_TRY(s, hipMemcpyAsync(slot, X[cur], 18 * P * sizeof(double), hipMemcpyDeviceToDevice,
+                                s->st));
+      HIP_TRY(s, hipMemcpyAsync(slot + 18 * P, fy, P * sizeof(double), hipMemcpyDeviceToDevice,
+                                s->st));
+      ckpt->push_back(slot);
+    }
+    // Nu(S_{m+1}) from X_m
+    HIP_TRY(s, launch_nu_rows(s->geo, s->geq, X[cur], theta, acc, nullptr, s->st));
+  }
+  std::vector<double> h(s->p.ny);
+  HIP_TRY(s, hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  *j_out = sum / (double)s->p.ny * s->d.lz / 1.0 / (double)K;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cavlbm_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny, double cs,
+                     uint64_t window_steps, double* j_out) {
+  if (!s || !j_out) return CAVLBM_E_PARAM;
+  WindowWork w;
+  double* theta = nullptr;
+  int rc = window_setup(s, theta_hot, ny, window_steps, w, theta);
+  if (rc) return rc;
+  return window_forward(s, theta, cs, window_steps, 1, w, nullptr, j_out);
+}
+
+int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny, double cs,
+                             uint64_t window_steps, uint64_t checkpoint_every, double* j_out,
+                             double* grad_theta, double* grad_cs) {
+  if (!s || !j_out || !grad_theta || !grad_cs) return CAVLBM_E_PARAM;
+  const uint64_t K = window_steps;
+  const uint64_t C = checkpoint_every ? checkpoint_every : 100;
+  WindowWork w;
+  double* theta = nullptr;
+  int rc = window_setup(s, theta_hot, ny, K, w, theta);
+  if (rc) return rc;
+  std::vector<double*> ckpt;
+  rc = window_forward(s, theta, cs, K, C, w, &ckpt, j_out);
+  if (rc) return rc;
+
+  const size_t P = (size_t)s->geo.plane;
+  const uint64_t t0 = s->time_step;
+  const uint64_t seg_len = std::min<uint64_t>(C, K);
+  std::vector<double*> seg(seg_len);
+  for (auto& q : seg)
+    if (!(q = w.alloc<double>(19 * P, s->st))) return fail(s, CAVLBM_E_HIP, "segment allocation failed");
+  double* lam[2] = {w.alloc<double>(19 * P, s->st), w.alloc<double>(19 * P, s->st)};
+  double* mu = w.alloc<double>(19 * P, s->st);
+  double* nut = w.alloc<double>(P, s->st);
+  double* sgn = w.alloc<double>(s->p.ny, s->st);
+  double* th_acc = w.alloc<double>(s->p.ny, s->st);
+  double* cs_acc = w.alloc<double>(P, s->st);
+  double* cs_sum = w.alloc<double>(1, s->st);
+  if (!lam[0] || !lam[1] || !mu || !nut || !sgn || !th_acc || !cs_acc || !cs_sum)
+    return fail(s, CAVLBM_E_HIP, "adjoint allocation failed");
+  const double obj_w = s->d.lz / ((double)s->p.ny * 1.0 * (double)K);
+
+  AdjArgs a{};
+  a.theta = theta;
+  a.theta_acc = th_acc;
+  a.cs_acc = cs_acc;
+  a.obj_w = obj_w;
+  a.sgn = sgn;
+  int lc = 0;              // lam[lc] holds the adjoint of X_{m+1}
+  bool lam_zero = true;    // X_K is not used by J
+  const uint64_t nseg = (K + C - 1) / C;
+  for (uint64_t q = nseg; q-- > 0;) {
+    const uint64_t m_lo = q * C, m_hi = std::min<uint64_t>(m_lo + C, K);
+    // recompute X_m, fy_m for m in [m_lo, m_hi) from the checkpoint
+    HIP_TRY(s, hipMemcpyAsync(seg[0], ckpt[q], 19 * P * sizeof(double), hipMemcpyDeviceToDevice,
+                              s->st));
+    for (uint64_t m = m_lo + 1; m < m_hi; ++m) {
+      double* prev = seg[m - 1 - m_lo];
+      double* next = seg[m - m_lo];
+      HIP_TRY(s, hipMemcpyAsync(next + 18 * P, prev + 18 * P, P * sizeof(double),
+                                hipMemcpyDeviceToDevice, s->st));
+      HIP_TRY(s, window_step(s, prev, next, next + 18 * P, nut, theta, window_ctx(s, t0 + m, cs)));
+    }
+    for (uint64_t m = m_hi; m-- > m_lo;) {
+      double* Xm = seg[m - m_lo];
+      HIP_TRY(s, launch_nu_rows(s->geo, s->geq, Xm, theta, nullptr, sgn, s->st));
+      a.X = Xm;
+      a.fyp = Xm + 18 * P;
+      a.lam = lam_zero ? nullptr : lam[lc];
+      a.lam_fy = lam_zero ? nullptr : lam[lc] + 18 * P;
+      a.mu = mu;
+      a.mu_fy = mu + 18 * P;
+      HIP_TRY(s, launch_adj_step(s->geo, s->geq, false, window_ctx(s, t0 + m + 1, cs), a, s->st));
+      HIP_TRY(s, launch_adj_gather(s->geo, mu, mu + 18 * P, lam[lc ^ 1], lam[lc ^ 1] + 18 * P,
+                                   s->st));
+      lc ^= 1;
+      lam_zero = false;
+    }
+  }
+  // the first collision X_0 = Collide(S_0): its Cs dependence
+  a.X = nullptr;
+  a.fyp = nullptr;
+  a.mpops = s->mpops;
+  a.mrho = s->fld[0];
+  a.mux = s->fld[1];
+  a.muy = s->fld[2];
+  a.mtemp = s->fld[3];
+  a.lam = lam[lc];
+  a.lam_fy = lam[lc] + 18 * P;
+  a.sgn = nullptr;
+  HIP_TRY(s, launch_adj_step(s->geo, s->geq, true, window_ctx(s, t0, cs), a, s->st));
+  HIP_TRY(s, launch_sum_nodes(s->geo, cs_acc, cs_sum, s->st));
+  std::vector<double> th(s->p.ny);
+  double gcs = 0.0;
+  HIP_TRY(s, hipMemcpyAsync(th.data(), th_acc, th.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            s->st));
+  HIP_TRY(s, hipMemcpyAsync(&gcs, cs_sum, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  // fold the per-row contributions of column 0 onto theta indices: the
+  // corner rows use theta[1] and theta[ny-2]
+  const uint64_t n = s->p.ny;
+  for (uint64_t i = 0; i < n; ++i) grad_theta[i] = 0.0;
+  for (uint64_t iy = 0; iy < n; ++iy) {
+    const uint64_t iyc = iy == 0 ? 1 : (iy == n - 1 ? n - 2 : iy);
+    grad_theta[iyc] += th[iy];
+  }
+  *grad_cs = gcs;
+  return 0;
+}
+
+}  // extern "C"

@@ -77,4 +77,32 @@ hipError_t launch_step_batch_lowp(int storage, int geq, int shape, const CavDesc
                                   const int* tile_cav, int total_tiles, int parity, int step,
                                   int write_aux, int xcd_remap, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Adjoint (reverse mode) of the FP64 fused step, see lbm_adjoint.hip.
+struct AdjArgs {
+  const double* X;       // X_m (18 planes); unused for the first collision
+  const double* fyp;     // fy_m
+  const double* mpops;   // S_0 populations (first collision only)
+  const double* mrho;    // S_0 fields (first collision only)
+  const double* mux;
+  const double* muy;
+  const double* mtemp;
+  const double* lam;     // adjoint of X_{m+1} (18 planes), null when zero
+  const double* lam_fy;  // adjoint of fy_{m+1}, null when zero
+  const double* sgn;     // sign(T(1,iy) - T(2,iy)) of S_{m+1}, null: no objective term
+  double obj_w;          // Lz / (ny dT K)
+  const double* theta;   // hot-wall profile (ny)
+  double* mu;            // out: adjoint of the 18 values each node pulled
+  double* mu_fy;         // out: adjoint of fy_m
+  double* theta_acc;     // += dJ/dtheta contribution, by node row of column 0
+  double* cs_acc;        // += dJ/dCs contribution, per node
+};
+hipError_t launch_nu_rows(const Geom& geo, int geq, const double* X, const double* theta,
+                          double* acc, double* sgn, hipStream_t st);
+hipError_t launch_adj_step(const Geom& geo, int geq, bool first, const StepCtx& ctx,
+                           const AdjArgs& a, hipStream_t st);
+hipError_t launch_adj_gather(const Geom& geo, const double* mu, const double* mu_fy, double* lam,
+                             double* lam_fy, hipStream_t st);
+hipError_t launch_sum_nodes(const Geom& geo, const double* v, double* out, hipStream_t st);
+
 }  // namespace cavlbm

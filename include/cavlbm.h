@@ -174,6 +174,26 @@ int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
 int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
 int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
 
+/* ---- differentiation (FP64 storage only) -------------------------------
+ * Objective over a window of K steps starting from the solver's current
+ * state S_0 (left unchanged by these calls):
+ *     J = (1/K) sum_{m=1..K} nusselt_hot(S_m)          (post.rs:68-78)
+ * with the hot-wall profile theta_hot[iy] (ny values; NULL: constant 0.5) in
+ * the isothermal condition (solver.rs:957-963) and the Smagorinsky constant
+ * cs in collide_node (solver.rs:1120) during the window.  No counterpart in
+ * the reference (which has no derivatives); for calibration / inverse
+ * problems on the reference's model. */
+int cavlbm_nu_window(cavlbm_solver *s, const double *theta_hot, uint64_t ny,
+                     double cs, uint64_t window_steps, double *j_out);
+/* J and its gradient by the hand-written adjoint of the fused step, with
+ * checkpoints every checkpoint_every steps and recomputation in between.
+ * grad_theta has ny entries (rows 0 and ny-1 are exactly 0: the adiabatic
+ * rows overwrite the corners, solver.rs:974-987). */
+int cavlbm_adjoint_nu_window(cavlbm_solver *s, const double *theta_hot,
+                             uint64_t ny, double cs, uint64_t window_steps,
+                             uint64_t checkpoint_every, double *j_out,
+                             double *grad_theta, double *grad_cs);
+
 /* ---- batched form -------------------------------------------------------
  * A batch advances many independent cavities (sizes, Ra, Cs may differ) with
  * one kernel launch per step, captured in HIP graphs after the buoyancy
