@@ -80,6 +80,7 @@ SIGNATURES = {
     "cavlbm_last_error": (C.c_char_p, [_vp]),
     "cavlbm_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _dp]),
     "cavlbm_adjoint_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _u64, _dp, _dp, _dp]),
+    "cavlbm_tangent_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _dp, _d, _u64, _dp, _dp]),
     "cavlbm_batch_create": (C.c_int, [C.POINTER(Params), _u64, C.POINTER(_vp)]),
     "cavlbm_batch_destroy": (C.c_int, [_vp]),
     "cavlbm_batch_run": (C.c_int, [_vp, _u64]),

@@ -53,6 +53,19 @@ def adjoint_nu_window(solver, theta, cs, window_steps, checkpoint_every=100):
     return j.value, gth, gcs.value
 
 
+def tangent_nu_window(solver, theta, cs, dtheta, dcs, window_steps):
+    """(J, dJ[v]) along v = (dtheta, dcs) by forward-mode FP64 steps."""
+    th = _theta(solver, theta)
+    dth = _theta(solver, dtheta)
+    j = C.c_double()
+    dj = C.c_double()
+    solver._check(solver._L.cavlbm_tangent_nu_window(
+        solver._h, None if th is None else _dp(th), solver.ny(), float(cs),
+        None if dth is None else _dp(dth), float(dcs), int(window_steps), C.byref(j),
+        C.byref(dj)))
+    return j.value, dj.value
+
+
 def _torch_function():
     import torch
 

@@ -28,6 +28,7 @@ UNITS = {
     "lbm_lowp.hip": ["-ffp-contract=fast"],
     "lbm_aux.hip": ["-ffp-contract=off"],
     "lbm_adjoint.hip": ["-ffp-contract=off"],
+    "lbm_tangent.hip": ["-ffp-contract=off"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 # one source, two objects: the batched kernel in FP64 (no contraction) and in

@@ -1332,3 +1332,50 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
 }
 
 }  // extern "C"
+
+extern "C" int cavlbm_tangent_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny,
+                                        double cs, const double* dtheta_hot, double dcs,
+                                        uint64_t window_steps, double* j_out, double* dj_out) {
+  if (!s || !j_out || !dj_out) return CAVLBM_E_PARAM;
+  const uint64_t K = window_steps;
+  WindowWork w;
+  double* theta = nullptr;
+  int rc = window_setup(s, theta_hot, ny, K, w, theta);
+  if (rc) return rc;
+  if (dtheta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "dtheta_hot must have ny values");
+  const size_t P = (size_t)s->geo.plane;
+  double* dtheta = w.alloc<double>(s->p.ny, s->st);
+  double* X[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
+  double* Xd[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
+  double* fy = w.alloc<double>(P, s->st);
+  double* fyd = w.alloc<double>(P, s->st);
+  double* acc = w.alloc<double>(s->p.ny, s->st);
+  double* dacc = w.alloc<double>(s->p.ny, s->st);
+  if (!dtheta || !X[0] || !X[1] || !Xd[0] || !Xd[1] || !fy || !fyd || !acc || !dacc)
+    return fail(s, CAVLBM_E_HIP, "tangent allocation failed");
+  if (dtheta_hot)
+    HIP_TRY(s, hipMemcpyAsync(dtheta, dtheta_hot, s->p.ny * sizeof(double), hipMemcpyHostToDevice,
+                              s->st));
+  const uint64_t t0 = s->time_step;
+  int cur = 0;
+  for (uint64_t m = 0; m < K; ++m) {
+    const bool first = m == 0;
+    const int out = first ? cur : cur ^ 1;
+    HIP_TRY(s, launch_tangent_step(s->geo, s->geq, first, window_ctx(s, t0 + m, cs), dcs, theta,
+                                   dtheta, X[cur], Xd[cur], fy, fyd, X[out], Xd[out], s->mpops,
+                                   s->fld[0], s->fld[1], s->fld[2], s->fld[3], s->st));
+    cur = out;
+    HIP_TRY(s, launch_nu_rows_dual(s->geo, X[cur], Xd[cur], theta, dtheta, acc, dacc, s->st));
+  }
+  std::vector<double> h(s->p.ny), hd(s->p.ny);
+  HIP_TRY(s, hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(s, hipMemcpyAsync(hd.data(), dacc, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            s->st));
+  HIP_TRY(s, hipStreamSynchronize(s->st));
+  double sum = 0.0, dsum = 0.0;
+  for (double v : h) sum += v;
+  for (double v : hd) dsum += v;
+  *j_out = sum / (double)s->p.ny * s->d.lz / 1.0 / (double)K;
+  *dj_out = dsum / (double)s->p.ny * s->d.lz / 1.0 / (double)K;
+  return 0;
+}

@@ -228,7 +228,7 @@ __device__ __forceinline__ R geq(int i, R t, R ux, R uy) {
 // from pi_neq and the Guo odd/even force split for f; BGK for g.
 template <class R, int GEQ>
 __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, R uy,
-                                             R temp, const StepCtx& c, R& fy_out,
+                                             R temp, const StepCtx& c, R cs, R& fy_out,
                                              R& nut_out) {
   using A = Arith<R>;
   const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
@@ -263,7 +263,7 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
   const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
   const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
-  const R nu_t = R(c.cs) * R(c.cs) * s_mag;
+  const R nu_t = cs * cs * s_mag;
   nut_out = nu_t;
 
   const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
@@ -435,7 +435,7 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   }
 
   R fy_new, nut_new;
-  collide_node<R, GEQ>(f, g, rho, ux, uy, temp, ctx, fy_new, nut_new);
+  collide_node<R, GEQ>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new);
   static_cast<R*>(b.fy)[k] = fy_new;
   if (write_aux) static_cast<R*>(b.nut)[k] = nut_new;
 

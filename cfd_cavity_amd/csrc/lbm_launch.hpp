@@ -105,4 +105,15 @@ hipError_t launch_adj_gather(const Geom& geo, const double* mu, const double* mu
                              double* lam_fy, hipStream_t st);
 hipError_t launch_sum_nodes(const Geom& geo, const double* v, double* out, hipStream_t st);
 
+// Tangent-linear (forward mode) step and Nusselt rows, see lbm_tangent.hip.
+hipError_t launch_tangent_step(const Geom& geo, int geq, bool first, const StepCtx& ctx,
+                               double dcs, const double* theta, const double* dtheta,
+                               const double* X, const double* Xd, double* fy, double* fyd,
+                               double* Y, double* Yd, const double* mpops, const double* mrho,
+                               const double* mux, const double* muy, const double* mtemp,
+                               hipStream_t st);
+hipError_t launch_nu_rows_dual(const Geom& geo, const double* X, const double* Xd,
+                               const double* theta, const double* dtheta, double* acc,
+                               double* dacc, hipStream_t st);
+
 }  // namespace cavlbm

@@ -194,6 +194,14 @@ int cavlbm_adjoint_nu_window(cavlbm_solver *s, const double *theta_hot,
                              uint64_t checkpoint_every, double *j_out,
                              double *grad_theta, double *grad_cs);
 
+/* J and its directional derivative dJ[v] along v = (dtheta_hot[ny], dcs) by
+ * forward-mode (tangent-linear) FP64 steps; J is bitwise the value of
+ * cavlbm_nu_window. */
+int cavlbm_tangent_nu_window(cavlbm_solver *s, const double *theta_hot,
+                             uint64_t ny, double cs, const double *dtheta_hot,
+                             double dcs, uint64_t window_steps, double *j_out,
+                             double *dj_out);
+
 /* ---- batched form -------------------------------------------------------
  * A batch advances many independent cavities (sizes, Ra, Cs may differ) with
  * one kernel launch per step, captured in HIP graphs after the buoyancy

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from cfd_cavity_amd import CavitySolver
-from cfd_cavity_amd.adjoint import NuWindow, adjoint_nu_window, nu_window
+from cfd_cavity_amd.adjoint import NuWindow, adjoint_nu_window, nu_window, tangent_nu_window
 from tests.helpers import bits_equal
 
 pytestmark = pytest.mark.gpu
@@ -44,9 +44,18 @@ def fd_check(s, theta, cs, K, C, dirs, h=1e-6, tol=1e-4):
         fd = (jp - jm) / (2 * h)
         ad = float(gth @ dth + gcs * dcs)
         errs.append(abs(ad - fd) / max(abs(fd), 1e-12))
+    # the FP64 tangent (forward mode, same per-node templates on dual numbers)
+    terrs = []
+    for dth, dcs in dirs:
+        jt, djt = tangent_nu_window(s, theta, cs, dth, dcs, K)
+        assert jt == float(j)  # same forward arithmetic, bitwise
+        ad = float(gth @ dth + gcs * dcs)
+        terrs.append(abs(ad - djt) / max(abs(djt), 1e-12))
     print(f"\n[adjoint] J={float(j):.10f} |dJ/dtheta|={np.linalg.norm(gth):.4e} dJ/dCs={gcs:.4e} "
-          f"max rel err vs FD over {len(dirs)} directions: {max(errs):.2e}")
+          f"max rel err over {len(dirs)} directions: vs FD {max(errs):.2e}, vs tangent "
+          f"{max(terrs):.2e}")
     assert max(errs) <= tol, errs
+    assert max(terrs) <= 1e-8, terrs
     return float(j), gth, gcs
 
 
