@@ -6,7 +6,7 @@ include/cavlbm.h); this package is its host-side mirror of the reference's
 ``CavitySolver`` API.  There is no CPU fallback.
 """
 from .solver import CavitySolver, DELTA_T, T_COLD, T_HOT  # noqa: F401
-from . import post, run  # noqa: F401
+from . import post, run, snapshot  # noqa: F401
 from .batch import CavityBatch  # noqa: F401
 
-__all__ = ["CavitySolver", "CavityBatch", "T_HOT", "T_COLD", "DELTA_T", "post", "run"]
+__all__ = ["CavitySolver", "CavityBatch", "T_HOT", "T_COLD", "DELTA_T", "post", "run", "snapshot"]

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcavlbm.so")
 
-E_PARAM, E_SIZE, E_HIP, E_UNSUPPORTED, E_NODEVICE = -1, -2, -3, -4, -5
+E_PARAM, E_SIZE, E_HIP, E_UNSUPPORTED, E_NODEVICE, E_IO = -1, -2, -3, -4, -5, -6
 STORAGE = {"fp64": 0, "fp32": 1, "fp16_offset": 2}
 FIELD = {"rho": 0, "ux": 1, "uy": 2, "temperature": 3, "nu_t": 4, "fy_node": 5}
 PHASE = {"collide": 0, "stream": 1, "thermal_bc": 2, "update_macroscopic": 3,
@@ -78,6 +78,16 @@ SIGNATURES = {
     "cavlbm_time_run": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_time_step_kernel": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_last_error": (C.c_char_p, [_vp]),
+    "cavlbm_average_reset": (C.c_int, [_vp]),
+    "cavlbm_average_accumulate": (C.c_int, [_vp]),
+    "cavlbm_average_count": (C.c_int, [_vp, C.POINTER(_u64)]),
+    "cavlbm_average_means": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _u64]),
+    "cavlbm_snapshot_write": (C.c_int, [C.c_char_p, _u64, _u64, _dp, _dp, _dp, _dp]),
+    "cavlbm_snapshot_write_atomic": (C.c_int, [C.c_char_p, _u64, _u64, _dp, _dp, _dp, _dp]),
+    "cavlbm_snapshot_read": (C.c_int, [C.c_char_p, C.POINTER(_u64), C.POINTER(_u64), _dp, _dp,
+                                       _dp, _dp, _u64]),
+    "cavlbm_interpolate_field": (C.c_int, [_dp, _u64, _u64, _dp, _u64, _u64]),
+    "cavlbm_snapshot_last_error": (C.c_char_p, []),
     "cavlbm_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _dp]),
     "cavlbm_adjoint_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _u64, _dp, _dp, _dp]),
     "cavlbm_tangent_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _dp, _d, _u64, _dp, _dp]),

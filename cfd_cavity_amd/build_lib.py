@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcavlbm.so")
+CLI = os.path.join(HERE, "cavity_les")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -30,6 +31,7 @@ UNITS = {
     "lbm_adjoint.hip": ["-ffp-contract=off"],
     "lbm_tangent.hip": ["-ffp-contract=off"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
+    "snapshot.cpp": ["-ffp-contract=off"],
 }
 # one source, two objects: the batched kernel in FP64 (no contraction) and in
 # the FP32-arithmetic forms (contraction allowed)
@@ -70,6 +72,15 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the uniform-mode driver (C++ host code over the C-ABI)
+    exe_tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(CSRC, "cavity_les.cpp"), "-L", HERE, "-lcavlbm",
+           "-Wl,-rpath,$ORIGIN", "-o", exe_tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(exe_tmp, CLI)
     return LIB
 
 

@@ -1,0 +1,389 @@
+// cavity_les.cpp — the uniform-mode driver of the reference (bbrangeo/cfd_cavity
+// src/main.rs:42-443 `run_uniform`, flags of src/cli.rs:19-384) on the GPU
+// solver of libcavlbm, through the C++ mirror in include/cavlbm.hpp.
+//
+// Same flags (--key val or --key=val), defaults, validation messages' intent,
+// progress lines, abort guards and exit codes:
+//   0 ok / help, 1 CLI error, 2 abort (max_u > 0.10 or non-finite),
+//   3 no averaging samples, 4 output directory, 5 write failure,
+//   6 snapshot read / restart failure.
+// Outputs in --out: progress.log, snapshot.bin (CAVSNAP1), summary.json.  The
+// CSV / VTK field exporters of post.rs are not reproduced.
+// Extra flags: --storage fp64|fp32|fp16_offset, --geq-order 1|2, --device N.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cavlbm.hpp"
+
+namespace {
+
+struct Args {
+  size_t nx = 64, ny = 256;
+  double ra = 1.0e6, pr = 0.71, cs = 0.10, pr_t = 0.9;
+  size_t steps = 200000, sample_start = 100000, sample_every = 100;
+  std::string out = "output/runs_uniforme/ra1e6";
+  double utarget = 0.02;
+  uint64_t seed = 42;
+  size_t progress_every = 1000, jobs = 1;
+  double alpha_mult = 1.0, nu_start_mult = 1.0;
+  std::string restart;
+  double nu = -1.0;
+  size_t snapshot_every = 0;
+  bool dynamic_smag = false;
+  int storage = CAVLBM_STORAGE_FP64, geq_order = 2, device = 0;
+};
+
+constexpr double kMaxUAbort = 0.10;  // src/main.rs:46
+
+void usage() {
+  std::printf(
+      "Usage: cavity_les [OPTIONS]   (uniform mode of bbrangeo/cfd_cavity on an MI355X)\n"
+      "  --nx --ny --ra --pr --cs --pr_t --steps --sample-start --sample-every --out\n"
+      "  --utarget --seed --progress-every --jobs --alpha-mult --nu-start-mult --restart\n"
+      "  --nu --snapshot-every --dynamic-smag --storage fp64|fp32|fp16_offset --geq-order 1|2\n"
+      "  --device N\n");
+}
+
+bool parse_size(const std::string& v, size_t& out) {
+  char* e = nullptr;
+  const unsigned long long x = std::strtoull(v.c_str(), &e, 10);
+  if (v.empty() || *e || v[0] == '-') return false;
+  out = (size_t)x;
+  return true;
+}
+bool parse_f64(const std::string& v, double& out) {
+  char* e = nullptr;
+  out = std::strtod(v.c_str(), &e);
+  return !v.empty() && !*e;
+}
+
+// src/cli.rs:190-384 (uniform subset), returns "" or an error message.
+std::string parse(int argc, char** argv, Args& a, bool& help) {
+  for (int i = 1; i < argc; ++i) {
+    std::string arg = argv[i];
+    if (arg == "-h" || arg == "--help") {
+      help = true;
+      return "";
+    }
+    std::string key = arg, val;
+    bool inl = false;
+    const size_t eq = arg.find('=');
+    if (eq != std::string::npos) {
+      key = arg.substr(0, eq);
+      val = arg.substr(eq + 1);
+      inl = true;
+    }
+    while (!key.empty() && key[0] == '-') key.erase(0, 1);
+    for (auto& c : key)
+      if (c == '_' && key != "pr_t") c = '-';
+    if (key == "dynamic-smag") {
+      a.dynamic_smag = !inl || val == "true" || val == "1";
+      if (inl && val != "true" && val != "1" && val != "false" && val != "0")
+        return "--dynamic-smag : booléen attendu, lu '" + val + "'";
+      continue;
+    }
+    if (!inl) {
+      if (i + 1 >= argc) return "valeur manquante pour --" + key;
+      val = argv[++i];
+    }
+    bool ok = true;
+    if (key == "mode") {
+      if (val == "octree") return "--mode octree : non disponible sur ce solveur GPU (uniforme seul)";
+      ok = val == "uniform";
+    } else if (key == "nx") ok = parse_size(val, a.nx);
+    else if (key == "ny") ok = parse_size(val, a.ny);
+    else if (key == "ra") ok = parse_f64(val, a.ra);
+    else if (key == "pr") ok = parse_f64(val, a.pr);
+    else if (key == "cs") ok = parse_f64(val, a.cs);
+    else if (key == "pr_t" || key == "pr-t") ok = parse_f64(val, a.pr_t);
+    else if (key == "steps") ok = parse_size(val, a.steps);
+    else if (key == "sample-start") ok = parse_size(val, a.sample_start);
+    else if (key == "sample-every") ok = parse_size(val, a.sample_every);
+    else if (key == "out") a.out = val;
+    else if (key == "utarget") ok = parse_f64(val, a.utarget);
+    else if (key == "seed") {
+      size_t s = 0;
+      ok = parse_size(val, s);
+      a.seed = s;
+    } else if (key == "progress-every") ok = parse_size(val, a.progress_every);
+    else if (key == "jobs") ok = parse_size(val, a.jobs);
+    else if (key == "alpha-mult") ok = parse_f64(val, a.alpha_mult);
+    else if (key == "nu-start-mult") ok = parse_f64(val, a.nu_start_mult);
+    else if (key == "restart") a.restart = val;
+    else if (key == "snapshot-every") ok = parse_size(val, a.snapshot_every);
+    else if (key == "nu") {
+      ok = parse_f64(val, a.nu);
+      if (ok && !(a.nu > 0.0 && std::isfinite(a.nu))) return "--nu doit être > 0 et fini";
+      if (ok && a.nu >= 0.002)
+        return "--nu " + val + " ≥ 0.002 : inutile, le calage utarget suffit ; --nu est réservé au sous-plancher";
+    } else if (key == "storage") {
+      if (val == "fp64") a.storage = CAVLBM_STORAGE_FP64;
+      else if (val == "fp32") a.storage = CAVLBM_STORAGE_FP32;
+      else if (val == "fp16_offset") a.storage = CAVLBM_STORAGE_FP16_OFFSET;
+      else ok = false;
+    } else if (key == "geq-order") {
+      size_t g = 0;
+      ok = parse_size(val, g) && (g == 1 || g == 2);
+      a.geq_order = (int)g;
+    } else if (key == "device") {
+      size_t d = 0;
+      ok = parse_size(val, d);
+      a.device = (int)d;
+    } else if (key == "smag-alpha" || key == "nu-sm-min" || key == "eta-test") {
+      double x;
+      ok = parse_f64(val, x);
+    } else {
+      return "option inconnue : --" + key;
+    }
+    if (!ok) return "--" + key + " : valeur invalide '" + val + "'";
+  }
+  // src/cli.rs:334-356
+  if (a.nx < 8 || a.ny < 8) return "nx et ny doivent être ≥ 8";
+  if (a.ny <= a.nx) return "cavité haute : ny doit être > nx (Az ≈ 4)";
+  if (a.ra <= 0.0 || a.pr <= 0.0) return "ra et pr doivent être > 0";
+  if (a.sample_every == 0) return "sample-every doit être ≥ 1";
+  if (a.jobs == 0) return "jobs doit être ≥ 1";
+  if (!(a.alpha_mult > 0.0) || !std::isfinite(a.alpha_mult)) return "alpha-mult doit être > 0 et fini";
+  if (!(a.nu_start_mult >= 1.0) || !std::isfinite(a.nu_start_mult))
+    return "nu-start-mult doit être ≥ 1 et fini";
+  if (a.progress_every == 0) a.progress_every = 1;
+  return "";
+}
+
+// Rust's {:.3e}: mantissa, 'e', exponent without '+' or zero padding.
+std::string rust_exp(double v, int prec) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*e", prec, v);
+  std::string s(buf);
+  const size_t e = s.find('e');
+  if (e == std::string::npos) return s;
+  const int ex = std::atoi(s.c_str() + e + 1);
+  return s.substr(0, e) + "e" + std::to_string(ex);
+}
+
+// main.rs:740-753
+double nusselt_from_mean(const std::vector<double>& t, size_t nx, size_t ny, double lz, bool hot) {
+  double sum = 0.0;
+  for (size_t iy = 0; iy < ny; ++iy) {
+    const double g = hot ? t[iy * nx + 1] - t[iy * nx + 2] : t[iy * nx + nx - 3] - t[iy * nx + nx - 2];
+    sum += std::fabs(g);
+  }
+  return sum / (double)ny * lz / 1.0;
+}
+
+// main.rs:756-770 (max_of)
+std::pair<double, double> max_of(const std::vector<double>& prof, size_t nx) {
+  double best = 0.0;
+  size_t bi = 1;
+  for (size_t i = 1; i + 1 < nx; ++i)
+    if (std::fabs(prof[i]) > best) {
+      best = std::fabs(prof[i]);
+      bi = i;
+    }
+  return {best, (double)bi / (double)(nx - 1)};
+}
+
+bool write_text(const std::string& path, const std::string& text) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return false;
+  const size_t w = std::fwrite(text.data(), 1, text.size(), f);
+  return std::fclose(f) == 0 && w == text.size();
+}
+
+bool make_dirs(const std::string& path) {
+  std::string cur;
+  for (size_t i = 0; i <= path.size(); ++i) {
+    if (i == path.size() || path[i] == '/') {
+      if (!cur.empty()) {
+        struct stat st;
+        if (stat(cur.c_str(), &st) != 0 && mkdir(cur.c_str(), 0755) != 0) return false;
+      }
+    }
+    if (i < path.size()) cur.push_back(path[i]);
+  }
+  return true;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args a;
+  bool help = false;
+  const std::string err = parse(argc, argv, a, help);
+  if (help) {
+    usage();
+    return 0;
+  }
+  if (!err.empty()) {
+    std::fprintf(stderr, "Erreur : %s\n", err.c_str());
+    usage();
+    return 1;
+  }
+  std::printf("=== Cavité différentiellement chauffée 2D (LBM D2Q9 + Smagorinsky, MI355X) ===\n");
+  std::printf("Grille : %zux%zu (Az = %.2f) | Ra = %.3e | Pr = %g | Cs = %g | Pr_t = %g | jobs = %zu\n",
+              a.nx, a.ny, (double)(a.ny - 1) / (double)(a.nx - 1), a.ra, a.pr, a.cs, a.pr_t, a.jobs);
+  try {
+    cavlbm::CavitySolver sim(a.nx, a.ny, a.ra, a.pr, a.cs, a.pr_t, a.utarget, a.seed, a.jobs,
+                             a.alpha_mult, a.nu_start_mult, a.nu, a.storage, a.geq_order,
+                             a.device);
+    if (a.dynamic_smag) sim.set_dynamic_sgs(true);
+    if (!a.restart.empty()) {  // main.rs:104-128
+      uint64_t snx = 0, sny = 0;
+      if (cavlbm_snapshot_read(a.restart.c_str(), &snx, &sny, nullptr, nullptr, nullptr, nullptr, 0)) {
+        std::fprintf(stderr, "Erreur de lecture du snapshot %s : %s\n", a.restart.c_str(),
+                     cavlbm_snapshot_last_error());
+        return 6;
+      }
+      const size_t ns = snx * sny, n = a.nx * a.ny;
+      std::vector<double> f[4], g[4];
+      for (int q = 0; q < 4; ++q) {
+        f[q].resize(ns);
+        g[q].resize(n);
+      }
+      if (cavlbm_snapshot_read(a.restart.c_str(), &snx, &sny, f[0].data(), f[1].data(),
+                               f[2].data(), f[3].data(), ns)) {
+        std::fprintf(stderr, "Erreur de lecture du snapshot %s : %s\n", a.restart.c_str(),
+                     cavlbm_snapshot_last_error());
+        return 6;
+      }
+      std::printf("Restart depuis %s : snapshot %llux%llu → grille %zux%zu%s\n", a.restart.c_str(),
+                  (unsigned long long)snx, (unsigned long long)sny, a.nx, a.ny,
+                  (snx == a.nx && sny == a.ny) ? "" : " (interpolation bilinéaire)");
+      for (int q = 0; q < 4; ++q)
+        cavlbm_interpolate_field(f[q].data(), snx, sny, g[q].data(), a.nx, a.ny);
+      sim.init_from_fields(g[0], g[1], g[2], g[3]);
+    }
+    const cavlbm_derived d = sim.derived();
+    std::printf("Unités réseau : nu = %.6f | alpha = %.6f | tau = %.4f | tau_t = %.4f | beta_eff = %s\n",
+                d.nu, d.alpha, d.tau, d.tau_t, rust_exp(d.beta_eff, 4).c_str());
+
+    const size_t n_nodes = a.nx * a.ny;
+    std::vector<std::string> log_lines;
+    const std::string snap_path = a.out + "/snapshot.bin";
+    if (a.snapshot_every > 0 && !make_dirs(a.out)) {
+      std::fprintf(stderr, "Impossible de créer %s\n", a.out.c_str());
+      return 4;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t done = 0;
+    auto is_progress = [&](size_t s) { return s % a.progress_every == 0 || s + 1 == a.steps; };
+    auto is_sample = [&](size_t s) {
+      return s >= a.sample_start && (s - a.sample_start) % a.sample_every == 0;
+    };
+    auto is_snap = [&](size_t s) { return a.snapshot_every > 0 && (s + 1) % a.snapshot_every == 0; };
+    for (size_t step = 0; step < a.steps; ++step) {
+      if (!(is_progress(step) || is_sample(step) || is_snap(step))) continue;
+      sim.run(step + 1 - done);  // steps done..step issued back to back on the device
+      done = step + 1;
+      if (is_progress(step)) {  // main.rs:168-276
+        const cavlbm_diag dg = sim.diagnostics();
+        char line[256];
+        std::snprintf(line, sizeof line,
+                      "step %7zu | w_max = %.5f | max_u = %.5f | Nuc = %.4f | Nuf = %.4f | "
+                      "masse = %.6f | nu_t_max = %s",
+                      step, dg.max_uy, dg.max_u, dg.nusselt_hot, dg.nusselt_cold,
+                      dg.total_mass / (double)n_nodes, rust_exp(dg.nu_t_max, 3).c_str());
+        std::printf("%s\n", line);
+        log_lines.push_back(line);
+        if (dg.max_u > kMaxUAbort) {
+          char m[256];
+          std::snprintf(m, sizeof m,
+                        "AVORT : max_u = %.5f dépasse le garde-fou %.2f (Ma_local = max_u·√3 ≈ "
+                        "%.3f > marge de sécurité)",
+                        dg.max_u, kMaxUAbort, dg.max_u * std::sqrt(3.0));
+          std::fprintf(stderr, "%s\n", m);
+          log_lines.push_back(m);
+          make_dirs(a.out);
+          std::string text;
+          for (auto& l : log_lines) text += l + "\n";
+          write_text(a.out + "/progress.log", text);
+          return 2;
+        }
+        if (!dg.all_finite) {
+          std::fprintf(stderr, "DIVERGENCE détectée au pas %zu (valeurs non finies).\n", step);
+          return 2;
+        }
+      }
+      if (is_sample(step)) sim.average_accumulate();
+      if (is_snap(step)) {  // main.rs:284-300
+        const auto rho = sim.rho(), ux = sim.ux(), uy = sim.uy(), t = sim.temperature();
+        if (cavlbm_snapshot_write_atomic(snap_path.c_str(), a.nx, a.ny, rho.data(), ux.data(),
+                                         uy.data(), t.data())) {
+          std::fprintf(stderr, "Erreur d'export du snapshot au pas %zu : %s\n", step,
+                       cavlbm_snapshot_last_error());
+          return 5;
+        }
+        std::printf("snapshot exporté au pas %zu\n", step);
+      }
+    }
+    if (done < a.steps) sim.run(a.steps - done);
+    const double el =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Simulation terminée en %.1f s (%.1fk pas/s)\n", el, (double)a.steps / el / 1000.0);
+    if (sim.average_count() == 0) {
+      std::fprintf(stderr, "Aucun échantillon moyenné : sample_start ≥ steps ?\n");
+      return 3;
+    }
+    std::printf("Moyennage sur %llu échantillons.\n", (unsigned long long)sim.average_count());
+    std::vector<double> ux_m, uy_m, t_m, nut_m, uu_m;
+    sim.average_means(ux_m, uy_m, t_m, nut_m, uu_m);
+    const double nuc = nusselt_from_mean(t_m, a.nx, a.ny, d.lz, true);
+    const double nuf = nusselt_from_mean(t_m, a.nx, a.ny, d.lz, false);
+    std::vector<double> mid(a.nx), q75(a.nx);
+    for (size_t ix = 0; ix < a.nx; ++ix) {
+      mid[ix] = uy_m[(a.ny / 2) * a.nx + ix];
+      q75[ix] = uy_m[(3 * a.ny / 4) * a.nx + ix];
+    }
+    const auto wm = max_of(mid, a.nx), w75 = max_of(q75, a.nx);
+    const cavlbm_diag dg = sim.diagnostics();
+    double s2 = 0.0;
+    for (size_t k = 0; k < n_nodes; ++k)
+      s2 += std::fmax(uu_m[k] - ux_m[k] * ux_m[k] - uy_m[k] * uy_m[k], 0.0);
+    const double s_half = std::sqrt(s2 / (double)n_nodes);
+    if (!make_dirs(a.out)) {
+      std::fprintf(stderr, "Impossible de créer %s\n", a.out.c_str());
+      return 4;
+    }
+    const auto rho = sim.rho(), ux = sim.ux(), uy = sim.uy(), t = sim.temperature();
+    char js[2048];
+    std::snprintf(js, sizeof js,
+                  "{\n  \"case\": \"differentially_heated_cavity_2d\",\n  \"ra\": %s,\n"
+                  "  \"cs\": %g,\n  \"grid\": [{\"nx\": %zu, \"ny\": %zu}],\n"
+                  "  \"aspect_ratio\": %.4f,\n  \"steps\": %zu,\n  \"time_samples\": %llu,\n"
+                  "  \"nusselt_hot\": %.4f,\n  \"nusselt_cold\": %.4f,\n  \"nusselt_volume\": %.4f,\n"
+                  "  \"wmax_z05\": %.5f,\n  \"x_wmax_z05\": %.5f,\n  \"wmax_z075\": %.5f,\n"
+                  "  \"x_wmax_z075\": %.5f,\n  \"y_plus_hot\": %.3f,\n  \"s_half\": %.4f\n}\n",
+                  rust_exp(a.ra, 6).c_str(), a.cs, a.nx, a.ny,
+                  (double)(a.ny - 1) / (double)(a.nx - 1), a.steps,
+                  (unsigned long long)sim.average_count(), nuc, nuf, dg.nusselt_volume, wm.first,
+                  wm.second, w75.first, w75.second, dg.y_plus_hot, s_half);
+    std::string text;
+    for (auto& l : log_lines) text += l + "\n";
+    if (cavlbm_snapshot_write(snap_path.c_str(), a.nx, a.ny, rho.data(), ux.data(), uy.data(),
+                              t.data()) ||
+        !write_text(a.out + "/summary.json", js) || !write_text(a.out + "/progress.log", text)) {
+      std::fprintf(stderr, "Erreur d'écriture dans %s\n", a.out.c_str());
+      return 5;
+    }
+    std::printf("--- Résultats (champs moyens) ---\n");
+    std::printf("Nuc = %.4f | Nuf = %.4f\n", nuc, nuf);
+    std::printf("w_max(z=0.5) = %.5f en x = %.5f | w_max(z=0.75) = %.5f en x = %.5f\n", wm.first,
+                wm.second, w75.first, w75.second);
+    std::printf("y+ (paroi chaude) = %.3f | S^(1/2) = %.4f\n", dg.y_plus_hot, s_half);
+    std::printf("Artefacts écrits dans %s\n", a.out.c_str());
+  } catch (const std::invalid_argument& e) {
+    std::fprintf(stderr, "Erreur : %s\n", e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "Erreur : %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -56,6 +56,8 @@ struct cavlbm_solver {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer parity
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
+  uint64_t avg_n = 0;
   std::string err;
 };
 
@@ -220,6 +222,7 @@ void free_all(cavlbm_solver* s) {
   f(s->theta);
   f(s->red_part);
   f(s->red_out);
+  f(s->avg);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -710,6 +713,55 @@ int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms
 
 const char* cavlbm_last_error(const cavlbm_solver* s) {
   return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int cavlbm_average_reset(cavlbm_solver* s) {
+  if (!s) return CAVLBM_E_PARAM;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  const size_t bytes = 5 * (size_t)s->geo.plane * sizeof(double);
+  if (!s->avg) HIP_TRY(s, hipMalloc((void**)&s->avg, bytes));
+  HIP_TRY(s, hipMemsetAsync(s->avg, 0, bytes, s->st));
+  s->avg_n = 0;
+  return 0;
+}
+
+int cavlbm_average_accumulate(cavlbm_solver* s) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (!s->avg) {
+    int rc = cavlbm_average_reset(s);
+    if (rc) return rc;
+  }
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  int rc = materialize(s);
+  if (rc) return rc;
+  HIP_TRY(s, launch_avg_accumulate(s->geo, s->storage, s->fld[1], s->fld[2], s->fld[3], s->nut,
+                                   s->avg, s->st));
+  s->avg_n += 1;
+  return 0;
+}
+
+int cavlbm_average_count(const cavlbm_solver* s, uint64_t* count) {
+  if (!s || !count) return CAVLBM_E_PARAM;
+  *count = s->avg_n;
+  return 0;
+}
+
+int cavlbm_average_means(cavlbm_solver* s, double* ux, double* uy, double* t, double* nu_t,
+                         double* uu, uint64_t n) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (n != s->p.nx * s->p.ny) return fail(s, CAVLBM_E_SIZE, "average_means: size mismatch");
+  if (!s->avg || s->avg_n == 0) return fail(s, CAVLBM_E_PARAM, "no samples accumulated");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  // TimeAverager::means (post.rs:52-56): v * (1/n)
+  const double inv = 1.0 / (double)s->avg_n;
+  double* outs[5] = {ux, uy, t, nu_t, uu};
+  for (int q = 0; q < 5; ++q) {
+    if (!outs[q]) continue;
+    int rc = field_down(s, s->avg + (size_t)q * s->geo.plane, outs[q]);
+    if (rc) return rc;
+    for (uint64_t k = 0; k < n; ++k) outs[q][k] *= inv;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -254,9 +254,39 @@ __global__ __launch_bounds__(kBlock) void k_convert(Geom geo, const R* src_r, do
     dst_r[k] = (R)src_d[k];
 }
 
+// TimeAverager::accumulate (src/uniform/post.rs:37-49): per-node FP64 sums of
+// ux, uy, T, nu_t and ux*ux + uy*uy (5 planes).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_avg_acc(Geom geo, const double* ux, const double* uy,
+                                                     const double* temp, const R* nut,
+                                                     double* sums) {
+  int ix, iy;
+  if (!node_of_thread(geo, ix, iy)) return;
+  const long long k = (long long)iy * geo.pitch + ix;
+  const long long P = geo.plane;
+  const double u = ux[k], v = uy[k];
+  sums[k] += u;
+  sums[P + k] += v;
+  sums[2 * P + k] += temp[k];
+  sums[3 * P + k] += (double)nut[k];
+  sums[4 * P + k] += u * u + v * v;
+}
+
 }  // namespace
 
 int diag_blocks(const Geom& geo) { return nblocks_of(geo); }
+
+hipError_t launch_avg_accumulate(const Geom& geo, int storage, const double* ux, const double* uy,
+                                 const double* temp, const void* nut, double* sums,
+                                 hipStream_t st) {
+  if (storage == ST_FP64)
+    hipLaunchKernelGGL(k_avg_acc<double>, dim3(nblocks_of(geo)), dim3(kBlock), 0, st, geo, ux, uy,
+                       temp, static_cast<const double*>(nut), sums);
+  else
+    hipLaunchKernelGGL(k_avg_acc<float>, dim3(nblocks_of(geo)), dim3(kBlock), 0, st, geo, ux, uy,
+                       temp, static_cast<const float*>(nut), sums);
+  return hipGetLastError();
+}
 
 hipError_t launch_init_eq(const Geom& geo, int geq, double* mpops, const double* rho,
                           double* ux, double* uy, const double* temp, hipStream_t st) {

@@ -43,6 +43,10 @@ hipError_t launch_diagnostics(const Geom& geo, int storage, const double* mpops,
                               const double* rho, const double* ux, const double* uy,
                               const double* temp, const void* nut, double nu,
                               double* partials, double* out, hipStream_t st);
+// TimeAverager sums: 5 FP64 planes (ux, uy, T, nu_t, ux^2 + uy^2).
+hipError_t launch_avg_accumulate(const Geom& geo, int storage, const double* ux, const double* uy,
+                                 const double* temp, const void* nut, double* sums,
+                                 hipStream_t st);
 // Type conversion of an R-typed per-node field (fy_node / nu_t) to/from FP64.
 hipError_t launch_field_convert(const Geom& geo, int storage, const void* src_r,
                                 double* dst_d, int to_double, void* dst_r,

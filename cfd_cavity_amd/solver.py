@@ -327,6 +327,25 @@ class CavitySolver:
         self._check(self._L.cavlbm_max_uy_on_row(self._h, int(iy), C.byref(w), C.byref(x)))
         return w.value, x.value
 
+    # -- time averaging (TimeAverager, src/uniform/post.rs:11-57) ------------
+    def average_reset(self):
+        self._check(self._L.cavlbm_average_reset(self._h))
+
+    def average_accumulate(self):
+        self._check(self._L.cavlbm_average_accumulate(self._h))
+
+    def average_count(self):
+        v = C.c_uint64()
+        self._check(self._L.cavlbm_average_count(self._h, C.byref(v)))
+        return v.value
+
+    def average_means(self):
+        """(ux, uy, T, nu_t, u^2) means, like TimeAverager::means."""
+        n = self._nx * self._ny
+        out = [np.empty(n) for _ in range(5)]
+        self._check(self._L.cavlbm_average_means(self._h, *[_dp(a) for a in out], n))
+        return tuple(out)
+
     # -- timing helpers -----------------------------------------------------
     def time_run(self, nsteps):
         ms = C.c_double()

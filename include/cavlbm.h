@@ -37,7 +37,8 @@ enum {
   CAVLBM_E_SIZE = -2,        /* array length mismatch (solver.rs:236-240) */
   CAVLBM_E_HIP = -3,         /* HIP runtime failure */
   CAVLBM_E_UNSUPPORTED = -4, /* feature not on this path (dynamic SGS) */
-  CAVLBM_E_NODEVICE = -5     /* no usable gfx950 device */
+  CAVLBM_E_NODEVICE = -5,    /* no usable gfx950 device */
+  CAVLBM_E_IO = -6           /* snapshot file error (the reference exits 6) */
 };
 
 /* Population storage in HBM.  Arithmetic is FP64 for FP64 storage and FP32
@@ -173,6 +174,35 @@ int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
  * step kernels with its own event pair and returns the mean kernel time. */
 int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
 int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
+
+/* ---- time averaging and snapshots (the run loop's helpers) ---------------
+ * TimeAverager (post.rs:11-57) on the device: FP64 sums of ux, uy, T, nu_t and
+ * ux^2 + uy^2 per node; means = sums * (1/count). */
+int cavlbm_average_reset(cavlbm_solver *s);
+int cavlbm_average_accumulate(cavlbm_solver *s);
+int cavlbm_average_count(const cavlbm_solver *s, uint64_t *count);
+int cavlbm_average_means(cavlbm_solver *s, double *ux, double *uy, double *t,
+                         double *nu_t, double *uu, uint64_t n);
+
+/* CAVSNAP1 files (snapshot.rs:10-95): magic, nx, ny (u64 LE), then rho, ux,
+ * uy, T (f64 LE, k = iy*nx + ix).  Host-side I/O; errors are CAVLBM_E_IO with
+ * the message in cavlbm_snapshot_last_error(). */
+int cavlbm_snapshot_write(const char *path, uint64_t nx, uint64_t ny,
+                          const double *rho, const double *ux,
+                          const double *uy, const double *t);
+/* Temporary file + rename, so the destination is never left half written. */
+int cavlbm_snapshot_write_atomic(const char *path, uint64_t nx, uint64_t ny,
+                                 const double *rho, const double *ux,
+                                 const double *uy, const double *t);
+/* With null field pointers only nx, ny are returned; otherwise each buffer
+ * holds capacity >= nx*ny values. */
+int cavlbm_snapshot_read(const char *path, uint64_t *nx, uint64_t *ny,
+                         double *rho, double *ux, double *uy, double *t,
+                         uint64_t capacity);
+/* interpolate_field (snapshot.rs:100-131): bilinear, normalised coordinates. */
+int cavlbm_interpolate_field(const double *src, uint64_t nx0, uint64_t ny0,
+                             double *dst, uint64_t nx1, uint64_t ny1);
+const char *cavlbm_snapshot_last_error(void);
 
 /* ---- differentiation (FP64 storage only) -------------------------------
  * Objective over a window of K steps starting from the solver's current

@@ -1,0 +1,79 @@
+"""The uniform-mode driver (cfd_cavity_amd/cavity_les, C++ over the C-ABI):
+flag validation and exit codes of src/cli.rs:334-383 and src/main.rs (CPU),
+and a short GPU run with averaging, snapshots, restart and the abort guard."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "cfd_cavity_amd", "cavity_les")
+
+
+def run(*args, timeout=120):
+    return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.parametrize("args,needle", [
+    (["--nx", "4"], "≥ 8"),
+    (["--nx", "64", "--ny", "64"], "ny doit être > nx"),
+    (["--ra", "0"], "ra et pr"),
+    (["--sample-every", "0"], "sample-every"),
+    (["--jobs", "0"], "jobs"),
+    (["--alpha-mult", "-1"], "alpha-mult"),
+    (["--nu-start-mult", "0.5"], "nu-start-mult"),
+    (["--nu", "0.01"], "0.002"),
+    (["--nx"], "manquante"),
+    (["--bogus", "1"], "inconnue"),
+    (["--mode", "octree"], "octree"),
+])
+def test_cli_validation_exit_1(args, needle):
+    r = run(*args)
+    assert r.returncode == 1 and needle in r.stderr
+
+
+def test_cli_help():
+    r = run("--help")
+    assert r.returncode == 0 and "Usage" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_run_averaging_snapshot_and_log(tmp_path):
+    from cfd_cavity_amd import CavitySolver, snapshot
+
+    out = tmp_path / "run"
+    r = run("--nx", 32, "--ny", 128, "--ra", 1e6, "--steps", 2500, "--sample-start", 1000,
+            "--sample-every", 100, "--progress-every", 1000, "--snapshot-every", 1000,
+            "--out", out)
+    assert r.returncode == 0, r.stderr
+    lines = (out / "progress.log").read_text().splitlines()
+    assert [ln.split("|")[0].strip() for ln in lines] == ["step       0", "step    1000",
+                                                         "step    2000", "step    2499"]
+    summ = json.loads((out / "summary.json").read_text())
+    assert summ["time_samples"] == 15 and summ["steps"] == 2500
+    nx, ny, rho, ux, uy, t = snapshot.read_snapshot(out / "snapshot.bin")
+    # the driver's final state equals the library stepped the same number of steps
+    s = CavitySolver(32, 128, 1e6, 0.71, 0.1, 0.9, 0.02, 42, 1)
+    s.run(2500)
+    assert (nx, ny) == (32, 128) and np.array_equal(t, s.get_field("temperature"))
+    d = s.diagnostics()
+    assert f"Nuc = {d['nusselt_hot']:.4f}" in lines[-1]
+
+
+@pytest.mark.gpu
+def test_cli_restart_errors_and_abort(tmp_path):
+    from cfd_cavity_amd import snapshot
+
+    r = run("--restart", tmp_path / "missing.bin", "--steps", 10, "--out", tmp_path / "o")
+    assert r.returncode == 6
+    nx, ny = 32, 128
+    n = nx * ny
+    ux = np.full(n, 0.2)
+    t = np.array([0.5 - (k % nx) / (nx - 1) for k in range(n)])
+    path = tmp_path / "fast.bin"
+    snapshot.write_snapshot(path, nx, ny, np.ones(n), ux, np.zeros(n), t)
+    r = run("--nx", nx, "--ny", ny, "--restart", path, "--steps", 100, "--out", tmp_path / "a")
+    assert r.returncode == 2 and "AVORT" in r.stderr
+    assert "AVORT" in (tmp_path / "a" / "progress.log").read_text()

@@ -306,3 +306,46 @@ def test_thermal_wall_link_conductance(oracle):    # tests.rs:602-643
     o = oracle.OracleSolver(21, 21, 1.0, 1.0, 0.0, 1.0, 0.02, 42, 1, nu_override=0.00169)
     o.run(20_000)
     assert bits_equal(s.lbm.temperature, o.temperature)
+
+
+# ---------------------------------------------------------------------------
+# run-loop helpers: time averaging on the device, snapshot restart
+
+def test_time_averager_matches_host_sums(oracle):      # post.rs:11-57
+    g, o = pair(oracle)
+    sums = [np.zeros(SMALL["nx"] * SMALL["ny"]) for _ in range(5)]
+    g.average_reset()
+    for step in range(1, 301):
+        g.step()
+        o.step()
+        if step >= 100 and step % 20 == 0:
+            g.average_accumulate()
+            ux, uy = o.ux.copy(), o.uy.copy()
+            for a, v in zip(sums, (ux, uy, o.temperature, o.nu_t, ux * ux + uy * uy)):
+                a += v
+    n = g.average_count()
+    assert n == 11
+    inv = 1.0 / n
+    for got, s in zip(g.average_means(), sums):
+        assert bits_equal(got, s * inv)
+
+
+def test_snapshot_restart_regrid_matches_oracle(oracle, tmp_path):   # main.rs:104-128
+    from cfd_cavity_amd import snapshot
+
+    g, o = pair(oracle)
+    g.run(150)
+    path = tmp_path / "snap.bin"
+    snapshot.save(g, path)
+    # restart on a finer grid: interpolate, init_from_fields, both backends
+    big = dict(SMALL, nx=24, ny=60)
+    g2, o2 = pair(oracle, big)
+    snapshot.restart(g2, path)
+    snx, sny, rho, ux, uy, t = snapshot.read_snapshot(path)
+    assert (snx, sny) == (16, 40)
+    f = [snapshot.interpolate_field(a, snx, sny, 24, 60) for a in (rho, ux, uy, t)]
+    o2.init_from_fields(*f)
+    assert g2.time_step == 0
+    g2.run(40)
+    o2.run(40)
+    assert_state_bitwise(g2, o2, "after regrid restart + 40")
