@@ -73,6 +73,8 @@ SIGNATURES = {
     "cavlbm_set_time_step": (C.c_int, [_vp, _u64]),
     "cavlbm_set_hot_wall": (C.c_int, [_vp, _dp, _u64]),
     "cavlbm_set_dynamic_sgs": (C.c_int, [_vp, _i32, _d, _d, _d]),
+    "cavlbm_dynamic_stats": (C.c_int, [_vp, _dp]),
+    "cavlbm_get_sgs_pre": (C.c_int, [_vp, _i32, _dp, _u64]),
     "cavlbm_set_jobs": (C.c_int, [_u64]),
     "cavlbm_phase": (C.c_int, [_vp, _i32, _u64]),
     "cavlbm_time_run": (C.c_int, [_vp, _u64, _dp]),

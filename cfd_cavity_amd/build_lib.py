@@ -30,6 +30,7 @@ UNITS = {
     "lbm_aux.hip": ["-ffp-contract=off"],
     "lbm_adjoint.hip": ["-ffp-contract=off"],
     "lbm_tangent.hip": ["-ffp-contract=off"],
+    "lbm_dynsgs.hip": ["-ffp-contract=off"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
     "snapshot.cpp": ["-ffp-contract=off"],
 }

@@ -12,6 +12,7 @@
 // Extra flags: --storage fp64|fp32|fp16_offset, --geq-order 1|2, --device N.
 #include <sys/stat.h>
 
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -37,6 +38,7 @@ struct Args {
   double nu = -1.0;
   size_t snapshot_every = 0;
   bool dynamic_smag = false;
+  double smag_alpha = 0.5, nu_sm_min = -0.1, eta_test = 2.0;
   int storage = CAVLBM_STORAGE_FP64, geq_order = 2, device = 0;
 };
 
@@ -47,8 +49,8 @@ void usage() {
       "Usage: cavity_les [OPTIONS]   (uniform mode of bbrangeo/cfd_cavity on an MI355X)\n"
       "  --nx --ny --ra --pr --cs --pr_t --steps --sample-start --sample-every --out\n"
       "  --utarget --seed --progress-every --jobs --alpha-mult --nu-start-mult --restart\n"
-      "  --nu --snapshot-every --dynamic-smag --storage fp64|fp32|fp16_offset --geq-order 1|2\n"
-      "  --device N\n");
+      "  --nu --snapshot-every --dynamic-smag --smag-alpha --nu-sm-min --eta-test\n"
+      "  --storage fp64|fp32|fp16_offset --geq-order 1|2 --device N\n");
 }
 
 bool parse_size(const std::string& v, size_t& out) {
@@ -136,10 +138,10 @@ std::string parse(int argc, char** argv, Args& a, bool& help) {
       size_t d = 0;
       ok = parse_size(val, d);
       a.device = (int)d;
-    } else if (key == "smag-alpha" || key == "nu-sm-min" || key == "eta-test") {
-      double x;
-      ok = parse_f64(val, x);
-    } else {
+    } else if (key == "smag-alpha") ok = parse_f64(val, a.smag_alpha);
+    else if (key == "nu-sm-min") ok = parse_f64(val, a.nu_sm_min);
+    else if (key == "eta-test") ok = parse_f64(val, a.eta_test);
+    else {
       return "option inconnue : --" + key;
     }
     if (!ok) return "--" + key + " : valeur invalide '" + val + "'";
@@ -153,6 +155,10 @@ std::string parse(int argc, char** argv, Args& a, bool& help) {
   if (!(a.alpha_mult > 0.0) || !std::isfinite(a.alpha_mult)) return "alpha-mult doit être > 0 et fini";
   if (!(a.nu_start_mult >= 1.0) || !std::isfinite(a.nu_start_mult))
     return "nu-start-mult doit être ≥ 1 et fini";
+  if (!(a.smag_alpha >= 0.0 && a.smag_alpha <= 1.0) || !std::isfinite(a.smag_alpha))
+    return "smag-alpha doit être ∈ [0, 1] (0 = Smagorinsky pur, 1 = similarité pure)";
+  if (!(a.eta_test >= 1.0) || !std::isfinite(a.eta_test))
+    return "eta-test doit être ≥ 1 (ratio filtre test/grille)";
   if (a.progress_every == 0) a.progress_every = 1;
   return "";
 }
@@ -166,6 +172,36 @@ std::string rust_exp(double v, int prec) {
   if (e == std::string::npos) return s;
   const int ex = std::atoi(s.c_str() + e + 1);
   return s.substr(0, e) + "e" + std::to_string(ex);
+}
+
+// Rust's {} for f64: the shortest decimal that reads back to the same value,
+// positional (no exponent), without a trailing ".0".
+std::string rust_display(double v) {
+  if (!std::isfinite(v)) return std::isnan(v) ? "NaN" : (v > 0 ? "inf" : "-inf");
+  if (v == 0.0) return std::signbit(v) ? "-0" : "0";
+  char buf[64];
+  int prec = 1;
+  for (; prec <= 17; ++prec) {
+    std::snprintf(buf, sizeof buf, "%.*e", prec - 1, v);
+    if (std::strtod(buf, nullptr) == v) break;
+  }
+  std::string m(buf);
+  const size_t e = m.find('e');
+  const int ex = std::atoi(m.c_str() + e + 1);
+  std::string digits;
+  for (size_t i = 0; i < e; ++i)
+    if (std::isdigit((unsigned char)m[i])) digits += m[i];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out = v < 0 ? "-" : "";
+  const int point = ex + 1;  // digits before the decimal point
+  if (point <= 0) {
+    out += "0." + std::string(-point, '0') + digits;
+  } else if ((size_t)point >= digits.size()) {
+    out += digits + std::string(point - digits.size(), '0');
+  } else {
+    out += digits.substr(0, point) + "." + digits.substr(point);
+  }
+  return out;
 }
 
 // main.rs:740-753
@@ -233,7 +269,12 @@ int main(int argc, char** argv) {
     cavlbm::CavitySolver sim(a.nx, a.ny, a.ra, a.pr, a.cs, a.pr_t, a.utarget, a.seed, a.jobs,
                              a.alpha_mult, a.nu_start_mult, a.nu, a.storage, a.geq_order,
                              a.device);
-    if (a.dynamic_smag) sim.set_dynamic_sgs(true);
+    // main.rs:94-102
+    sim.set_dynamic_sgs(a.dynamic_smag, a.smag_alpha, a.nu_sm_min, a.eta_test);
+    if (a.dynamic_smag)
+      std::printf("LES dynamique : échelles mixtes α = %s (Tab. 2.3) | η_test = %s | νsm_min = %s·ν\n",
+                  rust_display(a.smag_alpha).c_str(), rust_display(a.eta_test).c_str(),
+                  rust_display(a.nu_sm_min).c_str());
     if (!a.restart.empty()) {  // main.rs:104-128
       uint64_t snx = 0, sny = 0;
       if (cavlbm_snapshot_read(a.restart.c_str(), &snx, &sny, nullptr, nullptr, nullptr, nullptr, 0)) {
@@ -284,12 +325,20 @@ int main(int argc, char** argv) {
       done = step + 1;
       if (is_progress(step)) {  // main.rs:168-276
         const cavlbm_diag dg = sim.diagnostics();
-        char line[256];
+        char line[384];
+        char dyn[160] = "";
+        if (a.dynamic_smag) {  // main.rs:179-187
+          double st[4];
+          cavlbm_dynamic_stats(sim.raw(), st);
+          std::snprintf(dyn, sizeof dyn,
+                        " | cs_dyn_mean = %.4f | cs_dyn_max = %.4f | kap_dyn_mean = %.4f", st[0],
+                        st[1], st[2]);
+        }
         std::snprintf(line, sizeof line,
                       "step %7zu | w_max = %.5f | max_u = %.5f | Nuc = %.4f | Nuf = %.4f | "
-                      "masse = %.6f | nu_t_max = %s",
+                      "masse = %.6f | nu_t_max = %s%s",
                       step, dg.max_uy, dg.max_u, dg.nusselt_hot, dg.nusselt_cold,
-                      dg.total_mass / (double)n_nodes, rust_exp(dg.nu_t_max, 3).c_str());
+                      dg.total_mass / (double)n_nodes, rust_exp(dg.nu_t_max, 3).c_str(), dyn);
         std::printf("%s\n", line);
         log_lines.push_back(line);
         if (dg.max_u > kMaxUAbort) {
@@ -371,6 +420,28 @@ int main(int argc, char** argv) {
         !write_text(a.out + "/summary.json", js) || !write_text(a.out + "/progress.log", text)) {
       std::fprintf(stderr, "Erreur d'écriture dans %s\n", a.out.c_str());
       return 5;
+    }
+    if (a.dynamic_smag) {  // main.rs:393-424, a companion file beside summary.json
+      double st[4];
+      cavlbm_dynamic_stats(sim.raw(), st);
+      double nsum = 0.0, nmax = 0.0;
+      for (double v : nut_m) {
+        nsum += v;
+        nmax = std::fmax(nmax, std::fabs(v));
+      }
+      char dj[1024];
+      std::snprintf(dj, sizeof dj,
+                    "{\n  \"dynamic_smag\": true,\n  \"smag_alpha\": %s,\n  \"eta_test\": %s,\n"
+                    "  \"nu_sm_min\": %s,\n  \"cs_dyn_mean\": %.6f,\n  \"cs_dyn_max\": %.6f,\n"
+                    "  \"kappa_dyn_mean\": %.6f,\n  \"kappa_dyn_max\": %.6f,\n"
+                    "  \"nu_t_mean\": %s,\n  \"nu_t_max\": %s\n}",
+                    rust_display(a.smag_alpha).c_str(), rust_display(a.eta_test).c_str(),
+                    rust_display(a.nu_sm_min).c_str(), st[0], st[1], st[2], st[3],
+                    rust_exp(nsum / (double)nut_m.size(), 6).c_str(), rust_exp(nmax, 6).c_str());
+      if (!write_text(a.out + "/dynamic_sgs.json", dj))
+        std::fprintf(stderr, "Warning : dynamic_sgs.json non écrit\n");
+      std::printf("LES dynamique : cs_dyn_mean = %.4f | cs_dyn_max = %.4f (clamp [0, 0.04])\n",
+                  st[0], st[1]);
     }
     std::printf("--- Résultats (champs moyens) ---\n");
     std::printf("Nuc = %.4f | Nuf = %.4f\n", nuc, nuf);

@@ -58,6 +58,14 @@ struct cavlbm_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
   uint64_t avg_n = 0;
+  bool in_batch = false;  // owned by a cavlbm_batch (stepped by the batched kernel)
+  // Dynamic mixed-scale SGS (solver.rs:76-103).  dyn_on: compute nu_t/alpha_t
+  // before every collision; dyn_pre: the per-node values exist and are used by
+  // the collision (they persist, as the reference's Option does, after the
+  // model is switched off).
+  bool dyn_on = false, dyn_pre = false;
+  double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
+  double* dyn = nullptr;  // kDynPlanes FP64 planes, allocated on first enable
   std::string err;
 };
 
@@ -115,6 +123,13 @@ StepBufs bufs(cavlbm_solver* s) {
   b.ux = s->fld[1];
   b.uy = s->fld[2];
   b.temp = s->fld[3];
+  b.nut_pre = nullptr;
+  b.at_pre = nullptr;
+  if (s->dyn) {
+    const size_t P = (size_t)s->geo.plane;
+    b.nut_pre = s->dyn + 18 * P;
+    b.at_pre = s->dyn + 19 * P;
+  }
   return b;
 }
 
@@ -142,9 +157,74 @@ void drop_graphs(cavlbm_solver* s) {
     }
 }
 
+// Scratch layout of the dynamic model, planes of s->dyn: ubx, uby, tb, 5 box
+// products, qsm, phi, 4 Lilly terms, nu_mix, kappa_mix, C_nu, C_kappa,
+// nu_t_pre, alpha_t_pre (indices 0..19).
+DynSgsArgs dyn_args(const cavlbm_solver* s, const StepCtx& ctx) {
+  const size_t P = (size_t)s->geo.plane;
+  DynSgsArgs a{};
+  a.mpops = s->mpops;
+  a.rho = s->fld[0];
+  a.ux = s->fld[1];
+  a.uy = s->fld[2];
+  a.temp = s->fld[3];
+  a.tau = ctx.tau;
+  // compute_dynamic_sgs (solver.rs:573-582): eta = max(eta_test, 1), width =
+  // round(eta) nodes.  Every width from 2*max(nx, ny) + 1 on averages the whole
+  // domain, so larger ones are capped there.
+  const double eta = std::fmax(s->dyn_eta, 1.0);
+  const double cap = 2.0 * (double)std::max(s->p.nx, s->p.ny) + 1.0;
+  const double wr = std::round(eta);
+  a.width = (int)(wr < cap ? wr : cap);
+  if (a.width < 1) a.width = 1;
+  a.half_left = (a.width - 1) / 2;
+  a.dtest = eta * 1.0;
+  a.alpha_exp = s->dyn_alpha;
+  a.beta_exp = 1.0 - s->dyn_alpha;
+  const double margin = (kTauTrtFloor - 0.5) / 3.0;
+  a.nu_t_min = std::fmax(s->dyn_min * s->d.nu, margin - s->d.nu);
+  a.alpha_t_min = std::fmax(s->dyn_min * s->d.alpha, margin - s->d.alpha);
+  double* q = s->dyn;
+  a.ubx = q + 0 * P;
+  a.uby = q + 1 * P;
+  a.tb = q + 2 * P;
+  a.lb = q + 3 * P;
+  a.qsm = q + 8 * P;
+  a.phi = q + 9 * P;
+  a.terms = q + 10 * P;
+  a.nu_mix_g = q + 14 * P;
+  a.kap_mix_g = q + 15 * P;
+  a.cnu = q + 16 * P;
+  a.ck = q + 17 * P;
+  a.nut_pre = q + 18 * P;
+  a.at_pre = q + 19 * P;
+  return a;
+}
+
+// A step with the per-node SGS values (solver.rs:279-293 with
+// compute_dynamic_sgs first): the model reads the post-stream state, so the
+// M-form is made current, the model runs when enabled, and the collision
+// takes nu_t / alpha_t from it.
+int dyn_step(cavlbm_solver* s, int write_aux) {
+  int rc = materialize(s);
+  if (rc) return rc;
+  const StepCtx ctx = make_ctx(s, s->time_step);
+  if (s->dyn_on) {
+    HIP_TRY(s, launch_dynamic_sgs(s->geo, dyn_args(s, ctx), s->st));
+    s->dyn_pre = true;
+  }
+  HIP_TRY(s, launch(s, MODE_FIRST_PRE, ctx, write_aux));
+  s->cur ^= 1;
+  s->c_valid = true;
+  s->m_valid = false;
+  s->time_step += 1;
+  return 0;
+}
+
 // One step: collide(time_step) on the current state, stream/BC/moments/walls
 // deferred to the next kernel.  Advances the C-form buffers.
 int one_step(cavlbm_solver* s, int write_aux) {
+  if (s->dyn_on || s->dyn_pre) return dyn_step(s, write_aux);
   const StepCtx ctx = make_ctx(s, s->time_step);
   HIP_TRY(s, launch(s, s->c_valid ? MODE_FUSED : MODE_FIRST, ctx, write_aux));
   s->cur ^= 1;
@@ -185,8 +265,8 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
     // Steady part: whole graphs of kGraphSteps fused steps (constant per-step
     // scalars after spin-up), keeping the final step direct so that it writes
     // nu_t.
-    if (s->c_valid && !s->graph_failed && s->time_step >= s->d.spin_up_steps &&
-        left > (uint64_t)kGraphSteps) {
+    if (s->c_valid && !s->graph_failed && !s->dyn_on && !s->dyn_pre &&
+        s->time_step >= s->d.spin_up_steps && left > (uint64_t)kGraphSteps) {
       hipGraphExec_t& g = s->graph[s->cur];
       if (!g) g = capture(s, s->cur);
       if (!g) {
@@ -223,6 +303,7 @@ void free_all(cavlbm_solver* s) {
   f(s->red_part);
   f(s->red_out);
   f(s->avg);
+  f(s->dyn);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -630,12 +711,58 @@ int cavlbm_set_hot_wall(cavlbm_solver* s, const double* theta_hot, uint64_t ny) 
   return fill_theta(s, theta_hot);
 }
 
-int cavlbm_set_dynamic_sgs(cavlbm_solver* s, int32_t enable, double, double, double) {
+int cavlbm_set_dynamic_sgs(cavlbm_solver* s, int32_t enable, double smag_alpha,
+                           double nu_sm_min, double eta_test) {
   if (!s) return CAVLBM_E_PARAM;
-  if (enable)
+  if (enable && s->in_batch)
     return fail(s, CAVLBM_E_UNSUPPORTED,
-                "dynamic mixed-scale SGS (--dynamic-smag) is not on the GPU path");
+                "dynamic SGS is not available for cavities of a batch");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  if (enable && !s->dyn) {
+    const size_t bytes = (size_t)kDynPlanes * (size_t)s->geo.plane * sizeof(double);
+    HIP_TRY(s, hipMalloc((void**)&s->dyn, bytes));
+    HIP_TRY(s, hipMemsetAsync(s->dyn, 0, bytes, s->st));
+    HIP_TRY(s, hipStreamSynchronize(s->st));
+  }
+  s->dyn_on = enable != 0;
+  s->dyn_alpha = smag_alpha;
+  s->dyn_min = nu_sm_min;
+  s->dyn_eta = eta_test;
   return 0;
+}
+
+int cavlbm_dynamic_stats(cavlbm_solver* s, double out[4]) {
+  if (!s || !out) return CAVLBM_E_PARAM;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (!s->dyn) return 0;
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  const size_t P = (size_t)s->geo.plane, n = s->p.nx * s->p.ny;
+  std::vector<double> cnu(n), ck(n);
+  int rc = field_down(s, s->dyn + 16 * P, cnu.data());
+  if (!rc) rc = field_down(s, s->dyn + 17 * P, ck.data());
+  if (rc) return rc;
+  // sequential sums and maxima in node order (solver.rs:784-799, :821-824)
+  double cs = 0.0, cm = 0.0, ks = 0.0, km = 0.0;
+  for (size_t k = 0; k < n; ++k) {
+    cs += cnu[k];
+    if (cnu[k] > cm) cm = cnu[k];
+    ks += ck[k];
+    if (ck[k] > km) km = ck[k];
+  }
+  out[0] = cs / (double)n;
+  out[1] = cm;
+  out[2] = ks / (double)n;
+  out[3] = km;
+  return 0;
+}
+
+int cavlbm_get_sgs_pre(cavlbm_solver* s, int32_t which, double* out, uint64_t n) {
+  if (!s || !out || (which != 0 && which != 1)) return CAVLBM_E_PARAM;
+  if (n != s->p.nx * s->p.ny) return fail(s, CAVLBM_E_SIZE, "nx*ny values expected");
+  if (!s->dyn_pre) return fail(s, CAVLBM_E_PARAM, "the dynamic SGS model has not run");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  const size_t P = (size_t)s->geo.plane;
+  return field_down(s, s->dyn + (18 + which) * P, out);
 }
 
 int cavlbm_set_jobs(uint64_t) { return 0; }
@@ -648,6 +775,9 @@ int cavlbm_phase(cavlbm_solver* s, int32_t phase, uint64_t step) {
   const Walls w{s->theta};
   switch (phase) {
     case CAVLBM_PHASE_COLLIDE:
+      if (s->dyn_pre)
+        return fail(s, CAVLBM_E_UNSUPPORTED,
+                    "collide phase alone with dynamic SGS values is not provided");
       HIP_TRY(s, launch(s, MODE_COLLIDE_M, make_ctx(s, step), 1));
       break;
     case CAVLBM_PHASE_STREAM:
@@ -975,6 +1105,7 @@ int cavlbm_batch_create(const cavlbm_params* cavities, uint64_t n, cavlbm_batch*
       cavlbm_batch_destroy(b);
       return rc;  // message already in cavlbm_last_error(NULL)
     }
+    s->in_batch = true;
     b->cav.push_back(s);
     b->max_spin = std::max<uint64_t>(b->max_spin, s->d.spin_up_steps);
     b->nodes += s->p.nx * s->p.ny;
@@ -1217,6 +1348,9 @@ int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_
                  WindowWork& w, double*& theta) {
   if (s->storage != ST_FP64)
     return fail(s, CAVLBM_E_UNSUPPORTED, "window objective and adjoint need FP64 storage");
+  if (s->dyn_on || s->dyn_pre)
+    return fail(s, CAVLBM_E_UNSUPPORTED,
+                "window objective and adjoint are for the fixed-Cs Smagorinsky model");
   if (K == 0) return fail(s, CAVLBM_E_PARAM, "window_steps must be >= 1");
   if (theta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "theta_hot must have ny values");
   HIP_TRY(s, hipSetDevice(s->p.device));

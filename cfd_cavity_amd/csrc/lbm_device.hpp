@@ -47,7 +47,8 @@ enum Mode {
   MODE_FUSED = 0,        // C-form -> stream+BC+moments+walls -> collide -> C-form
   MODE_FIRST = 1,        // M-form pops+fields -> collide -> C-form
   MODE_MATERIALIZE = 2,  // C-form -> stream+BC+moments+walls -> M-form
-  MODE_COLLIDE_M = 3     // M-form pops+fields -> collide -> M-form pops (in place)
+  MODE_COLLIDE_M = 3,    // M-form pops+fields -> collide -> M-form pops (in place)
+  MODE_FIRST_PRE = 4     // MODE_FIRST with per-node nu_t / alpha_t given (dynamic SGS)
 };
 
 struct Geom {
@@ -77,6 +78,8 @@ struct StepBufs {
   double* ux;
   double* uy;
   double* temp;
+  const double* nut_pre;  // MODE_FIRST_PRE: nu_t per node (FP64), else unused
+  const double* at_pre;   // MODE_FIRST_PRE: alpha_t per node (FP64), else unused
 };
 
 constexpr int kTileX = 64;
@@ -225,11 +228,14 @@ __device__ __forceinline__ R geq(int i, R t, R ux, R uy) {
 
 // ---------------------------------------------------------------------------
 // collide_node (solver.rs:1054-1185): TRT Lambda = 3/16 with Smagorinsky nu_t
-// from pi_neq and the Guo odd/even force split for f; BGK for g.
-template <class R, int GEQ>
+// from pi_neq and the Guo odd/even force split for f; BGK for g.  With PRE the
+// turbulent viscosity and diffusivity are the given per-node values of the
+// dynamic model (solver.rs:1118-1139) instead of (Cs dx)^2 |S| and nu_t/Pr_t.
+template <class R, int GEQ, bool PRE = false>
 __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, R uy,
                                              R temp, const StepCtx& c, R cs, R& fy_out,
-                                             R& nut_out) {
+                                             R& nut_out, R nut_pre = R(0.0),
+                                             R at_pre = R(0.0)) {
   using A = Arith<R>;
   const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
   fy_out = fy;
@@ -263,12 +269,12 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
   const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
   const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
-  const R nu_t = cs * cs * s_mag;
+  const R nu_t = PRE ? nut_pre : cs * cs * s_mag;
   nut_out = nu_t;
 
   const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
   const R tau_minus = R(0.5) + A::div(R(3.0 / 16.0), tau_plus - R(0.5));
-  const R alpha_t = A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
+  const R alpha_t = PRE ? at_pre : A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
   const R tau_t_plus = R(0.5) + R(3.0) * (R(c.alpha) + alpha_t);
   // 1/tau (IEEE); 0.5/tau = 0.5*(1/tau) exactly (power-of-two scaling).
   const R itp = A::rcp(tau_plus);
@@ -435,7 +441,11 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   }
 
   R fy_new, nut_new;
-  collide_node<R, GEQ>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new);
+  if (MODE == MODE_FIRST_PRE)
+    collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new,
+                               R(b.nut_pre[k]), R(b.at_pre[k]));
+  else
+    collide_node<R, GEQ>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new);
   static_cast<R*>(b.fy)[k] = fy_new;
   if (write_aux) static_cast<R*>(b.nut)[k] = nut_new;
 

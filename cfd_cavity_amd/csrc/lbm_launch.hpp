@@ -55,6 +55,40 @@ hipError_t launch_field_convert(const Geom& geo, int storage, const void* src_r,
 inline int nblocks_of(const Geom& g) { return g.nbx * g.nby; }
 
 // ---------------------------------------------------------------------------
+// Dynamic mixed-scale SGS (lbm_dynsgs.hip).  All arrays are FP64 pitched
+// planes of the solver's geometry; the inputs are the M-form state.
+constexpr double kCDynMax = 0.04;          // clamp of the Lilly coefficients
+constexpr double kTauTrtFloor = 0.505;     // TRT stability floor of tau+
+constexpr double kDeltaT = kTHot - kTCold; // global temperature difference
+struct DynSgsArgs {
+  const double* mpops;  // post-stream f (planes 0..8 used)
+  const double* rho;
+  const double* ux;
+  const double* uy;
+  const double* temp;
+  double tau;            // molecular tau of this step's collision
+  int width, half_left;  // box filter: width nodes, half_left = (width-1)/2
+  double dtest;          // test-filter width eta * dx
+  double alpha_exp, beta_exp;       // alpha and 1 - alpha
+  double nu_t_min, alpha_t_min;     // floors of nu_t / alpha_t
+  double* ubx;    // 1 plane each
+  double* uby;
+  double* tb;
+  double* lb;     // 5 planes: box of uu, uv, vv, uT, vT
+  double* qsm;
+  double* phi;
+  double* terms;  // 4 planes: L:M, M:M, Lj Mj, Mj Mj
+  double* nu_mix_g;
+  double* kap_mix_g;
+  double* cnu;    // out: C_nu per node
+  double* ck;     // out: C_kappa per node
+  double* nut_pre;  // out: nu_t per node
+  double* at_pre;   // out: alpha_t per node
+};
+constexpr int kDynPlanes = 20;  // planes of scratch + outputs above
+hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, hipStream_t st);
+
+// ---------------------------------------------------------------------------
 // Batched step: many cavities advanced by one launch.  One descriptor per
 // cavity (device memory); `tile_cav` maps every global tile to its cavity.
 struct CavDesc {

@@ -181,8 +181,56 @@ class CavitySolver:
         lib().cavlbm_set_jobs(int(jobs))
 
     def set_dynamic_sgs(self, dynamic_smag, smag_alpha=0.5, nu_sm_min=-0.1, eta_test=2.0):
+        """CavitySolver::set_dynamic_sgs (solver.rs:215-220)."""
         self._check(self._L.cavlbm_set_dynamic_sgs(self._h, 1 if dynamic_smag else 0,
                                                     smag_alpha, nu_sm_min, eta_test))
+        self._dyn = (bool(dynamic_smag), smag_alpha, nu_sm_min, eta_test)
+
+    @property
+    def dynamic_smag(self):
+        return getattr(self, "_dyn", (False,))[0]
+
+    def dynamic_stats(self):
+        """(cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max) of the last
+        dynamic step (solver.rs:821-824); zeros before the first."""
+        out = np.zeros(4)
+        self._check(self._L.cavlbm_dynamic_stats(self._h, _dp(out)))
+        return tuple(float(v) for v in out)
+
+    @property
+    def cs_dyn_mean(self):
+        return self.dynamic_stats()[0]
+
+    @property
+    def cs_dyn_max(self):
+        return self.dynamic_stats()[1]
+
+    @property
+    def kappa_dyn_mean(self):
+        return self.dynamic_stats()[2]
+
+    @property
+    def kappa_dyn_max(self):
+        return self.dynamic_stats()[3]
+
+    def _sgs_pre(self, which):
+        n = self._nx * self._ny
+        out = np.empty(n)
+        rc = self._L.cavlbm_get_sgs_pre(self._h, which, _dp(out), n)
+        if rc == _lib.E_PARAM:
+            return None  # the reference's None before the first dynamic step
+        self._check(rc)
+        return out
+
+    @property
+    def nu_t_pre(self):
+        """Per-node nu_t of the last dynamic step, or None (solver.rs:89)."""
+        return self._sgs_pre(0)
+
+    @property
+    def alpha_t_pre(self):
+        """Per-node alpha_t of the last dynamic step, or None (solver.rs:93)."""
+        return self._sgs_pre(1)
 
     def init_from_fields(self, rho, ux, uy, t):
         n = self._nx * self._ny

@@ -155,10 +155,22 @@ int cavlbm_set_time_step(cavlbm_solver *s, uint64_t t);
  * NULL restores the constant +0.5. */
 int cavlbm_set_hot_wall(cavlbm_solver *s, const double *theta_hot, uint64_t ny);
 
-/* CavitySolver::set_dynamic_sgs (solver.rs:215-220): enable = 0 is accepted,
- * enable != 0 returns CAVLBM_E_UNSUPPORTED (dynamic SGS is not on this path). */
+/* CavitySolver::set_dynamic_sgs (solver.rs:215-220): switches the dynamic
+ * mixed-scale model (compute_dynamic_sgs, solver.rs:562-828) on or off.  When
+ * on, every step first computes nu_t and alpha_t per node from the post-stream
+ * state and the collision uses them; once computed they stay in use after the
+ * model is switched off (the reference keeps its Some(..) arrays).  Returns
+ * CAVLBM_E_UNSUPPORTED for a cavity of a batch. */
 int cavlbm_set_dynamic_sgs(cavlbm_solver *s, int32_t enable, double smag_alpha,
                            double nu_sm_min, double eta_test);
+/* The fields cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max
+ * (solver.rs:95-102, set at :821-824): out[0..3]; zeros before the first
+ * dynamic step. */
+int cavlbm_dynamic_stats(cavlbm_solver *s, double out[4]);
+/* The per-node arrays nu_t_pre (which = 0) or alpha_t_pre (which = 1) of the
+ * last dynamic step (solver.rs:89-93), n = nx*ny values in [iy*nx + ix]
+ * order; CAVLBM_E_PARAM before the model has run. */
+int cavlbm_get_sgs_pre(cavlbm_solver *s, int32_t which, double *out, uint64_t n);
 /* CavitySolver::set_jobs (solver.rs:373-383): accepted and ignored. */
 int cavlbm_set_jobs(uint64_t jobs);
 

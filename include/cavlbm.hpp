@@ -96,7 +96,13 @@ class CavitySolver {
     return {w, x};
   }
   void set_dynamic_sgs(bool on, double a = 0.5, double m = -0.1, double e = 2.0) {
-    check(cavlbm_set_dynamic_sgs(s_, on ? 1 : 0, a, m, e));
+    check(cavlbm_set_dynamic_sgs(s_, on ? 1 : 0, a, m, e));   // solver.rs:215
+  }
+  // cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max (solver.rs:95-102)
+  std::vector<double> dynamic_stats() {
+    std::vector<double> v(4);
+    check(cavlbm_dynamic_stats(s_, v.data()));
+    return v;
   }
   // TimeAverager (post.rs:11-57), accumulated on the device
   void average_accumulate() { check(cavlbm_average_accumulate(s_)); }

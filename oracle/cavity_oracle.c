@@ -58,6 +58,11 @@ struct orc_solver {
   size_t jobs;
   uint64_t time_step, spin_up_steps;
   int geq_order;
+  /* dynamic mixed-scale SGS (solver.rs:76-103) */
+  int dynamic_smag;
+  double smag_alpha, nu_sm_min, eta_test;
+  double *nu_t_pre, *alpha_t_pre; /* NULL when off */
+  double cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max;
 };
 
 /* ---- xorshift64* (src/uniform/rng.rs:11-30) ---------------------------- */
@@ -193,6 +198,10 @@ orc_solver *orc_new(size_t nx, size_t ny, double ra, double pr, double cs,
   }
   s->nu_start_mult = fmax(nu_start_mult, 1.0);
   s->geq_order = geq_order == 1 ? 1 : 2;
+  s->dynamic_smag = 0; /* solver.rs:199-208 */
+  s->smag_alpha = 0.5;
+  s->nu_sm_min = -0.1;
+  s->eta_test = 2.0;
 
   /* init loop, iy outer / ix inner (solver.rs:160-176) */
   uint64_t rng = seed | 1ULL;
@@ -229,6 +238,8 @@ void orc_free(orc_solver *s) {
   free(s->nu_t);
   free(s->fy_node);
   free(s->theta_hot);
+  free(s->nu_t_pre);
+  free(s->alpha_t_pre);
   free(s);
 }
 
@@ -301,9 +312,10 @@ void orc_step_scalars(const orc_solver *s, uint64_t step, double out[4]) {
 }
 
 /* ---- collide_node (src/uniform/solver.rs:1054-1185) ----------------------- */
-static void collide_node(double *f, double *g, double *nu_t_out, double *fy_out,
-                         double rho, double ux, double uy, double temp,
-                         const collide_ctx *c, int geq_order) {
+static void collide_node_pre(double *f, double *g, double *nu_t_out, double *fy_out,
+                             double rho, double ux, double uy, double temp,
+                             const double *nu_t_pre, const double *alpha_t_pre,
+                             const collide_ctx *c, int geq_order) {
   /* boussinesq_force(rho, beta, 1.0, T, t_ref) = rho*beta*(T - t_ref) */
   double fy = rho * c->beta * (temp - c->t_ref);
   *fy_out = fy;
@@ -323,13 +335,14 @@ static void collide_node(double *f, double *g, double *nu_t_out, double *fy_out,
   }
   double pi2 = pi_xx * pi_xx + 2.0 * pi_xy * pi_xy + pi_yy * pi_yy;
   double s_mag = sqrt(pi2) / (2.0 * CS2 * c->tau * fmax(rho, 1e-12));
-  /* compute_turbulent_viscosity(s_mag, cs) = (cs*dx)^2*|S|, dx = 1 */
-  double nu_t = c->cs * c->cs * s_mag;
+  /* compute_turbulent_viscosity(s_mag, cs) = (cs*dx)^2*|S|, dx = 1, or the
+   * precomputed dynamic value (solver.rs:1118-1121) */
+  double nu_t = nu_t_pre ? *nu_t_pre : c->cs * c->cs * s_mag;
   *nu_t_out = nu_t;
 
   double tau_plus = 0.5 + 3.0 * (c->nu + nu_t);
   double tau_minus = 0.5 + c->lambda / (tau_plus - 0.5);
-  double alpha_t = nu_t / c->pr_t;
+  double alpha_t = alpha_t_pre ? *alpha_t_pre : nu_t / c->pr_t; /* solver.rs:1136-1139 */
   double tau_t_plus = 0.5 + 3.0 * (c->alpha + alpha_t);
   double tau_t_minus = tau_t_plus;
   double a_f = 0.5 * (1.0 / tau_plus + 1.0 / tau_minus);
@@ -358,6 +371,12 @@ static void collide_node(double *f, double *g, double *nu_t_out, double *fy_out,
   }
 }
 
+static void collide_node(double *f, double *g, double *nu_t_out, double *fy_out,
+                         double rho, double ux, double uy, double temp,
+                         const collide_ctx *c, int geq_order) {
+  collide_node_pre(f, g, nu_t_out, fy_out, rho, ux, uy, temp, NULL, NULL, c, geq_order);
+}
+
 void orc_collide_node(double f[9], double g[9], double *nu_t_out,
                       double *fy_out, double rho, double ux, double uy,
                       double temp, double tau_mol, double beta, double t_ref,
@@ -378,8 +397,10 @@ void orc_collide(orc_solver *s, uint64_t step) {
   for (long iy = 0; iy < ny; ++iy) {
     for (long ix = 0; ix < nx; ++ix) {
       long k = iy * nx + ix;
-      collide_node(s->f[k].f, s->g[k].f, &s->nu_t[k], &s->fy_node[k], s->rho[k],
-                   s->ux[k], s->uy[k], s->temp[k], &c, go);
+      collide_node_pre(s->f[k].f, s->g[k].f, &s->nu_t[k], &s->fy_node[k], s->rho[k],
+                       s->ux[k], s->uy[k], s->temp[k],
+                       s->nu_t_pre ? &s->nu_t_pre[k] : NULL,
+                       s->alpha_t_pre ? &s->alpha_t_pre[k] : NULL, &c, go);
     }
   }
 }
@@ -493,8 +514,213 @@ void orc_enforce_wall_velocity(orc_solver *s) {
   }
 }
 
+
+/* ---- dynamic mixed-scale SGS (src/uniform/solver.rs:562-828, helpers
+ *      :1195-1261) ---------------------------------------------------------- */
+#define C_DYN_MAX 0.04
+#define TAU_TRT_FLOOR 0.505
+
+void orc_set_dynamic_sgs(orc_solver *s, int on, double smag_alpha, double nu_sm_min,
+                         double eta_test) { /* solver.rs:215-220 */
+  s->dynamic_smag = on ? 1 : 0;
+  s->smag_alpha = smag_alpha;
+  s->nu_sm_min = nu_sm_min;
+  s->eta_test = eta_test;
+}
+
+double orc_lilly_coefficient(double num, double den) { /* solver.rs:1251-1261 */
+  if (!isfinite(den) || !isfinite(num) || den <= 0.0) return 0.0;
+  double c = 0.5 * num / den;
+  if (isfinite(c)) return c < 0.0 ? 0.0 : (c > C_DYN_MAX ? C_DYN_MAX : c);
+  return 0.0;
+}
+
+double orc_mixed_factor(double a, double b, double alpha) { /* solver.rs:1195-1197 */
+  return pow(a, alpha) * pow(b, 1.0 - alpha);
+}
+
+/* box_filter (solver.rs:1204-1224): window [i - half_left, i + width -
+ * half_left) clamped to the domain, sum in (yy, xx) order, divided by count. */
+static void box_filter(const double *field, size_t nx, size_t ny, size_t width, double *out) {
+  size_t half_left = (width - 1) / 2;
+  for (size_t iy = 0; iy < ny; ++iy) {
+    size_t y0 = iy >= half_left ? iy - half_left : 0;
+    size_t y1 = iy + width - half_left < ny ? iy + width - half_left : ny;
+    for (size_t ix = 0; ix < nx; ++ix) {
+      size_t x0 = ix >= half_left ? ix - half_left : 0;
+      size_t x1 = ix + width - half_left < nx ? ix + width - half_left : nx;
+      double sum = 0.0;
+      for (size_t yy = y0; yy < y1; ++yy)
+        for (size_t xx = x0; xx < x1; ++xx) sum += field[yy * nx + xx];
+      double cnt = (double)((y1 - y0) * (x1 - x0));
+      out[iy * nx + ix] = sum / cnt;
+    }
+  }
+}
+
+static double ddx(const double *f, size_t nx, size_t ix, size_t iy) { /* :1229-1234 */
+  size_t xm = ix == 0 ? 0 : ix - 1;
+  size_t xp = ix + 1 >= nx ? nx - 1 : ix + 1;
+  return 0.5 * (f[iy * nx + xp] - f[iy * nx + xm]);
+}
+
+static double ddy(const double *f, size_t nx, size_t ny, size_t ix, size_t iy) { /* :1238-1242 */
+  size_t ym = iy == 0 ? 0 : iy - 1;
+  size_t yp = iy + 1 >= ny ? ny - 1 : iy + 1;
+  return 0.5 * (f[yp * nx + ix] - f[ym * nx + ix]);
+}
+
+void orc_compute_dynamic_sgs(orc_solver *s) {
+  const size_t nx = s->nx, ny = s->ny, n = s->n;
+  collide_ctx c = make_ctx(s, s->time_step);
+  const double tau = c.tau;
+  const double alpha_exp = s->smag_alpha;
+  const double eta = fmax(s->eta_test, 1.0);
+  /* (eta.round() as usize).max(1); every width from 2*max(nx, ny) + 1 on
+   * averages the whole domain, and the cap keeps the cast defined */
+  const double cap = 2.0 * (double)(nx > ny ? nx : ny) + 1.0;
+  const double wr = round(eta);
+  size_t width = (size_t)(wr < cap ? wr : cap);
+  if (width < 1) width = 1;
+  const double dx = 1.0, dtest = eta * dx, delta_t = DELTA_T;
+  const double nu_mol = s->nu, alpha_mol = s->alpha, nu_sm_min = s->nu_sm_min;
+  const double *rho = s->rho, *ux = s->ux, *uy = s->uy, *temp = s->temp;
+  enum { UBX, UBY, TB, PXX, PXY, PYY, PTX, PTY, LXX, LXY, LYY, LTX, LTY,
+         SHXX, SHXY, SHYY, SMH, SBXX, SBXY, SBYY, SMB, GTX, GTY, GBX, GBY,
+         QSM, PHI, TMH, TMB, QBAR, PHIBAR, NUMV, DENV, NUMK, DENK, NUG, KAPG,
+         NUMVB, DENVB, NUMKB, DENKB, NARR };
+  double *a = (double *)xcalloc(n * NARR, sizeof(double));
+#define A(i) (a + (size_t)(i) * n)
+  box_filter(ux, nx, ny, width, A(UBX));
+  box_filter(uy, nx, ny, width, A(UBY));
+  box_filter(temp, nx, ny, width, A(TB));
+  for (size_t k = 0; k < n; ++k) {
+    A(PXX)[k] = ux[k] * ux[k];
+    A(PXY)[k] = ux[k] * uy[k];
+    A(PYY)[k] = uy[k] * uy[k];
+    A(PTX)[k] = ux[k] * temp[k];
+    A(PTY)[k] = uy[k] * temp[k];
+  }
+  box_filter(A(PXX), nx, ny, width, A(LXX));
+  box_filter(A(PXY), nx, ny, width, A(LXY));
+  box_filter(A(PYY), nx, ny, width, A(LYY));
+  box_filter(A(PTX), nx, ny, width, A(LTX));
+  box_filter(A(PTY), nx, ny, width, A(LTY));
+  for (size_t iy = 0; iy < ny; ++iy)
+    for (size_t ix = 0; ix < nx; ++ix) {
+      size_t k = iy * nx + ix;
+      double rho_k = rho[k], ux_k = ux[k], uy_k = uy[k], t_k = temp[k];
+      double pi_xx = 0.0, pi_xy = 0.0, pi_yy = 0.0;
+      for (int i = 0; i < 9; ++i) {
+        double cx = (double)CX[i], cy = (double)CY[i];
+        double fneq = s->f[k].f[i] - orc_feq(i, rho_k, ux_k, uy_k);
+        pi_xx += fneq * cx * cx;
+        pi_xy += fneq * cx * cy;
+        pi_yy += fneq * cy * cy;
+      }
+      double rho_m = fmax(rho_k, 1e-12);
+      double denom = 2.0 * CS2 * tau * rho_m;
+      double pi2 = pi_xx * pi_xx + 2.0 * pi_xy * pi_xy + pi_yy * pi_yy;
+      A(SHXX)[k] = -pi_xx / denom;
+      A(SHXY)[k] = -pi_xy / denom;
+      A(SHYY)[k] = -pi_yy / denom;
+      A(SMH)[k] = sqrt(pi2) / denom;
+      double sxx = ddx(A(UBX), nx, ix, iy);
+      double syy = ddy(A(UBY), nx, ny, ix, iy);
+      double sxy = 0.5 * (ddx(A(UBY), nx, ix, iy) + ddy(A(UBX), nx, ny, ix, iy));
+      A(SBXX)[k] = sxx;
+      A(SBXY)[k] = sxy;
+      A(SBYY)[k] = syy;
+      A(SMB)[k] = sqrt(sxx * sxx + 2.0 * sxy * sxy + syy * syy);
+      A(GTX)[k] = ddx(temp, nx, ix, iy);
+      A(GTY)[k] = ddy(temp, nx, ny, ix, iy);
+      A(GBX)[k] = ddx(A(TB), nx, ix, iy);
+      A(GBY)[k] = ddy(A(TB), nx, ny, ix, iy);
+      double dux = ux_k - A(UBX)[k], duy = uy_k - A(UBY)[k];
+      double du2 = dux * dux + duy * duy;
+      A(QSM)[k] = sqrt(0.5 * du2) / 2.0;
+      double dt_local = t_k - A(TB)[k];
+      A(PHI)[k] = 0.5 * fabs(dt_local) * sqrt(0.5 * du2);
+      double th_x = A(GTX)[k] * A(SHXX)[k] + A(GTY)[k] * A(SHXY)[k];
+      double th_y = A(GTX)[k] * A(SHXY)[k] + A(GTY)[k] * A(SHYY)[k];
+      A(TMH)[k] = sqrt(2.0 * (th_x * th_x + th_y * th_y));
+      double tb_x = A(GBX)[k] * A(SBXX)[k] + A(GBY)[k] * A(SBXY)[k];
+      double tb_y = A(GBX)[k] * A(SBXY)[k] + A(GBY)[k] * A(SBYY)[k];
+      A(TMB)[k] = sqrt(2.0 * (tb_x * tb_x + tb_y * tb_y));
+    }
+  box_filter(A(QSM), nx, ny, width, A(QBAR));
+  box_filter(A(PHI), nx, ny, width, A(PHIBAR));
+  for (size_t k = 0; k < n; ++k) {
+    double nu_mix_t = pow(dtest * dtest * A(SMB)[k], alpha_exp) *
+                      pow(dtest * A(QBAR)[k], 1.0 - alpha_exp);
+    double nu_mix_g = pow(dx * dx * A(SMH)[k], alpha_exp) * pow(dx * A(QSM)[k], 1.0 - alpha_exp);
+    double kap_mix_t = pow(dtest * dtest * dtest / delta_t * A(TMB)[k], alpha_exp) *
+                       pow(dtest / delta_t * A(PHIBAR)[k], 1.0 - alpha_exp);
+    double kap_mix_g = pow(dx * dx * dx / delta_t * A(TMH)[k], alpha_exp) *
+                       pow(dx / delta_t * A(PHI)[k], 1.0 - alpha_exp);
+    A(NUG)[k] = nu_mix_g;
+    A(KAPG)[k] = kap_mix_g;
+    double mxx = nu_mix_t * A(SBXX)[k] - nu_mix_g * A(SHXX)[k];
+    double mxy = nu_mix_t * A(SBXY)[k] - nu_mix_g * A(SHXY)[k];
+    double myy = nu_mix_t * A(SBYY)[k] - nu_mix_g * A(SHYY)[k];
+    double mx = kap_mix_t * A(GBX)[k] - kap_mix_g * A(GTX)[k];
+    double my = kap_mix_t * A(GBY)[k] - kap_mix_g * A(GTY)[k];
+    double lxx = A(LXX)[k] - A(UBX)[k] * A(UBX)[k];
+    double lxy = A(LXY)[k] - A(UBX)[k] * A(UBY)[k];
+    double lyy = A(LYY)[k] - A(UBY)[k] * A(UBY)[k];
+    double lkk3 = (lxx + lyy) / 3.0;
+    double ldxx = lxx - lkk3, ldyy = lyy - lkk3;
+    double ljx = A(LTX)[k] - A(UBX)[k] * A(TB)[k];
+    double ljy = A(LTY)[k] - A(UBY)[k] * A(TB)[k];
+    A(NUMV)[k] = ldxx * mxx + 2.0 * lxy * mxy + ldyy * myy;
+    A(DENV)[k] = mxx * mxx + 2.0 * mxy * mxy + myy * myy;
+    A(NUMK)[k] = ljx * mx + ljy * my;
+    A(DENK)[k] = mx * mx + my * my;
+  }
+  box_filter(A(NUMV), nx, ny, width, A(NUMVB));
+  box_filter(A(DENV), nx, ny, width, A(DENVB));
+  box_filter(A(NUMK), nx, ny, width, A(NUMKB));
+  box_filter(A(DENK), nx, ny, width, A(DENKB));
+  if (!s->nu_t_pre) s->nu_t_pre = (double *)xcalloc(n, sizeof(double));
+  if (!s->alpha_t_pre) s->alpha_t_pre = (double *)xcalloc(n, sizeof(double));
+  double cnu_sum = 0.0, cnu_max = 0.0, ck_sum = 0.0, ck_max = 0.0;
+  const double tau_floor_margin = (TAU_TRT_FLOOR - 0.5) / 3.0;
+  for (size_t k = 0; k < n; ++k) {
+    double cnu = orc_lilly_coefficient(A(NUMVB)[k], A(DENVB)[k]);
+    double ck = orc_lilly_coefficient(A(NUMKB)[k], A(DENKB)[k]);
+    cnu_sum += cnu;
+    if (cnu > cnu_max) cnu_max = cnu;
+    ck_sum += ck;
+    if (ck > ck_max) ck_max = ck;
+    double nt = cnu * A(NUG)[k];
+    double nu_t_min = fmax(nu_sm_min * nu_mol, tau_floor_margin - nu_mol);
+    s->nu_t_pre[k] = nt < nu_t_min ? nu_t_min : nt;
+    double at = ck * A(KAPG)[k];
+    double alpha_t_min = fmax(nu_sm_min * alpha_mol, tau_floor_margin - alpha_mol);
+    s->alpha_t_pre[k] = at < alpha_t_min ? alpha_t_min : at;
+  }
+  s->cs_dyn_mean = cnu_sum / (double)n;
+  s->cs_dyn_max = cnu_max;
+  s->kappa_dyn_mean = ck_sum / (double)n;
+  s->kappa_dyn_max = ck_max;
+#undef A
+  free(a);
+}
+
+void orc_dynamic_stats(const orc_solver *s, double out[4]) {
+  out[0] = s->cs_dyn_mean;
+  out[1] = s->cs_dyn_max;
+  out[2] = s->kappa_dyn_mean;
+  out[3] = s->kappa_dyn_max;
+}
+
+double *orc_sgs_pre(orc_solver *s, int which) {
+  return which == 0 ? s->nu_t_pre : s->alpha_t_pre;
+}
+
 /* ---- step (src/uniform/solver.rs:279-293) --------------------------------- */
 void orc_step(orc_solver *s) {
+  if (s->dynamic_smag) orc_compute_dynamic_sgs(s); /* solver.rs:284-286 */
   orc_collide(s, s->time_step);
   orc_stream(s);
   orc_thermal_bc(s);

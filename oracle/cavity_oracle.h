@@ -85,6 +85,17 @@ double orc_wall_shear_hot(const orc_solver *s, size_t iy);
 double orc_y_plus_hot(const orc_solver *s);
 int orc_check_finite(const orc_solver *s);
 
+/* Dynamic mixed-scale SGS (solver.rs:215-220, :562-828, :1195-1261). */
+void orc_set_dynamic_sgs(orc_solver *s, int on, double smag_alpha, double nu_sm_min,
+                         double eta_test);
+void orc_compute_dynamic_sgs(orc_solver *s);
+double orc_lilly_coefficient(double num, double den);
+double orc_mixed_factor(double a, double b, double alpha);
+/* cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max */
+void orc_dynamic_stats(const orc_solver *s, double out[4]);
+/* nu_t_pre (0) or alpha_t_pre (1); NULL before the first dynamic step */
+double *orc_sgs_pre(orc_solver *s, int which);
+
 /* Crate primitives as restated here (call sites listed in the .c file). */
 double orc_feq(int i, double rho, double ux, double uy);
 double orc_geq(int i, double t, double ux, double uy, int geq_order);

@@ -52,6 +52,12 @@ def lib():
         "orc_enforce_wall_velocity": (None, [vp]),
         "orc_init_from_fields": (i, [vp, dp, dp, dp, dp, sz]),
         "orc_set_hot_wall": (None, [vp, dp]),
+        "orc_set_dynamic_sgs": (None, [vp, i, d, d, d]),
+        "orc_compute_dynamic_sgs": (None, [vp]),
+        "orc_lilly_coefficient": (d, [d, d]),
+        "orc_mixed_factor": (d, [d, d, d]),
+        "orc_dynamic_stats": (None, [vp, dp]),
+        "orc_sgs_pre": (dp, [vp, i]),
         "orc_field": (dp, [vp, i]),
         "orc_pops": (dp, [vp, i]),
         "orc_nx": (sz, [vp]),
@@ -91,6 +97,14 @@ def lib():
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lilly_coefficient(num, den):
+    return lib().orc_lilly_coefficient(num, den)
+
+
+def mixed_factor(a, b, alpha):
+    return lib().orc_mixed_factor(a, b, alpha)
 
 
 def lattice():
@@ -192,6 +206,24 @@ class OracleSolver:
             a = np.ascontiguousarray(theta, dtype=np.float64)
             assert a.size == self.ny
             self._L.orc_set_hot_wall(self._h, _dp(a))
+
+    def set_dynamic_sgs(self, on, smag_alpha=0.5, nu_sm_min=-0.1, eta_test=2.0):
+        self._L.orc_set_dynamic_sgs(self._h, 1 if on else 0, smag_alpha, nu_sm_min, eta_test)
+
+    def compute_dynamic_sgs(self):
+        self._L.orc_compute_dynamic_sgs(self._h)
+
+    def dynamic_stats(self):
+        out = np.zeros(4)
+        self._L.orc_dynamic_stats(self._h, _dp(out))
+        return tuple(float(v) for v in out)
+
+    def sgs_pre(self, which):
+        """Copy of nu_t_pre (0) / alpha_t_pre (1), or None before the model ran."""
+        p = self._L.orc_sgs_pre(self._h, which)
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(self._n,)).copy()
 
     @property
     def time_step(self):

@@ -28,6 +28,9 @@ def run(*args, timeout=120):
     (["--nx"], "manquante"),
     (["--bogus", "1"], "inconnue"),
     (["--mode", "octree"], "octree"),
+    (["--smag-alpha", "1.5"], "smag-alpha"),
+    (["--eta-test", "0.5"], "eta-test"),
+    (["--dynamic-smag=maybe"], "booléen"),
 ])
 def test_cli_validation_exit_1(args, needle):
     r = run(*args)
@@ -77,3 +80,23 @@ def test_cli_restart_errors_and_abort(tmp_path):
     r = run("--nx", nx, "--ny", ny, "--restart", path, "--steps", 100, "--out", tmp_path / "a")
     assert r.returncode == 2 and "AVORT" in r.stderr
     assert "AVORT" in (tmp_path / "a" / "progress.log").read_text()
+
+
+@pytest.mark.gpu
+def test_cli_dynamic_smag(tmp_path):
+    """--dynamic-smag (main.rs:94-102, :179-187, :393-424): the announcement,
+    the coefficient suffix of the progress lines and dynamic_sgs.json; a
+    following flag is not taken as the switch's value (cli.rs:201-211)."""
+    out = tmp_path / "dyn"
+    r = run("--nx", 24, "--ny", 96, "--ra", 1e6, "--steps", 600, "--sample-start", 300,
+            "--sample-every", 50, "--progress-every", 200, "--dynamic-smag", "--eta-test", 3,
+            "--out", out)
+    assert r.returncode == 0, r.stderr
+    assert "LES dynamique : échelles mixtes α = 0.5 (Tab. 2.3) | η_test = 3 | νsm_min = -0.1·ν" \
+        in r.stdout
+    lines = (out / "progress.log").read_text().splitlines()
+    assert all(" | cs_dyn_mean = " in ln and " | kap_dyn_mean = " in ln for ln in lines)
+    dj = json.loads((out / "dynamic_sgs.json").read_text())
+    assert dj["dynamic_smag"] is True and dj["eta_test"] == 3 and dj["smag_alpha"] == 0.5
+    assert 0.0 <= dj["cs_dyn_max"] <= 0.04 and dj["cs_dyn_mean"] <= dj["cs_dyn_max"]
+    assert "LES dynamique : cs_dyn_mean = " in r.stdout

@@ -193,9 +193,6 @@ def test_hot_wall_profile_bitwise(oracle):
 
 def test_errors_follow_reference_asserts():
     g = CavitySolver(*args_of(SMALL))
-    with pytest.raises(CavlbmError) as e:
-        g.set_dynamic_sgs(True)
-    assert e.value.code == E_UNSUPPORTED
     g.set_dynamic_sgs(False)
     with pytest.raises(ValueError):
         g.init_from_fields(np.ones(3), np.ones(3), np.ones(3), np.ones(3))

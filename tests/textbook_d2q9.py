@@ -34,7 +34,7 @@ class TextbookCavity:
     """State arrays are [9, ny, nx] populations and [ny, nx] fields."""
 
     def __init__(self, f, g, rho, ux, uy, temp, fy, nu_t, *, nu, alpha, beta_eff, spin_up,
-                 cs, pr_t, t_ref=0.0, theta_hot=None, quadratic_g=True):
+                 cs, pr_t, t_ref=0.0, theta_hot=None, quadratic_g=True, dynamic=None):
         self.f, self.g = f.copy(), g.copy()
         self.rho, self.ux, self.uy, self.temp = rho.copy(), ux.copy(), uy.copy(), temp.copy()
         self.fy, self.nu_t = fy.copy(), nu_t.copy()
@@ -42,6 +42,9 @@ class TextbookCavity:
         self.cs, self.pr_t, self.t_ref = cs, pr_t, t_ref
         self.theta_hot = theta_hot
         self.quad = quadratic_g
+        # dynamic mixed-scale model parameters (smag_alpha, nu_sm_min, eta_test)
+        # as keywords of tests.dynamic_sgs_numpy.dynamic_sgs, or None
+        self.dynamic = dynamic
         self.time_step = 0
 
     def collide(self, step):
@@ -55,7 +58,14 @@ class TextbookCavity:
         pxy = (CX * CY * fneq).sum(0)
         pyy = (CY * CY * fneq).sum(0)
         smag = np.sqrt(pxx ** 2 + 2.0 * pxy ** 2 + pyy ** 2) / (2.0 * CS2 * tau_mol * np.maximum(rho, 1e-12))
-        self.nu_t = self.cs ** 2 * smag
+        if self.dynamic is not None:
+            from tests.dynamic_sgs_numpy import dynamic_sgs
+            self.nu_t, alpha_t, _, _ = dynamic_sgs(self.f, rho, ux, uy, temp, tau=tau_mol,
+                                                   nu=self.nu, alpha_mol=self.alpha,
+                                                   **self.dynamic)
+        else:
+            self.nu_t = self.cs ** 2 * smag
+            alpha_t = self.nu_t / self.pr_t
         tp = 0.5 + 3.0 * (self.nu + self.nu_t)
         tm = 0.5 + (3.0 / 16.0) / (tp - 0.5)
         even = 0.5 * (fneq + fneq[OPP])
@@ -67,7 +77,7 @@ class TextbookCavity:
         s_even = 0.5 * (src + src[OPP])
         self.f = (self.f - even / tp - odd / tm + (1.0 - 0.5 / tm) * s_odd
                   + (1.0 - 0.5 / tp) * s_even)
-        tau_t = 0.5 + 3.0 * (self.alpha + self.nu_t / self.pr_t)
+        tau_t = 0.5 + 3.0 * (self.alpha + alpha_t)
         self.g = self.g - (self.g - equilibrium(temp, ux, uy, self.quad)) / tau_t
 
     def stream(self):
