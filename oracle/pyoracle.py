@@ -53,6 +53,54 @@ def lib():
         "orc_init_from_fields": (i, [vp, dp, dp, dp, dp, sz]),
         "orc_set_hot_wall": (None, [vp, dp]),
         "orc_set_dynamic_sgs": (None, [vp, i, d, d, d]),
+        # octree (octree_oracle.h)
+        "orc_oct_new": (vp, [sz, sz, sz, d, d, d, u64, sz, i, i, d, i, d, i, C.c_char_p, sz]),
+        "orc_oct_free": (None, [vp]),
+        "orc_oct_step": (None, [vp]),
+        "orc_oct_run": (None, [vp, u64]),
+        "orc_oct_ngrids": (sz, [vp]),
+        "orc_oct_grid_dims": (None, [vp, sz, C.POINTER(sz), C.POINTER(sz)]),
+        "orc_oct_field": (dp, [vp, sz, i]),
+        "orc_oct_pops": (dp, [vp, sz, i]),
+        "orc_oct_time_step": (u64, [vp]),
+        "orc_oct_derived": (None, [vp, dp]),
+        "orc_oct_noise_stats": (None, [vp, dp]),
+        "orc_oct_max_u_all": (d, [vp]),
+        "orc_oct_check_guards": (i, [vp]),
+        "orc_oct_nusselt_hot": (d, [vp]),
+        "orc_oct_nusselt_cold": (d, [vp]),
+        "orc_oct_nusselt_hot_link0": (d, [vp]),
+        "orc_oct_nusselt_hot_fine": (d, [vp, sz, d]),
+        "orc_oct_set_from_snapshot": (i, [vp, dp, dp, dp, dp, sz]),
+        "orc_oct_n_ghosts": (sz, [vp, sz]),
+        "orc_oct_n_coincident": (sz, [vp, sz]),
+        "orc_oct_ghost": (None, [vp, sz, sz, C.POINTER(sz), C.POINTER(sz), dp]),
+        "orc_oct_coincident": (None, [vp, sz, sz, C.POINTER(sz), C.POINTER(sz), dp]),
+        "orc_oct_collide_level": (None, [vp, sz, dp, d, i]),
+        "orc_oct_stream_level": (None, [vp, sz]),
+        "orc_oct_stream_block_level": (None, [vp, sz, i, i, i, i]),
+        "orc_oct_bc_level": (None, [vp, sz]),
+        "orc_oct_bc_block_level": (None, [vp, sz, i, i]),
+        "orc_oct_macro_level": (None, [vp, sz]),
+        "orc_oct_walls_level": (None, [vp, sz]),
+        "orc_oct_walls_block_level": (None, [vp, sz, i, i]),
+        "orc_oct2_new": (vp, [sz, sz, sz, sz, sz, sz, d, d, i]),
+        "orc_oct2_free": (None, [vp]),
+        "orc_oct2_dims": (None, [vp, C.POINTER(sz), C.POINTER(sz)]),
+        "orc_oct2_field": (dp, [vp, i, i]),
+        "orc_oct2_pops": (dp, [vp, i, i]),
+        "orc_oct2_xi": (None, [vp, dp]),
+        "orc_oct2_coarse_to_fine": (None, [vp]),
+        "orc_oct2_fine_to_coarse": (None, [vp]),
+        "orc_oct2_n_ghosts": (sz, [vp]),
+        "orc_oct2_n_coincident": (sz, [vp]),
+        "orc_oct2_ghost_k": (sz, [vp, sz]),
+        "orc_oct2_coincident": (None, [vp, sz, C.POINTER(sz), C.POINTER(sz), dp]),
+        "orc_oct_relax_times": (None, [d, d, dp]),
+        "orc_oct_rescaling_factors": (None, [dp, dp, sz, i, dp]),
+        "orc_oct_rescale_fneq": (None, [dp, dp, dp]),
+        "orc_oct_rescale_fneq_inverse": (None, [dp, dp, dp]),
+        "orc_oct_bilinear_stencil": (None, [d, d, sz, sz, C.POINTER(sz), dp]),
         "orc_compute_dynamic_sgs": (None, [vp]),
         "orc_lilly_coefficient": (d, [d, d]),
         "orc_mixed_factor": (d, [d, d, d]),
@@ -290,3 +338,228 @@ def collide_node(f, g, rho, ux, uy, temp, tau_mol, beta, t_ref, cs, pr_t, nu, al
 
 def poiseuille_channel(nx, ny, nu, fy, nsteps):
     return lib().orc_poiseuille_channel(nx, ny, nu, fy, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# octree oracle (octree_oracle.c): the reference's src/octree restated
+
+OCT_DERIVED = ("nu_lu", "alpha_lu", "beta_eff_coarse", "beta_eff_fine", "spin_up_steps",
+               "tau_plus_l0", "tau_minus_l0", "tau_t_l0", "tau_plus_l1", "tau_minus_l1",
+               "tau_t_l1", "tau_plus_l2", "tau_minus_l2", "tau_t_l2", "xi_even_a", "xi_odd_a",
+               "xi_thermal_a", "xi_even_b", "xi_odd_b", "xi_thermal_b", "n_nodes_total",
+               "levels")
+
+
+def _view(ptr, shape):
+    return np.ctypeslib.as_array(ptr, shape=shape)
+
+
+class OracleLevel:
+    """numpy views of one level: fields [n], pops [n, 9] (k = iy*nx + ix)."""
+
+    def __init__(self, L, h, gi, field_fn, pops_fn, nx, ny):
+        n = nx * ny
+        self.nx, self.ny = nx, ny
+        self.rho = _view(field_fn(h, gi, 0), (n,))
+        self.ux = _view(field_fn(h, gi, 1), (n,))
+        self.uy = _view(field_fn(h, gi, 2), (n,))
+        self.temperature = _view(field_fn(h, gi, 3), (n,))
+        self.f = _view(pops_fn(h, gi, 0), (n, 9))
+        self.g = _view(pops_fn(h, gi, 1), (n, 9))
+
+
+class OracleOctree:
+    """The reference's octree ``CavitySolver`` (src/octree/solver.rs) on the
+    CPU.  ``grids[gi]`` in snapshot order: 0 coarse, 1 hot L1, 2 cold L1,
+    3 hot L2, 4 cold L2."""
+
+    def __init__(self, nx0, ny0, bl_width, ra, pr=0.71, utarget=0.02, seed=42, jobs=1,
+                 odd_exact=False, levels=2, nu_start_mult=1.0, nu_override=None, geq_order=2):
+        L = lib()
+        err = C.create_string_buffer(512)
+        h = L.orc_oct_new(nx0, ny0, bl_width, ra, pr, utarget, seed, jobs, int(odd_exact),
+                          levels, nu_start_mult, 0 if nu_override is None else 1,
+                          0.0 if nu_override is None else nu_override, geq_order, err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h, self._L = h, L
+        self.nx0, self.ny0 = nx0, ny0
+        self.grids = []
+        for gi in range(L.orc_oct_ngrids(h)):
+            nx, ny = C.c_size_t(), C.c_size_t()
+            L.orc_oct_grid_dims(h, gi, C.byref(nx), C.byref(ny))
+            lv = OracleLevel(L, h, gi, L.orc_oct_field, L.orc_oct_pops, nx.value, ny.value)
+            lv.fy = _view(L.orc_oct_field(h, gi, 4), (nx.value * ny.value,))
+            self.grids.append(lv)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.orc_oct_free(h)
+            self._h = None
+
+    def step(self):
+        self._L.orc_oct_step(self._h)
+
+    def run(self, n):
+        self._L.orc_oct_run(self._h, n)
+
+    @property
+    def time_step(self):
+        return self._L.orc_oct_time_step(self._h)
+
+    def derived(self):
+        out = np.zeros(22)
+        self._L.orc_oct_derived(self._h, _dp(out))
+        return dict(zip(OCT_DERIVED, out.tolist()))
+
+    def noise_stats(self):
+        out = np.zeros(3)
+        self._L.orc_oct_noise_stats(self._h, _dp(out))
+        return dict(max_rho_dev=out[0], max_rho_dev_ifaces=out[1], masse=out[2])
+
+    def max_u_all(self):
+        return self._L.orc_oct_max_u_all(self._h)
+
+    def check_guards(self):
+        return self._L.orc_oct_check_guards(self._h)
+
+    def nusselt_hot(self):
+        return self._L.orc_oct_nusselt_hot(self._h)
+
+    def nusselt_cold(self):
+        return self._L.orc_oct_nusselt_cold(self._h)
+
+    def nusselt_hot_link0(self):
+        return self._L.orc_oct_nusselt_hot_link0(self._h)
+
+    def nusselt_hot_fine(self, block, lx_coarse):
+        return self._L.orc_oct_nusselt_hot_fine(self._h, block, lx_coarse)
+
+    def set_from_snapshot(self, rho, ux, uy, t):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, t)]
+        rc = self._L.orc_oct_set_from_snapshot(self._h, *[_dp(a) for a in arrs], arrs[0].size)
+        if rc:
+            raise ValueError("set_from_snapshot: size mismatch")
+
+    def ghosts(self, gi):
+        out = []
+        for j in range(self._L.orc_oct_n_ghosts(self._h, gi)):
+            kf = C.c_size_t()
+            kc = (C.c_size_t * 4)()
+            w = np.zeros(4)
+            self._L.orc_oct_ghost(self._h, gi, j, C.byref(kf), kc, _dp(w))
+            out.append((kf.value, list(kc), w))
+        return out
+
+    def coincident(self, gi):
+        out = []
+        for j in range(self._L.orc_oct_n_coincident(self._h, gi)):
+            k = C.c_size_t()
+            kf = (C.c_size_t * 9)()
+            w = np.zeros(9)
+            self._L.orc_oct_coincident(self._h, gi, j, C.byref(k), kf, _dp(w))
+            out.append((k.value, list(kf), w))
+        return out
+
+    # per-level phases (cavity.rs)
+    def collide_level(self, gi, tau, beta, interior_only):
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        self._L.orc_oct_collide_level(self._h, gi, _dp(t), beta, int(interior_only))
+
+    def stream_level(self, gi):
+        self._L.orc_oct_stream_level(self._h, gi)
+
+    def stream_block_level(self, gi, wl, wr, wb, wt):
+        self._L.orc_oct_stream_block_level(self._h, gi, int(wl), int(wr), int(wb), int(wt))
+
+    def bc_level(self, gi):
+        self._L.orc_oct_bc_level(self._h, gi)
+
+    def bc_block_level(self, gi, hot, cold):
+        self._L.orc_oct_bc_block_level(self._h, gi, int(hot), int(cold))
+
+    def macro_level(self, gi):
+        self._L.orc_oct_macro_level(self._h, gi)
+
+    def walls_level(self, gi):
+        self._L.orc_oct_walls_level(self._h, gi)
+
+    def walls_block_level(self, gi, hot, cold):
+        self._L.orc_oct_walls_block_level(self._h, gi, int(hot), int(cold))
+
+
+class OracleOct2:
+    """The two-level interface harness ``OctreeSolver`` (solver.rs:66-116)."""
+
+    def __init__(self, nx0, ny0, bx0, by0, bx1, by1, nu, alpha, odd_exact=False):
+        L = lib()
+        h = L.orc_oct2_new(nx0, ny0, bx0, by0, bx1, by1, nu, alpha, int(odd_exact))
+        if not h:
+            raise ValueError("bloc fin hors domaine grossier / vide")
+        self._h, self._L = h, L
+        nx1, ny1 = C.c_size_t(), C.c_size_t()
+        L.orc_oct2_dims(h, C.byref(nx1), C.byref(ny1))
+        self.coarse = OracleLevel(L, h, 0, L.orc_oct2_field, L.orc_oct2_pops, nx0, ny0)
+        self.fine = OracleLevel(L, h, 1, L.orc_oct2_field, L.orc_oct2_pops, nx1.value, ny1.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.orc_oct2_free(h)
+            self._h = None
+
+    def xi(self):
+        out = np.zeros(3)
+        self._L.orc_oct2_xi(self._h, _dp(out))
+        return out
+
+    def coarse_to_fine(self):
+        self._L.orc_oct2_coarse_to_fine(self._h)
+
+    def fine_to_coarse(self):
+        self._L.orc_oct2_fine_to_coarse(self._h)
+
+    def ghost_ks(self):
+        return [self._L.orc_oct2_ghost_k(self._h, j)
+                for j in range(self._L.orc_oct2_n_ghosts(self._h))]
+
+    def coincident(self):
+        out = []
+        for j in range(self._L.orc_oct2_n_coincident(self._h)):
+            k = C.c_size_t()
+            kf = (C.c_size_t * 9)()
+            w = np.zeros(9)
+            self._L.orc_oct2_coincident(self._h, j, C.byref(k), kf, _dp(w))
+            out.append((k.value, list(kf), w))
+        return out
+
+
+def oct_relax_times(nu, alpha):
+    out = np.zeros(3)
+    lib().orc_oct_relax_times(nu, alpha, _dp(out))
+    return out
+
+
+def oct_rescaling_factors(tau_c, tau_f, substeps=2, odd_exact=False):
+    out = np.zeros(3)
+    a = np.ascontiguousarray(tau_c, dtype=np.float64)
+    b = np.ascontiguousarray(tau_f, dtype=np.float64)
+    lib().orc_oct_rescaling_factors(_dp(a), _dp(b), substeps, int(odd_exact), _dp(out))
+    return out
+
+
+def oct_rescale_fneq(fneq, xi, inverse=False):
+    a = np.ascontiguousarray(fneq, dtype=np.float64)
+    x = np.ascontiguousarray(xi, dtype=np.float64)
+    out = np.zeros(9)
+    fn = lib().orc_oct_rescale_fneq_inverse if inverse else lib().orc_oct_rescale_fneq
+    fn(_dp(a), _dp(x), _dp(out))
+    return out
+
+
+def oct_bilinear_stencil(x, y, nx0, ny0):
+    k = (C.c_size_t * 4)()
+    w = np.zeros(4)
+    lib().orc_oct_bilinear_stencil(x, y, nx0, ny0, k, _dp(w))
+    return list(k), w
