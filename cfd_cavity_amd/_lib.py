@@ -50,6 +50,34 @@ class Derived(C.Structure):
         "pr_t", "nu_start_mult", "t_ref")] + [("spin_up_steps", C.c_uint64)]
 
 
+class OctreeParams(C.Structure):
+    _fields_ = [
+        ("nx0", C.c_uint64), ("ny0", C.c_uint64), ("bl_width", C.c_uint64),
+        ("ra", C.c_double), ("pr", C.c_double), ("utarget", C.c_double),
+        ("seed", C.c_uint64), ("jobs", C.c_uint64),
+        ("odd_exact", C.c_int32), ("levels", C.c_int32),
+        ("nu_start_mult", C.c_double),
+        ("has_nu_override", C.c_int32), ("geq_order", C.c_int32),
+        ("nu_override", C.c_double),
+        ("device", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class OctreeDerived(C.Structure):
+    _fields_ = [("nu_lu", C.c_double), ("alpha_lu", C.c_double),
+                ("beta_eff_coarse", C.c_double), ("beta_eff_fine", C.c_double),
+                ("tau", (C.c_double * 3) * 3), ("xi_a", C.c_double * 3),
+                ("xi_b", C.c_double * 3), ("spin_up_steps", C.c_uint64),
+                ("n_nodes_total", C.c_uint64), ("levels", C.c_int32), ("n_grids", C.c_int32)]
+
+
+class OctreeDiag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "nusselt_hot", "nusselt_cold", "nusselt_hot_link0", "nusselt_hot_fine", "max_u_all",
+        "max_rho_dev", "max_rho_dev_ifaces", "masse")] + [
+        ("guard", C.c_int32), ("reserved", C.c_int32), ("time_step", C.c_uint64)]
+
+
 # name -> (restype, argtypes); the exported set of include/cavlbm.h
 _vp, _u64, _i32, _d = C.c_void_p, C.c_uint64, C.c_int32, C.c_double
 _dp = C.POINTER(C.c_double)
@@ -102,6 +130,24 @@ SIGNATURES = {
     "cavlbm_batch_nodes": (C.c_int, [_vp, C.POINTER(_u64)]),
     "cavlbm_batch_time_run": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_batch_last_error": (C.c_char_p, [_vp]),
+    "cavlbm_octree_params_default": (C.c_int, [C.POINTER(OctreeParams)]),
+    "cavlbm_octree_create": (C.c_int, [C.POINTER(OctreeParams), C.POINTER(_vp)]),
+    "cavlbm_octree_destroy": (C.c_int, [_vp]),
+    "cavlbm_octree_step": (C.c_int, [_vp]),
+    "cavlbm_octree_run": (C.c_int, [_vp, _u64]),
+    "cavlbm_octree_synchronize": (C.c_int, [_vp]),
+    "cavlbm_octree_time_run": (C.c_int, [_vp, _u64, _dp]),
+    "cavlbm_octree_time_step": (C.c_int, [_vp, C.POINTER(_u64)]),
+    "cavlbm_octree_derived_params": (C.c_int, [_vp, C.POINTER(OctreeDerived)]),
+    "cavlbm_octree_grid_dims": (C.c_int, [_vp, _i32, C.POINTER(_u64), C.POINTER(_u64)]),
+    "cavlbm_octree_get_field": (C.c_int, [_vp, _i32, _i32, _dp, _u64]),
+    "cavlbm_octree_set_field": (C.c_int, [_vp, _i32, _i32, _dp, _u64]),
+    "cavlbm_octree_get_pops": (C.c_int, [_vp, _i32, _i32, _dp, _u64]),
+    "cavlbm_octree_set_pops": (C.c_int, [_vp, _i32, _i32, _dp, _u64]),
+    "cavlbm_octree_set_from_snapshot": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _u64]),
+    "cavlbm_octree_diagnostics": (C.c_int, [_vp, C.POINTER(OctreeDiag)]),
+    "cavlbm_octree_set_dynamic_sgs": (C.c_int, [_vp, _i32, _d, _d, _d]),
+    "cavlbm_octree_last_error": (C.c_char_p, [_vp]),
 }
 
 _lib = None

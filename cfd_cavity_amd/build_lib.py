@@ -31,6 +31,8 @@ UNITS = {
     "lbm_adjoint.hip": ["-ffp-contract=off"],
     "lbm_tangent.hip": ["-ffp-contract=off"],
     "lbm_dynsgs.hip": ["-ffp-contract=off"],
+    "lbm_octree.hip": ["-ffp-contract=off"],
+    "cavlbm_octree.cpp": ["-ffp-contract=off", "-x", "hip"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
     "snapshot.cpp": ["-ffp-contract=off"],
 }

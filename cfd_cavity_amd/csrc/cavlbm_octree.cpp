@@ -1,0 +1,785 @@
+// cavlbm_octree.cpp — the octree (multi-level) cavity solver behind the C-ABI
+// (include/cavlbm.h, cavlbm_octree_*).  Reference: bbrangeo/cfd_cavity
+// src/octree/solver.rs (CavitySolver, step_cavity), grid.rs (interface
+// tables), interface.rs (rescaling factors), cavity.rs (diagnostics).
+//
+// All levels live on the device in M-form between phases; one coarse step is
+// a fixed sequence of per-level collide / stream launches and interface
+// transfers (lbm_octree.hip), captured once into a hipGraph when its
+// per-step scalars are constant (no viscosity ramp).  Diagnostics copy the
+// level fields back and reduce them on the host in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cavlbm.h"
+#include "lbm_octree.hpp"
+
+using namespace cavlbm;
+
+namespace {
+
+thread_local std::string g_oct_create_error;
+
+constexpr double kCScale = 1.7;        // calibration constant (cavity.rs:49)
+constexpr double kMaxUAbort = 0.10;    // cavity.rs:41
+constexpr double kNuTauRamp = 1000.0;  // solver.rs:58
+constexpr double kLambdaTrt = 3.0 / 16.0;
+
+struct Relax {
+  double tp, tm, tt;
+};
+
+// RelaxTimes::from_nu_alpha (lbm.rs:100-109)
+Relax relax_from(double nu, double alpha) {
+  Relax r;
+  r.tp = 0.5 + 3.0 * nu;
+  r.tm = 0.5 + kLambdaTrt / (r.tp - 0.5);
+  r.tt = 0.5 + 3.0 * alpha;
+  return r;
+}
+
+// rescaling_factors_multilevel (interface.rs:141-158)
+OctXi rescaling(const Relax& c, const Relax& f, int substeps, bool odd_exact) {
+  const double dt = 1.0 / (double)substeps;
+  OctXi x;
+  x.even = dt * f.tp / c.tp;
+  x.odd = odd_exact ? dt * f.tm / c.tm : dt;
+  x.thermal = dt * f.tt / c.tt;
+  return x;
+}
+
+// bilinear_stencil (grid.rs:216-229)
+void bilinear(double x_c, double y_c, size_t nx0, size_t ny0, int k[4], double w[4]) {
+  long long ix = (long long)std::floor(x_c), iy = (long long)std::floor(y_c);
+  ix = std::min<long long>(std::max<long long>(ix, 0), (long long)nx0 - 2);
+  iy = std::min<long long>(std::max<long long>(iy, 0), (long long)ny0 - 2);
+  const double fx = x_c - (double)ix, fy = y_c - (double)iy;
+  k[0] = (int)(iy * (long long)nx0 + ix);
+  w[0] = (1.0 - fx) * (1.0 - fy);
+  k[1] = (int)(iy * (long long)nx0 + ix + 1);
+  w[1] = fx * (1.0 - fy);
+  k[2] = (int)((iy + 1) * (long long)nx0 + ix);
+  w[2] = (1.0 - fx) * fy;
+  k[3] = (int)((iy + 1) * (long long)nx0 + ix + 1);
+  w[3] = fx * fy;
+}
+
+// OctreeGrid::build (grid.rs:121-203): the coarse side is nx0 x ny0, the fine
+// block [bx0, bx1) x [by0, by1) in coarse cells with one ghost layer.
+struct HostGrid {
+  size_t nx0 = 0, ny0 = 0, bx0 = 0, by0 = 0, bx1 = 0, by1 = 0, nx1 = 0, ny1 = 0;
+  std::vector<int> gk, gkc, ck, ckf;
+  std::vector<double> gw, cw;
+};
+
+HostGrid build_grid(size_t nx0, size_t ny0, size_t bx0, size_t by0, size_t bx1, size_t by1,
+                    bool wl, bool wr, bool wb, bool wt) {
+  HostGrid g;
+  g.nx0 = nx0;
+  g.ny0 = ny0;
+  g.bx0 = bx0;
+  g.by0 = by0;
+  g.bx1 = bx1;
+  g.by1 = by1;
+  g.nx1 = 2 * (bx1 - bx0) + 3;
+  g.ny1 = 2 * (by1 - by0) + 3;
+  for (size_t b = 0; b < g.ny1; ++b)
+    for (size_t a = 0; a < g.nx1; ++a)
+      if (a == 0 || b == 0 || a == g.nx1 - 1 || b == g.ny1 - 1) {
+        int k[4];
+        double w[4];
+        bilinear((double)bx0 - 0.5 + 0.5 * (double)a, (double)by0 - 0.5 + 0.5 * (double)b, nx0,
+                 ny0, k, w);
+        g.gk.push_back((int)(b * g.nx1 + a));
+        g.gkc.insert(g.gkc.end(), k, k + 4);
+        g.gw.insert(g.gw.end(), w, w + 4);
+      }
+  static const double wt3[3] = {0.25, 0.5, 0.25};
+  for (size_t cy = by0; cy < by1; ++cy)
+    for (size_t cx = bx0; cx < bx1; ++cx) {
+      if (cx == 0 || cx + 1 == nx0 || cy == 0 || cy + 1 == ny0) continue;
+      if ((wl && cx == bx0) || (wr && cx + 1 == bx1) || (wb && cy == by0) ||
+          (wt && cy + 1 == by1))
+        continue;
+      const size_t a = 2 * (cx - bx0) + 1, b = 2 * (cy - by0) + 1;
+      g.ck.push_back((int)(cy * nx0 + cx));
+      for (int jb = 0; jb < 3; ++jb)
+        for (int ja = 0; ja < 3; ++ja) {
+          g.ckf.push_back((int)((b + jb - 1) * g.nx1 + (a + ja - 1)));
+          g.cw.push_back(wt3[ja] * wt3[jb]);
+        }
+    }
+  return g;
+}
+
+}  // namespace
+
+struct cavlbm_octree {
+  cavlbm_octree_params p{};
+  int levels = 2, ngrids = 3, geq = 2;
+  size_t nx0 = 0, ny0 = 0;
+  OctLevel lv[5]{};
+  HostGrid hg[5];  // hg[gi], gi >= 1: tables whose fine side is grid gi
+  OctGhostTab gt[5]{};
+  OctCoincTab ct[5]{};
+  std::vector<void*> dev;  // every device allocation
+  Relax tau_coarse{}, tau_fine{}, tau_sub{};
+  OctXi xi_a{}, xi_b{};
+  double nu_lu = 0, alpha_lu = 0, nu_start_mult = 1, beta_eff_coarse = 0, beta_eff_fine = 0;
+  uint64_t spin_up_steps = 0, time_step = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph = nullptr;
+  bool graph_failed = false;
+  std::string err;
+};
+
+namespace {
+
+int ofail(cavlbm_octree* o, int code, const std::string& m) {
+  if (o)
+    o->err = m;
+  else
+    g_oct_create_error = m;
+  return code;
+}
+
+#define OHIP_TRY(o, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ofail((o), CAVLBM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void free_octree(cavlbm_octree* o) {
+  if (o->graph) (void)hipGraphExecDestroy(o->graph);
+  for (void* p : o->dev)
+    if (p) (void)hipFree(p);
+  if (o->ev0) (void)hipEventDestroy(o->ev0);
+  if (o->ev1) (void)hipEventDestroy(o->ev1);
+  if (o->st) (void)hipStreamDestroy(o->st);
+}
+
+template <class T>
+T* dalloc(cavlbm_octree* o, size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+  o->dev.push_back(p);
+  return static_cast<T*>(p);
+}
+
+// Stream/BC rules of grid gi: the coarse level has domain walls; a block or
+// strip bounces back from its wall-side ghost layers (stream_cavity_block
+// with wall_left = hot, wall_right = cold, bottom and top walls), with the
+// isothermal column of its wall (solver.rs:785-789, :846-850).
+OctStream stream_rules(const cavlbm_octree* o, int gi) {
+  OctStream s{};
+  if (gi == 0) return s;
+  const bool hot = gi == 1 || gi == 3;
+  s.block = 1;
+  s.wl = hot;
+  s.wr = !hot;
+  s.wb = 1;
+  s.wt = 1;
+  s.hot = hot;
+  s.cold = !hot;
+  (void)o;
+  return s;
+}
+
+// ramp_nu_alpha (solver.rs:600-606) and ramp_beta_factor (solver.rs:614-619)
+void ramp(const cavlbm_octree* o, uint64_t step, double& nu, double& alpha) {
+  if (o->nu_start_mult == 1.0) {
+    nu = o->nu_lu;
+    alpha = o->alpha_lu;
+    return;
+  }
+  const double m = 1.0 + (o->nu_start_mult - 1.0) * std::exp(-(double)step / kNuTauRamp);
+  nu = o->nu_lu * m;
+  alpha = o->alpha_lu * m;
+}
+
+double ramp_beta(const cavlbm_octree* o, uint64_t step) {
+  if (o->nu_start_mult == 1.0 || step >= o->spin_up_steps) return 1.0;
+  return (double)step / (double)o->spin_up_steps;
+}
+
+OctCollide collide_params(double beta, double nu, double alpha, const Relax& r, bool interior) {
+  OctCollide p{};
+  p.ctx.beta = beta;
+  p.ctx.nu = nu;
+  p.ctx.alpha = alpha;
+  p.ctx.tau = r.tp;  // enters |S| only, multiplied by Cs = 0
+  p.ctx.t_ref = 0.0;
+  p.ctx.cs = 0.0;
+  p.ctx.pr_t = 1.0;
+  p.ctx.inv_pr_t = 1.0;
+  p.interior_only = interior ? 1 : 0;
+  return p;
+}
+
+// One level sub-step: collide_cavity, then stream + BCs + moments + walls.
+hipError_t substep(cavlbm_octree* o, int gi, const OctCollide& c) {
+  hipError_t e = launch_oct_collide(o->lv[gi], c, o->geq, o->st);
+  if (e != hipSuccess) return e;
+  return launch_oct_stream(o->lv[gi], stream_rules(o, gi), o->geq, o->st);
+}
+
+// step_cavity (solver.rs:647-947) with the dynamic SGS off: enqueues the
+// launches of the coarse step `step`.
+hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
+  double nu_eff, alpha_eff;
+  ramp(o, step, nu_eff, alpha_eff);
+  const Relax tc = relax_from(nu_eff, alpha_eff);
+  const Relax tf = relax_from(2.0 * nu_eff, 2.0 * alpha_eff);
+  const Relax ts = relax_from(4.0 * nu_eff, 4.0 * alpha_eff);
+  const double br = ramp_beta(o, step);
+  const double beta_c = o->beta_eff_coarse * br;
+  const double beta_f = o->beta_eff_fine * br;
+  const double beta_s = 0.25 * o->beta_eff_coarse * br;
+  const OctCollide c0 = collide_params(beta_c, nu_eff, alpha_eff, tc, false);
+  const OctCollide c1 = collide_params(beta_f, 2.0 * nu_eff, 2.0 * alpha_eff, tf, true);
+  const OctCollide c2 = collide_params(beta_s, 4.0 * nu_eff, 4.0 * alpha_eff, ts, true);
+  hipError_t e;
+#define OCT_Q(x)              \
+  do {                        \
+    e = (x);                  \
+    if (e != hipSuccess) return e; \
+  } while (0)
+  OCT_Q(substep(o, 0, c0));
+  for (int b = 1; b <= 2; ++b)
+    OCT_Q(launch_oct_coarse_to_fine(o->lv[0], o->lv[b], o->gt[b], o->xi_a, o->geq, o->st));
+  for (int b = 1; b <= 2; ++b) {
+    OCT_Q(substep(o, b, c1));
+    if (o->levels == 3) {
+      const int s = b + 2;
+      OCT_Q(launch_oct_coarse_to_fine(o->lv[b], o->lv[s], o->gt[s], o->xi_b, o->geq, o->st));
+      OCT_Q(substep(o, s, c2));
+      OCT_Q(substep(o, s, c2));
+      OCT_Q(launch_oct_fine_to_coarse(o->lv[s], o->lv[b], o->ct[s], o->xi_b, o->geq, o->st));
+    }
+    OCT_Q(substep(o, b, c1));
+    OCT_Q(launch_oct_fine_to_coarse(o->lv[b], o->lv[0], o->ct[b], o->xi_a, o->geq, o->st));
+  }
+#undef OCT_Q
+  return hipSuccess;
+}
+
+hipGraphExec_t capture_step(cavlbm_octree* o) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipStreamBeginCapture(o->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  const bool ok = enqueue_step(o, o->time_step) == hipSuccess;
+  if (hipStreamEndCapture(o->st, &g) != hipSuccess || !ok) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  return ex;
+}
+
+int run_steps(cavlbm_octree* o, uint64_t n) {
+  for (uint64_t s = 0; s < n; ++s) {
+    // constant per-step scalars without the viscosity ramp: replay one graph
+    if (o->nu_start_mult == 1.0 && !o->graph_failed) {
+      if (!o->graph) o->graph = capture_step(o);
+      if (!o->graph) {
+        o->graph_failed = true;
+      } else {
+        OHIP_TRY(o, hipGraphLaunch(o->graph, o->st));
+        o->time_step += 1;
+        continue;
+      }
+    }
+    OHIP_TRY(o, enqueue_step(o, o->time_step));
+    o->time_step += 1;
+  }
+  return 0;
+}
+
+bool grid_ok(const cavlbm_octree* o, int32_t grid) { return grid >= 0 && grid < o->ngrids; }
+
+int plane_of_field(int32_t field) {
+  switch (field) {
+    case CAVLBM_FIELD_RHO: return kOctRho;
+    case CAVLBM_FIELD_UX: return kOctUx;
+    case CAVLBM_FIELD_UY: return kOctUy;
+    case CAVLBM_FIELD_T: return kOctT;
+    case CAVLBM_FIELD_FY_NODE: return kOctFy;
+  }
+  return -1;
+}
+
+int copy_down(cavlbm_octree* o, const double* dev, double* host, size_t n) {
+  OHIP_TRY(o, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, o->st));
+  OHIP_TRY(o, hipStreamSynchronize(o->st));
+  return 0;
+}
+
+int copy_up(cavlbm_octree* o, const double* host, double* dev, size_t n) {
+  OHIP_TRY(o, hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, o->st));
+  OHIP_TRY(o, hipStreamSynchronize(o->st));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cavlbm_octree_params_default(cavlbm_octree_params* p) {
+  if (!p) return CAVLBM_E_PARAM;
+  std::memset(p, 0, sizeof(*p));
+  p->nx0 = 64;
+  p->ny0 = 256;
+  p->bl_width = 8;
+  p->ra = 1.0e6;
+  p->pr = 0.71;
+  p->utarget = 0.02;
+  p->seed = 42;
+  p->jobs = 1;
+  p->odd_exact = 0;
+  p->levels = 2;
+  p->nu_start_mult = 1.0;
+  p->geq_order = 2;
+  return 0;
+}
+
+int cavlbm_octree_create(const cavlbm_octree_params* pin, cavlbm_octree** out) {
+  if (!pin || !out) return ofail(nullptr, CAVLBM_E_PARAM, "null argument");
+  *out = nullptr;
+  const cavlbm_octree_params p = *pin;
+  // solver.rs:356-361
+  if (p.bl_width < 2)
+    return ofail(nullptr, CAVLBM_E_PARAM, "largeur de BL fine >= 2 cellules grossieres requise");
+  if (!(2 * p.bl_width < p.nx0))
+    return ofail(nullptr, CAVLBM_E_PARAM, "les 2 blocs fins doivent laisser un noyau grossier");
+  if (p.levels != 2 && p.levels != 3)
+    return ofail(nullptr, CAVLBM_E_PARAM,
+                 "levels doit valoir 2 (config V2) ou 3 (escalier tanh V6)");
+  if (p.ny0 < 3 || p.nx0 > (1u << 16) || p.ny0 > (1u << 16))
+    return ofail(nullptr, CAVLBM_E_PARAM, "ny0 must be >= 3 and nx0, ny0 <= 65536");
+  if (p.geq_order != 1 && p.geq_order != 2)
+    return ofail(nullptr, CAVLBM_E_PARAM, "geq_order must be 1 or 2");
+  // calibration (solver.rs:363-392)
+  const double lz = (double)(p.ny0 - 1);
+  const double nu_cal =
+      std::pow(p.utarget * std::sqrt(p.pr) * lz / (kCScale * std::sqrt(p.ra)), 2.0 / 3.0);
+  double nu;
+  if (p.has_nu_override) {
+    if (!(p.nu_override > 0.0 && std::isfinite(p.nu_override)))
+      return ofail(nullptr, CAVLBM_E_PARAM, "--nu doit être > 0 et fini");
+    nu = p.nu_override;
+  } else {
+    nu = std::fmax(nu_cal, 0.002);
+  }
+  const double alpha = nu / p.pr;
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return ofail(nullptr, CAVLBM_E_NODEVICE, "no HIP device visible");
+  if (p.device < 0 || p.device >= ndev) return ofail(nullptr, CAVLBM_E_PARAM, "bad device ordinal");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p.device) != hipSuccess)
+    return ofail(nullptr, CAVLBM_E_NODEVICE, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return ofail(nullptr, CAVLBM_E_NODEVICE,
+                 std::string("libcavlbm is built for gfx950, device is ") + prop.gcnArchName);
+  if (hipSetDevice(p.device) != hipSuccess) return ofail(nullptr, CAVLBM_E_HIP, "hipSetDevice failed");
+
+  auto* o = new cavlbm_octree();
+  o->p = p;
+  o->levels = p.levels;
+  o->ngrids = p.levels == 2 ? 3 : 5;
+  o->geq = p.geq_order;
+  o->nx0 = p.nx0;
+  o->ny0 = p.ny0;
+  o->nu_lu = nu;
+  o->alpha_lu = alpha;
+  o->beta_eff_coarse = p.ra * nu * alpha / (lz * lz * lz);
+  o->beta_eff_fine = 0.5 * o->beta_eff_coarse;
+  o->tau_coarse = relax_from(nu, alpha);
+  o->tau_fine = relax_from(2.0 * nu, 2.0 * alpha);
+  o->tau_sub = relax_from(4.0 * nu, 4.0 * alpha);
+  o->xi_a = rescaling(o->tau_coarse, o->tau_fine, 2, p.odd_exact != 0);
+  o->xi_b = rescaling(o->tau_fine, o->tau_sub, 2, p.odd_exact != 0);
+  o->nu_start_mult = std::fmax(p.nu_start_mult, 1.0);
+  o->spin_up_steps = std::max<uint64_t>(p.ny0 / 10, 2000);
+  o->time_step = 0;
+
+  auto fail_free = [&](const std::string& m) {
+    free_octree(o);
+    delete o;
+    return ofail(nullptr, CAVLBM_E_HIP, m);
+  };
+  if (hipStreamCreateWithFlags(&o->st, hipStreamNonBlocking) != hipSuccess)
+    return fail_free("hipStreamCreate failed");
+  if (hipEventCreate(&o->ev0) != hipSuccess || hipEventCreate(&o->ev1) != hipSuccess)
+    return fail_free("hipEventCreate failed");
+
+  // grids (solver.rs:411-462): blocks [0, bl) and [nx0-1-bl, nx0-1) over the
+  // full height [0, ny0-1); strips [1, 3) / [nx1-4, nx1-2) x [1, ny1-2) of a
+  // block, with the wall column and the bottom row kept out of restriction.
+  const size_t bl = p.bl_width;
+  o->hg[1] = build_grid(p.nx0, p.ny0, 0, 0, bl, p.ny0 - 1, false, false, false, false);
+  o->hg[2] = build_grid(p.nx0, p.ny0, p.nx0 - 1 - bl, 0, p.nx0 - 1, p.ny0 - 1, false, false,
+                        false, false);
+  if (p.levels == 3) {
+    for (int b = 1; b <= 2; ++b) {
+      const HostGrid& pg = o->hg[b];
+      const bool hot = b == 1;
+      const size_t sbx0 = hot ? 1 : pg.nx1 - 4, sbx1 = hot ? 3 : pg.nx1 - 2;
+      o->hg[b + 2] = build_grid(pg.nx1, pg.ny1, sbx0, 1, sbx1, pg.ny1 - 2, true, false, true, false);
+    }
+  }
+  size_t dims[5][2] = {{p.nx0, p.ny0}};
+  for (int gi = 1; gi < o->ngrids; ++gi) {
+    dims[gi][0] = o->hg[gi].nx1;
+    dims[gi][1] = o->hg[gi].ny1;
+  }
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    OctLevel& L = o->lv[gi];
+    L.nx = (int)dims[gi][0];
+    L.ny = (int)dims[gi][1];
+    L.plane = (long long)L.nx * L.ny;
+    L.m = dalloc<double>(o, (size_t)kOctM * L.plane);
+    L.c = dalloc<double>(o, (size_t)18 * L.plane);
+    if (!L.m || !L.c) return fail_free("device allocation failed");
+    if (hipMemsetAsync(L.m, 0, (size_t)kOctM * L.plane * sizeof(double), o->st) != hipSuccess)
+      return fail_free("device memset failed");
+  }
+  for (int gi = 1; gi < o->ngrids; ++gi) {
+    const HostGrid& h = o->hg[gi];
+    int* gk = dalloc<int>(o, h.gk.size());
+    int* gkc = dalloc<int>(o, h.gkc.size());
+    double* gw = dalloc<double>(o, h.gw.size());
+    int* ck = dalloc<int>(o, h.ck.size());
+    int* ckf = dalloc<int>(o, h.ckf.size());
+    double* cw = dalloc<double>(o, h.cw.size());
+    if (!gk || !gkc || !gw || !ck || !ckf || !cw) return fail_free("device allocation failed");
+    bool ok = true;
+    ok &= hipMemcpyAsync(gk, h.gk.data(), h.gk.size() * 4, hipMemcpyHostToDevice, o->st) == hipSuccess;
+    ok &= hipMemcpyAsync(gkc, h.gkc.data(), h.gkc.size() * 4, hipMemcpyHostToDevice, o->st) == hipSuccess;
+    ok &= hipMemcpyAsync(gw, h.gw.data(), h.gw.size() * 8, hipMemcpyHostToDevice, o->st) == hipSuccess;
+    if (!h.ck.empty()) {
+      ok &= hipMemcpyAsync(ck, h.ck.data(), h.ck.size() * 4, hipMemcpyHostToDevice, o->st) == hipSuccess;
+      ok &= hipMemcpyAsync(ckf, h.ckf.data(), h.ckf.size() * 4, hipMemcpyHostToDevice, o->st) == hipSuccess;
+      ok &= hipMemcpyAsync(cw, h.cw.data(), h.cw.size() * 8, hipMemcpyHostToDevice, o->st) == hipSuccess;
+    }
+    if (!ok) return fail_free("table upload failed");
+    o->gt[gi] = OctGhostTab{(int)h.gk.size(), gk, gkc, gw};
+    o->ct[gi] = OctCoincTab{(int)h.ck.size(), ck, ckf, cw};
+  }
+
+  // init_cavity (solver.rs:500-575): coarse T linear in x plus the xorshift64*
+  // noise in (iy outer, ix inner) order; fine levels take its bilinear
+  // interpolation at their node positions; rho = 1, u = 0, equilibria.
+  std::vector<std::vector<double>> temp(o->ngrids);
+  temp[0].resize(p.nx0 * p.ny0);
+  uint64_t rng = p.seed | 1ULL;
+  for (size_t iy = 0; iy < p.ny0; ++iy)
+    for (size_t ix = 0; ix < p.nx0; ++ix) {
+      uint64_t x = rng;
+      x ^= x >> 12;
+      x ^= x << 25;
+      x ^= x >> 27;
+      rng = x;
+      const double u01 = (double)((x * 0x2545F4914F6CDD1DULL) >> 11) / (double)(1ULL << 53);
+      const double xf = (double)ix / (double)(p.nx0 - 1);
+      const double t_lin = kTHot + (kTCold - kTHot) * xf;
+      const double pert = 1.0e-3 * (2.0 * u01 - 1.0);
+      temp[0][iy * p.nx0 + ix] = t_lin + pert;
+    }
+  auto interp = [&](double x_c, double y_c) {
+    int k[4];
+    double w[4];
+    bilinear(x_c, y_c, p.nx0, p.ny0, k, w);
+    double t = 0.0;
+    for (int q = 0; q < 4; ++q) t += w[q] * temp[0][k[q]];
+    return t;
+  };
+  for (int b = 1; b <= 2; ++b) {
+    const HostGrid& h = o->hg[b];
+    temp[b].resize(h.nx1 * h.ny1);
+    for (size_t bb = 0; bb < h.ny1; ++bb)
+      for (size_t a = 0; a < h.nx1; ++a)
+        temp[b][bb * h.nx1 + a] =
+            interp((double)h.bx0 - 0.5 + 0.5 * (double)a, (double)h.by0 - 0.5 + 0.5 * (double)bb);
+    if (p.levels == 3) {
+      const HostGrid& s = o->hg[b + 2];
+      temp[b + 2].resize(s.nx1 * s.ny1);
+      for (size_t b2 = 0; b2 < s.ny1; ++b2)
+        for (size_t a2 = 0; a2 < s.nx1; ++a2)
+          temp[b + 2][b2 * s.nx1 + a2] =
+              interp((double)h.bx0 - 0.5 + 0.5 * ((double)s.bx0 - 0.5 + 0.5 * (double)a2),
+                     (double)h.by0 - 0.5 + 0.5 * ((double)s.by0 - 0.5 + 0.5 * (double)b2));
+    }
+  }
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const OctLevel& L = o->lv[gi];
+    std::vector<double> one((size_t)L.plane, 1.0);
+    if (copy_up(o, one.data(), L.m + kOctRho * L.plane, one.size()) ||
+        copy_up(o, temp[gi].data(), L.m + kOctT * L.plane, temp[gi].size()))
+      return fail_free(o->err);
+    if (launch_oct_init_eq(L, o->geq, o->st) != hipSuccess) return fail_free("init kernel failed");
+  }
+  if (hipStreamSynchronize(o->st) != hipSuccess) return fail_free("init failed");
+  *out = o;
+  return 0;
+}
+
+int cavlbm_octree_destroy(cavlbm_octree* o) {
+  if (!o) return 0;
+  (void)hipSetDevice(o->p.device);
+  if (o->st) (void)hipStreamSynchronize(o->st);
+  free_octree(o);
+  delete o;
+  return 0;
+}
+
+int cavlbm_octree_step(cavlbm_octree* o) { return cavlbm_octree_run(o, 1); }
+
+int cavlbm_octree_run(cavlbm_octree* o, uint64_t nsteps) {
+  if (!o) return CAVLBM_E_PARAM;
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  return run_steps(o, nsteps);
+}
+
+int cavlbm_octree_synchronize(cavlbm_octree* o) {
+  if (!o) return CAVLBM_E_PARAM;
+  OHIP_TRY(o, hipStreamSynchronize(o->st));
+  return 0;
+}
+
+int cavlbm_octree_time_run(cavlbm_octree* o, uint64_t nsteps, double* ms) {
+  if (!o || !ms) return CAVLBM_E_PARAM;
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  OHIP_TRY(o, hipEventRecord(o->ev0, o->st));
+  int rc = run_steps(o, nsteps);
+  if (rc) return rc;
+  OHIP_TRY(o, hipEventRecord(o->ev1, o->st));
+  OHIP_TRY(o, hipEventSynchronize(o->ev1));
+  float t = 0.f;
+  OHIP_TRY(o, hipEventElapsedTime(&t, o->ev0, o->ev1));
+  *ms = t;
+  return 0;
+}
+
+int cavlbm_octree_time_step(const cavlbm_octree* o, uint64_t* out) {
+  if (!o || !out) return CAVLBM_E_PARAM;
+  *out = o->time_step;
+  return 0;
+}
+
+int cavlbm_octree_derived_params(const cavlbm_octree* o, cavlbm_octree_derived* d) {
+  if (!o || !d) return CAVLBM_E_PARAM;
+  std::memset(d, 0, sizeof(*d));
+  d->nu_lu = o->nu_lu;
+  d->alpha_lu = o->alpha_lu;
+  d->beta_eff_coarse = o->beta_eff_coarse;
+  d->beta_eff_fine = o->beta_eff_fine;
+  const Relax r[3] = {o->tau_coarse, o->tau_fine, o->tau_sub};
+  for (int l = 0; l < 3; ++l) {
+    d->tau[l][0] = r[l].tp;
+    d->tau[l][1] = r[l].tm;
+    d->tau[l][2] = r[l].tt;
+  }
+  d->xi_a[0] = o->xi_a.even;
+  d->xi_a[1] = o->xi_a.odd;
+  d->xi_a[2] = o->xi_a.thermal;
+  d->xi_b[0] = o->xi_b.even;
+  d->xi_b[1] = o->xi_b.odd;
+  d->xi_b[2] = o->xi_b.thermal;
+  d->spin_up_steps = o->spin_up_steps;
+  uint64_t total = 0;
+  for (int gi = 0; gi < o->ngrids; ++gi) total += (uint64_t)o->lv[gi].plane;
+  d->n_nodes_total = total;
+  d->levels = o->levels;
+  d->n_grids = o->ngrids;
+  return 0;
+}
+
+int cavlbm_octree_grid_dims(const cavlbm_octree* o, int32_t grid, uint64_t* nx, uint64_t* ny) {
+  if (!o || !nx || !ny || !grid_ok(o, grid)) return CAVLBM_E_PARAM;
+  *nx = (uint64_t)o->lv[grid].nx;
+  *ny = (uint64_t)o->lv[grid].ny;
+  return 0;
+}
+
+int cavlbm_octree_get_field(cavlbm_octree* o, int32_t grid, int32_t field, double* out,
+                            uint64_t n) {
+  if (!o || !out || !grid_ok(o, grid)) return CAVLBM_E_PARAM;
+  const int pl = plane_of_field(field);
+  if (pl < 0) return ofail(o, CAVLBM_E_PARAM, "field not held by the octree levels");
+  const OctLevel& L = o->lv[grid];
+  if (n != (uint64_t)L.plane) return ofail(o, CAVLBM_E_SIZE, "nx*ny values expected");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  return copy_down(o, L.m + (size_t)pl * L.plane, out, n);
+}
+
+int cavlbm_octree_set_field(cavlbm_octree* o, int32_t grid, int32_t field, const double* in,
+                            uint64_t n) {
+  if (!o || !in || !grid_ok(o, grid)) return CAVLBM_E_PARAM;
+  const int pl = plane_of_field(field);
+  if (pl < 0) return ofail(o, CAVLBM_E_PARAM, "field not held by the octree levels");
+  const OctLevel& L = o->lv[grid];
+  if (n != (uint64_t)L.plane) return ofail(o, CAVLBM_E_SIZE, "nx*ny values expected");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  return copy_up(o, in, L.m + (size_t)pl * L.plane, n);
+}
+
+int cavlbm_octree_get_pops(cavlbm_octree* o, int32_t grid, int32_t which, double* out,
+                           uint64_t n9) {
+  if (!o || !out || !grid_ok(o, grid) || (which != 0 && which != 1)) return CAVLBM_E_PARAM;
+  const OctLevel& L = o->lv[grid];
+  const size_t n = (size_t)L.plane;
+  if (n9 != 9 * n) return ofail(o, CAVLBM_E_SIZE, "9*nx*ny values expected");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  std::vector<double> soa(9 * n);
+  int rc = copy_down(o, L.m + (size_t)which * 9 * n, soa.data(), soa.size());
+  if (rc) return rc;
+  for (size_t k = 0; k < n; ++k)
+    for (int i = 0; i < 9; ++i) out[k * 9 + i] = soa[(size_t)i * n + k];
+  return 0;
+}
+
+int cavlbm_octree_set_pops(cavlbm_octree* o, int32_t grid, int32_t which, const double* in,
+                           uint64_t n9) {
+  if (!o || !in || !grid_ok(o, grid) || (which != 0 && which != 1)) return CAVLBM_E_PARAM;
+  const OctLevel& L = o->lv[grid];
+  const size_t n = (size_t)L.plane;
+  if (n9 != 9 * n) return ofail(o, CAVLBM_E_SIZE, "9*nx*ny values expected");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  std::vector<double> soa(9 * n);
+  for (size_t k = 0; k < n; ++k)
+    for (int i = 0; i < 9; ++i) soa[(size_t)i * n + k] = in[k * 9 + i];
+  return copy_up(o, soa.data(), L.m + (size_t)which * 9 * n, soa.size());
+}
+
+int cavlbm_octree_set_from_snapshot(cavlbm_octree* o, const double* rho, const double* ux,
+                                    const double* uy, const double* t, uint64_t total) {
+  if (!o || !rho || !ux || !uy || !t) return CAVLBM_E_PARAM;
+  uint64_t need = 0;
+  for (int gi = 0; gi < o->ngrids; ++gi) need += (uint64_t)o->lv[gi].plane;
+  if (total != need) return ofail(o, CAVLBM_E_SIZE, "fields of every grid expected");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  size_t off = 0;
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const OctLevel& L = o->lv[gi];
+    const size_t n = (size_t)L.plane;
+    int rc = copy_up(o, rho + off, L.m + kOctRho * n, n);
+    if (!rc) rc = copy_up(o, ux + off, L.m + kOctUx * n, n);
+    if (!rc) rc = copy_up(o, uy + off, L.m + kOctUy * n, n);
+    if (!rc) rc = copy_up(o, t + off, L.m + kOctT * n, n);
+    if (rc) return rc;
+    OHIP_TRY(o, launch_oct_init_eq(L, o->geq, o->st));
+    OHIP_TRY(o, launch_oct_walls(L, stream_rules(o, gi), o->st));
+    off += n;
+  }
+  OHIP_TRY(o, hipStreamSynchronize(o->st));
+  o->time_step = 0;
+  return 0;
+}
+
+int cavlbm_octree_diagnostics(cavlbm_octree* o, cavlbm_octree_diag* d) {
+  if (!o || !d) return CAVLBM_E_PARAM;
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  std::memset(d, 0, sizeof(*d));
+  std::vector<std::vector<double>> m(o->ngrids);
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const OctLevel& L = o->lv[gi];
+    m[gi].resize((size_t)kOctM * L.plane);
+    int rc = copy_down(o, L.m, m[gi].data(), m[gi].size());
+    if (rc) return rc;
+  }
+  auto fld = [&](int gi, int pl) { return m[gi].data() + (size_t)pl * o->lv[gi].plane; };
+  const size_t nx = o->nx0, ny = o->ny0, n0 = nx * ny;
+  const double* tc = fld(0, kOctT);
+  // cavity.rs:672-715 (Nusselt on the coarse level, scale Lx = nx0 - 1)
+  double sh = 0.0, sc = 0.0, sl = 0.0;
+  for (size_t iy = 0; iy < ny; ++iy) {
+    sh += std::fabs(tc[iy * nx + 1] - tc[iy * nx + 2]);
+    sl += std::fabs(tc[iy * nx + 1] - tc[iy * nx]);
+    sc += std::fabs(tc[iy * nx + nx - 2] - tc[iy * nx + nx - 3]);
+  }
+  const double dT = kTHot - kTCold;
+  d->nusselt_hot = sh / (double)ny * (double)(nx - 1) / dT;
+  d->nusselt_cold = sc / (double)ny * (double)(nx - 1) / dT;
+  d->nusselt_hot_link0 = sl / (double)ny * (double)(nx - 1) / dT;
+  {  // cavity.rs:728-740 on the hot block
+    const OctLevel& F = o->lv[1];
+    const double* tf = fld(1, kOctT);
+    double s = 0.0;
+    for (int b = 1; b + 1 < F.ny; ++b)
+      s += std::fabs(tf[b * F.nx + 1] - tf[b * F.nx + 2]) * 2.0;
+    d->nusselt_hot_fine = s / (double)(F.ny - 2) * (double)(nx - 1) / dT;
+  }
+  // noise_stats (solver.rs:949-1000)
+  const double* rc = fld(0, kOctRho);
+  double rs = 0.0;
+  for (size_t k = 0; k < n0; ++k) rs += rc[k];
+  const double rho_bar = rs / (double)n0;
+  double dev = 0.0;
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const double* r = fld(gi, kOctRho);
+    for (long long k = 0; k < o->lv[gi].plane; ++k) dev = std::fmax(dev, std::fabs(r[k] - rho_bar));
+  }
+  double devi = 0.0;
+  for (int b = 1; b <= 2; ++b) {
+    const double* rf = fld(b, kOctRho);
+    for (int k : o->hg[b].gk) devi = std::fmax(devi, std::fabs(rf[k] - rho_bar));
+    for (int k : o->hg[b].ck) devi = std::fmax(devi, std::fabs(rc[k] - rho_bar));
+    if (o->levels == 3) {
+      const double* rs2 = fld(b + 2, kOctRho);
+      for (int k : o->hg[b + 2].gk) devi = std::fmax(devi, std::fabs(rs2[k] - rho_bar));
+      for (int k : o->hg[b + 2].ck) devi = std::fmax(devi, std::fabs(rf[k] - rho_bar));
+    }
+  }
+  d->max_rho_dev = dev;
+  d->max_rho_dev_ifaces = devi;
+  d->masse = rho_bar;
+  // max_u_all and check_guards (solver.rs:1003-1041)
+  double mu = 0.0;
+  bool finite = true;
+  // levels in the reference's order: coarse, then each block followed by its strip
+  int order[5] = {0, 1, 3, 2, 4};
+  for (int j = 0; j < o->ngrids; ++j) {
+    const int gi = o->levels == 3 ? order[j] : j;
+    const double* u = fld(gi, kOctUx);
+    const double* v = fld(gi, kOctUy);
+    for (long long k = 0; k < o->lv[gi].plane; ++k)
+      mu = std::fmax(mu, std::fmax(std::fabs(u[k]), std::fabs(v[k])));
+    for (int pl = 0; pl < kOctFy && finite; ++pl) {
+      const double* a = fld(gi, pl);
+      for (long long k = 0; k < o->lv[gi].plane; ++k)
+        if (!std::isfinite(a[k])) {
+          finite = false;
+          break;
+        }
+    }
+  }
+  d->max_u_all = mu;
+  d->guard = mu > kMaxUAbort ? 1 : (finite ? 0 : 2);
+  d->time_step = o->time_step;
+  return 0;
+}
+
+int cavlbm_octree_set_dynamic_sgs(cavlbm_octree* o, int32_t enable, double, double, double) {
+  if (!o) return CAVLBM_E_PARAM;
+  if (enable)
+    return ofail(o, CAVLBM_E_UNSUPPORTED, "the octree's dynamic SGS is not on the GPU path");
+  return 0;
+}
+
+const char* cavlbm_octree_last_error(const cavlbm_octree* o) {
+  return o ? o->err.c_str() : g_oct_create_error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,366 @@
+// lbm_octree.hip — per-level kernels and interface transfers of the octree
+// (multi-level) cavity solver.  Reference: bbrangeo/cfd_cavity
+// src/octree/cavity.rs:167-656 (per-level phases), interface.rs:164-229
+// (transfers), solver.rs:1188-1206 (moment refresh of restricted nodes).
+//
+// A level sub-step is two launches: k_oct_collide (M-form -> C-form: TRT+Guo
+// collision on the level's window, plain copy elsewhere) and k_oct_stream
+// (C-form -> M-form: pull streaming with the level's wall rules, the thermal
+// boundary conditions resolved per node, the moments with the Guo correction
+// and the wall velocities).  The transfers work on M-forms.  Built with
+// -ffp-contract=off: the results are bitwise those of a CPU evaluation in
+// the reference's operand order.
+#include "lbm_octree.hpp"
+
+namespace cavlbm {
+
+namespace {
+
+constexpr int kOctBlock = 256;
+
+__device__ __forceinline__ int node_id(long long n) {
+  const long long k = (long long)blockIdx.x * kOctBlock + threadIdx.x;
+  return k < n ? (int)k : -1;
+}
+
+// Sums of update_macroscopic_guo / refresh_moments (cavity.rs:296-315,
+// interface.rs:213-229): products by 0 and -1 are exact and the zero ones
+// leave the +0-started accumulators unchanged.
+struct Moments {
+  double r, mx, my, t;
+};
+__device__ __forceinline__ Moments moments(const double (&f)[9], const double (&g)[9]) {
+  Moments m;
+  m.r = 0.0 + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  m.mx = 0.0 + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
+  m.my = 0.0 + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
+  m.t = 0.0 + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+  return m;
+}
+
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_init_eq(OctLevel L) {
+  const int k = node_id(L.plane);
+  if (k < 0) return;
+  const long long P = L.plane;
+  double* m = L.m;
+  const double r = m[kOctRho * P + k], u = m[kOctUx * P + k], v = m[kOctUy * P + k],
+               t = m[kOctT * P + k];
+  EqTerms<double> e;
+  eq_terms<double>(u, v, e);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    m[i * P + k] = eq_quad<double>(i, r, e);
+    m[(9 + i) * P + k] = eq_g<double, GEQ>(i, t, e);
+  }
+}
+
+__device__ __forceinline__ bool is_wall_node(const OctLevel& L, const OctStream& s, int ix,
+                                             int iy) {
+  if (!s.block) return ix == 0 || ix == L.nx - 1 || iy == 0 || iy == L.ny - 1;
+  return (s.hot && ix == 1) || (s.cold && ix == L.nx - 2) || iy == 1 || iy == L.ny - 2;
+}
+
+// enforce_wall_velocity(_block) (cavity.rs:604-650) alone, after an injection
+// of fields (solver.rs:1160-1168).
+__global__ __launch_bounds__(kOctBlock) void k_oct_walls(OctLevel L, OctStream s) {
+  const int k = node_id(L.plane);
+  if (k < 0) return;
+  const int ix = k % L.nx, iy = k / L.nx;
+  if (is_wall_node(L, s, ix, iy)) {
+    L.m[kOctUx * L.plane + k] = 0.0;
+    L.m[kOctUy * L.plane + k] = 0.0;
+  }
+}
+
+// collide_cavity (cavity.rs:167-243): collide_node_cavity is the uniform
+// collide_node with nu_t = 0 (Cs = 0): tau+ = 0.5 + 3 nu, tau- = 0.5 +
+// (3/16)/(tau+ - 0.5), tau_t = 0.5 + 3 alpha, a_g = 1/tau_t, b_g = 0, all
+// bitwise the level's RelaxTimes.  Nodes outside the window keep their
+// populations (ghost layers carry the injected values).
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctLevel L, OctCollide p) {
+  const int k = node_id(L.plane);
+  if (k < 0) return;
+  const long long P = L.plane;
+  const int ix = k % L.nx, iy = k / L.nx;
+  const bool in = !p.interior_only ||
+                  (ix >= 1 && ix <= L.nx - 2 && iy >= 1 && iy <= L.ny - 2);
+  if (!in) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) L.c[i * P + k] = L.m[i * P + k];
+    return;
+  }
+  double f[9], g[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    f[i] = L.m[i * P + k];
+    g[i] = L.m[(9 + i) * P + k];
+  }
+  double fy, nut;
+  collide_node<double, GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
+                            L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx, 0.0, fy, nut);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    L.c[i * P + k] = f[i];
+    L.c[(9 + i) * P + k] = g[i];
+  }
+  L.m[kOctFy * P + k] = fy;
+}
+
+// Pull source rule of one direction for an interior node: the block form
+// bounces back from wall-side ghost layers (cavity.rs:423-446), the coarse
+// form from outside the domain (lbm.rs:143-171).
+__device__ __forceinline__ bool bounce(const OctLevel& L, const OctStream& s, int sx, int sy) {
+  if (!s.block) return !(sx >= 0 && sx < L.nx && sy >= 0 && sy < L.ny);
+  return (sx == 0 && s.wl) || (sx == L.nx - 1 && s.wr) || (sy == 0 && s.wb) ||
+         (sy == L.ny - 1 && s.wt);
+}
+
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctLevel L, OctStream s) {
+  const int k = node_id(L.plane);
+  if (k < 0) return;
+  const long long P = L.plane;
+  const int nx = L.nx, ny = L.ny;
+  const int ix = k % nx, iy = k / nx;
+  const double* C = L.c;
+  double f[9], g[9];
+
+  // f: pull; the block form leaves its ghost layers as they are
+  const bool interior = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
+  if (!s.block || interior) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const int sx = ix - kCx[i], sy = iy - kCy[i];
+      f[i] = bounce(L, s, sx, sy) ? C[kOpp[i] * P + k] : C[i * P + sy * nx + sx];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = C[i * P + k];
+  }
+
+  // g: the thermal boundary conditions resolved per node.  Every node ends
+  // with the wall equilibrium or with the streamed g of one interior node:
+  //  coarse (cavity.rs:500-533): columns 0 / nx-1 isothermal for every row,
+  //    then rows 0 / ny-1 copy rows 1 / ny-2;
+  //  block (cavity.rs:542-600): columns 1 / nx-2 isothermal, rows 1 / ny-2
+  //    copy rows 2 / ny-3, then the ghost columns and rows copy inwards.
+  int gx, gy;
+  int wall = 0;  // +1 hot, -1 cold
+  if (!s.block) {
+    wall = ix == 0 ? 1 : (ix == nx - 1 ? -1 : 0);
+    gx = ix;
+    gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+  } else {
+    const int b1 = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+    gx = ix == 0 ? 1 : (ix == nx - 1 ? nx - 2 : ix);
+    gy = b1 == 1 ? 2 : (b1 == ny - 2 ? ny - 3 : b1);
+    wall = (s.hot && gx == 1) ? 1 : ((s.cold && gx == nx - 2) ? -1 : 0);
+  }
+  if (wall != 0) {
+    EqTerms<double> e0;
+    eq_terms<double>(0.0, 0.0, e0);
+    const double tw = wall > 0 ? kTHot : kTCold;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = eq_g<double, GEQ>(i, tw, e0);
+  } else {
+    const int kg = gy * nx + gx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const int sx = gx - kCx[i], sy = gy - kCy[i];
+      g[i] = bounce(L, s, sx, sy) ? C[(9 + i) * P + kg] : C[(9 + i) * P + sy * nx + sx];
+    }
+  }
+
+  // update_macroscopic_guo (cavity.rs:296-315), then the wall velocities
+  const Moments mo = moments(f, g);
+  double ux = 0.0, uy = 0.0;
+  if (mo.r > kF64Eps) {
+    ux = mo.mx / mo.r;
+    uy = mo.my / mo.r + 0.5 * L.m[kOctFy * P + k] / mo.r;
+  }
+  if (is_wall_node(L, s, ix, iy)) {
+    ux = 0.0;
+    uy = 0.0;
+  }
+  double* M = L.m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    M[i * P + k] = f[i];
+    M[(9 + i) * P + k] = g[i];
+  }
+  M[kOctRho * P + k] = mo.r;
+  M[kOctUx * P + k] = ux;
+  M[kOctUy * P + k] = uy;
+  M[kOctT * P + k] = mo.t;
+}
+
+// coarse_to_fine (interface.rs:164-195): bilinear moments and non-equilibrium
+// of the coarse level, rescaled by xi, injected on the ghost nodes, whose
+// moments are then recomputed from the injected populations.
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctLevel Lc, OctLevel Lf, OctGhostTab t,
+                                                       OctXi xi) {
+  const int j = node_id(t.n);
+  if (j < 0) return;
+  const long long Pc = Lc.plane, Pf = Lf.plane;
+  const double* cm = Lc.m;
+  double rho = 0.0, ux = 0.0, uy = 0.0, temp = 0.0;
+  double fneq[9], gneq[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) fneq[i] = gneq[i] = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const int kc = t.kc[4 * j + q];
+    const double w = t.w[4 * j + q];
+    const double rc = cm[kOctRho * Pc + kc], uc = cm[kOctUx * Pc + kc],
+                 vc = cm[kOctUy * Pc + kc], tc = cm[kOctT * Pc + kc];
+    rho += w * rc;
+    ux += w * uc;
+    uy += w * vc;
+    temp += w * tc;
+    EqTerms<double> e;
+    eq_terms<double>(uc, vc, e);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      fneq[i] += w * (cm[i * Pc + kc] - eq_quad<double>(i, rc, e));
+      gneq[i] += w * (cm[(9 + i) * Pc + kc] - eq_g<double, GEQ>(i, tc, e));
+    }
+  }
+  EqTerms<double> e;
+  eq_terms<double>(ux, uy, e);
+  double f[9], g[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int o = kOpp[i];
+    const double even = 0.5 * (fneq[i] + fneq[o]);
+    const double odd = 0.5 * (fneq[i] - fneq[o]);
+    f[i] = eq_quad<double>(i, rho, e) + (xi.even * even + xi.odd * odd);
+    g[i] = eq_g<double, GEQ>(i, temp, e) + xi.thermal * gneq[i];
+  }
+  const int kf = t.k_fine[j];
+  double* fm = Lf.m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    fm[i * Pf + kf] = f[i];
+    fm[(9 + i) * Pf + kf] = g[i];
+  }
+  const Moments mo = moments(f, g);
+  fm[kOctRho * Pf + kf] = mo.r;
+  fm[kOctUx * Pf + kf] = mo.mx / mo.r;
+  fm[kOctUy * Pf + kf] = mo.my / mo.r;
+  fm[kOctT * Pf + kf] = mo.t;
+}
+
+// fine_to_coarse (interface.rs:200-210 with the 9-node restriction of
+// grid.rs:178-196), then refresh_moments_nodes (solver.rs:1188-1206) of the
+// restricted node.
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctLevel Lf, OctLevel Lc, OctCoincTab t,
+                                                       OctXi xi) {
+  const int j = node_id(t.n);
+  if (j < 0) return;
+  const long long Pc = Lc.plane, Pf = Lf.plane;
+  const double* fm = Lf.m;
+  double rho = 0.0, ux = 0.0, uy = 0.0, temp = 0.0;
+  double fneq[9], gneq[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) fneq[i] = gneq[i] = 0.0;
+  for (int q = 0; q < 9; ++q) {
+    const int kf = t.kf[9 * j + q];
+    const double w = t.w[9 * j + q];
+    const double rf = fm[kOctRho * Pf + kf], uf = fm[kOctUx * Pf + kf],
+                 vf = fm[kOctUy * Pf + kf], tf = fm[kOctT * Pf + kf];
+    rho += w * rf;
+    ux += w * uf;
+    uy += w * vf;
+    temp += w * tf;
+    EqTerms<double> e;
+    eq_terms<double>(uf, vf, e);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      fneq[i] += w * (fm[i * Pf + kf] - eq_quad<double>(i, rf, e));
+      gneq[i] += w * (fm[(9 + i) * Pf + kf] - eq_g<double, GEQ>(i, tf, e));
+    }
+  }
+  EqTerms<double> e;
+  eq_terms<double>(ux, uy, e);
+  double f[9], g[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int o = kOpp[i];
+    const double even = 0.5 * (fneq[i] + fneq[o]);
+    const double odd = 0.5 * (fneq[i] - fneq[o]);
+    f[i] = eq_quad<double>(i, rho, e) + (even / xi.even + odd / xi.odd);
+    g[i] = eq_g<double, GEQ>(i, temp, e) + gneq[i] / xi.thermal;
+  }
+  const int kc = t.k_coarse[j];
+  double* cm = Lc.m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    cm[i * Pc + kc] = f[i];
+    cm[(9 + i) * Pc + kc] = g[i];
+  }
+  const Moments mo = moments(f, g);
+  cm[kOctRho * Pc + kc] = mo.r;
+  cm[kOctT * Pc + kc] = mo.t;
+  cm[kOctUx * Pc + kc] = mo.r > kF64Eps ? mo.mx / mo.r : 0.0;
+  cm[kOctUy * Pc + kc] = mo.r > kF64Eps ? mo.my / mo.r : 0.0;
+}
+
+inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kOctBlock - 1) / kOctBlock)); }
+
+}  // namespace
+
+hipError_t launch_oct_init_eq(const OctLevel& L, int geq, hipStream_t st) {
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_init_eq<1>, grid_for(L.plane), dim3(kOctBlock), 0, st, L);
+  else
+    hipLaunchKernelGGL(k_oct_init_eq<2>, grid_for(L.plane), dim3(kOctBlock), 0, st, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_walls(const OctLevel& L, const OctStream& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_oct_walls, grid_for(L.plane), dim3(kOctBlock), 0, st, L, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_collide(const OctLevel& L, const OctCollide& p, int geq, hipStream_t st) {
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_collide<1>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, p);
+  else
+    hipLaunchKernelGGL(k_oct_collide<2>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_stream(const OctLevel& L, const OctStream& s, int geq, hipStream_t st) {
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_stream<1>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, s);
+  else
+    hipLaunchKernelGGL(k_oct_stream<2>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_coarse_to_fine(const OctLevel& coarse, const OctLevel& fine,
+                                     const OctGhostTab& t, const OctXi& xi, int geq,
+                                     hipStream_t st) {
+  if (t.n == 0) return hipSuccess;
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_c2f<1>, grid_for(t.n), dim3(kOctBlock), 0, st, coarse, fine, t, xi);
+  else
+    hipLaunchKernelGGL(k_oct_c2f<2>, grid_for(t.n), dim3(kOctBlock), 0, st, coarse, fine, t, xi);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_fine_to_coarse(const OctLevel& fine, const OctLevel& coarse,
+                                     const OctCoincTab& t, const OctXi& xi, int geq,
+                                     hipStream_t st) {
+  if (t.n == 0) return hipSuccess;
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_f2c<1>, grid_for(t.n), dim3(kOctBlock), 0, st, fine, coarse, t, xi);
+  else
+    hipLaunchKernelGGL(k_oct_f2c<2>, grid_for(t.n), dim3(kOctBlock), 0, st, fine, coarse, t, xi);
+  return hipGetLastError();
+}
+
+}  // namespace cavlbm

@@ -1,0 +1,69 @@
+// lbm_octree.hpp — device data and launchers of the octree (multi-level)
+// cavity solver (internal to libcavlbm; the public boundary is
+// include/cavlbm.h).  Reference: bbrangeo/cfd_cavity src/octree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_device.hpp"
+
+namespace cavlbm {
+
+// One level in HBM, flat layout k = iy*nx + ix (levels are narrow strips or a
+// small base grid, so no row padding).  M-form (post-stream state, as the
+// reference stores it): 23 planes f_0..f_8, g_0..g_8, rho, ux, uy, T, fy.
+// C-form: the 18 post-collision population planes.
+constexpr int kOctM = 23;
+constexpr int kOctRho = 18, kOctUx = 19, kOctUy = 20, kOctT = 21, kOctFy = 22;
+
+struct OctLevel {
+  int nx, ny;
+  long long plane;  // nx * ny
+  double* m;
+  double* c;
+};
+
+// Relaxation and force of one collision pass (RelaxTimes of the level,
+// beta of the level), handed to the shared collide_node with Smagorinsky off.
+struct OctCollide {
+  StepCtx ctx;
+  int interior_only;  // window [1, n-2]^2 (fine levels) or the whole level
+};
+
+// Streaming / boundary rules of a level.
+struct OctStream {
+  int block;               // 0: coarse level (domain walls), 1: fine block
+  int wl, wr, wb, wt;      // block: wall-side ghost layers (bounce-back sources)
+  int hot, cold;           // block: isothermal wall columns a = 1 / a = nx-2
+};
+
+// Interface tables on the device (grid.rs:95-203), node indices in the flat
+// layouts of the two levels.
+struct OctGhostTab {
+  int n;
+  const int* k_fine;   // n
+  const int* kc;       // 4n
+  const double* w;     // 4n
+};
+struct OctCoincTab {
+  int n;
+  const int* k_coarse;  // n
+  const int* kf;        // 9n
+  const double* w;      // 9n
+};
+struct OctXi {
+  double even, odd, thermal;
+};
+
+hipError_t launch_oct_init_eq(const OctLevel& L, int geq, hipStream_t st);
+hipError_t launch_oct_walls(const OctLevel& L, const OctStream& s, hipStream_t st);
+hipError_t launch_oct_collide(const OctLevel& L, const OctCollide& p, int geq, hipStream_t st);
+hipError_t launch_oct_stream(const OctLevel& L, const OctStream& s, int geq, hipStream_t st);
+hipError_t launch_oct_coarse_to_fine(const OctLevel& coarse, const OctLevel& fine,
+                                     const OctGhostTab& t, const OctXi& xi, int geq,
+                                     hipStream_t st);
+hipError_t launch_oct_fine_to_coarse(const OctLevel& fine, const OctLevel& coarse,
+                                     const OctCoincTab& t, const OctXi& xi, int geq,
+                                     hipStream_t st);
+
+}  // namespace cavlbm

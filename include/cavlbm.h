@@ -281,6 +281,94 @@ int cavlbm_batch_nodes(const cavlbm_batch *b, uint64_t *out);
 int cavlbm_batch_time_run(cavlbm_batch *b, uint64_t nsteps, double *ms);
 const char *cavlbm_batch_last_error(const cavlbm_batch *b);
 
+/* ===========================================================================
+ * Octree (multi-level) cavity: the reference's src/octree CavitySolver
+ * (solver.rs:118-1206) with its per-level phases (cavity.rs:167-656) and
+ * interface transfers (interface.rs:164-229) on the device.  Levels: a coarse
+ * base grid L0 over the whole cavity, two full-height blocks L1 (dx/2) on the
+ * isothermal walls, and with three levels an L2 strip (dx/4) in each block.
+ * Grids are numbered as in the reference's snapshots (solver.rs:1048-1066):
+ * 0 = L0, 1 = hot L1, 2 = cold L1, 3 = hot L2, 4 = cold L2; per-grid arrays
+ * are k = iy*nx + ix with the grid's own nx, ny (ghost layers included).
+ * FP64 only; the dynamic SGS of the octree is not on this path.
+ * ======================================================================== */
+typedef struct cavlbm_octree cavlbm_octree;
+
+/* The arguments of octree CavitySolver::new (solver.rs:340-353). */
+typedef struct {
+  uint64_t nx0, ny0, bl_width;
+  double ra, pr, utarget;
+  uint64_t seed;
+  uint64_t jobs;           /* recorded, ignored on the GPU */
+  int32_t odd_exact;       /* xi_odd = (1/2) tau-_fine / tau-_coarse */
+  int32_t levels;          /* 2 or 3 */
+  double nu_start_mult;
+  int32_t has_nu_override; /* Option<f64> nu_override */
+  int32_t geq_order;       /* 2: quadratic g equilibrium (default), 1: linear */
+  double nu_override;
+  int32_t device;
+  int32_t reserved;
+} cavlbm_octree_params;
+
+/* Calibration and level constants (solver.rs:363-468). */
+typedef struct {
+  double nu_lu, alpha_lu, beta_eff_coarse, beta_eff_fine;
+  double tau[3][3];        /* [L0, L1, L2][tau+, tau-, tau_t] */
+  double xi_a[3], xi_b[3]; /* interfaces L0-L1 and L1-L2: even, odd, thermal */
+  uint64_t spin_up_steps, n_nodes_total;
+  int32_t levels, n_grids;
+} cavlbm_octree_derived;
+
+/* Diagnostics of run_octree's progress lines (cavity.rs:672-790,
+ * solver.rs:949-1041). */
+typedef struct {
+  double nusselt_hot, nusselt_cold, nusselt_hot_link0;
+  double nusselt_hot_fine;  /* hot L1 block, scale lx_coarse = nx0 - 1 */
+  double max_u_all, max_rho_dev, max_rho_dev_ifaces, masse;
+  int32_t guard;            /* check_guards: 0 none, 1 max_u > 0.10, 2 non-finite */
+  int32_t reserved;
+  uint64_t time_step;
+} cavlbm_octree_diag;
+
+/* Defaults of src/cli.rs:121-130 (64x256, bl_width 8, Ra 1e6, Pr 0.71,
+ * utarget 0.02, seed 42, 2 levels) on device 0. */
+int cavlbm_octree_params_default(cavlbm_octree_params *p);
+/* CavitySolver::new + init_cavity (solver.rs:340-575); its asserts are
+ * CAVLBM_E_PARAM. */
+int cavlbm_octree_create(const cavlbm_octree_params *p, cavlbm_octree **out);
+int cavlbm_octree_destroy(cavlbm_octree *o);
+/* step_cavity (solver.rs:647-947); run = n of them. */
+int cavlbm_octree_step(cavlbm_octree *o);
+int cavlbm_octree_run(cavlbm_octree *o, uint64_t nsteps);
+int cavlbm_octree_synchronize(cavlbm_octree *o);
+int cavlbm_octree_time_run(cavlbm_octree *o, uint64_t nsteps, double *ms);
+int cavlbm_octree_time_step(const cavlbm_octree *o, uint64_t *out);
+int cavlbm_octree_derived_params(const cavlbm_octree *o, cavlbm_octree_derived *out);
+int cavlbm_octree_grid_dims(const cavlbm_octree *o, int32_t grid, uint64_t *nx,
+                            uint64_t *ny);
+/* Per-grid state: fields CAVLBM_FIELD_{RHO,UX,UY,T,FY_NODE}, populations
+ * [k][9]; the setters overwrite the state as the reference's public level
+ * fields allow. */
+int cavlbm_octree_get_field(cavlbm_octree *o, int32_t grid, int32_t field, double *out,
+                            uint64_t n);
+int cavlbm_octree_set_field(cavlbm_octree *o, int32_t grid, int32_t field, const double *in,
+                            uint64_t n);
+int cavlbm_octree_get_pops(cavlbm_octree *o, int32_t grid, int32_t which, double *out,
+                           uint64_t n9);
+int cavlbm_octree_set_pops(cavlbm_octree *o, int32_t grid, int32_t which, const double *in,
+                           uint64_t n9);
+/* set_from_snapshot (solver.rs:1095-1175): rho, ux, uy, T of every grid
+ * concatenated in grid order (total = sum of nx*ny); f = feq, g = geq, wall
+ * velocities zeroed on every level, time_step = 0. */
+int cavlbm_octree_set_from_snapshot(cavlbm_octree *o, const double *rho, const double *ux,
+                                    const double *uy, const double *t, uint64_t total);
+int cavlbm_octree_diagnostics(cavlbm_octree *o, cavlbm_octree_diag *out);
+/* CavitySolver::set_dynamic_sgs (solver.rs:475-480): enable != 0 returns
+ * CAVLBM_E_UNSUPPORTED. */
+int cavlbm_octree_set_dynamic_sgs(cavlbm_octree *o, int32_t enable, double smag_alpha,
+                                  double nu_sm_min, double eta_test);
+const char *cavlbm_octree_last_error(const cavlbm_octree *o);
+
 /* Message of the last failure on this handle (or of cavlbm_create when s is
  * NULL).  Valid until the next call on the same handle / thread. */
 const char *cavlbm_last_error(const cavlbm_solver *s);

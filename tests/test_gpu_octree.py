@@ -1,0 +1,193 @@
+"""Octree (multi-level) cavity on the GPU against the octree oracle.
+
+Every level's populations, moments and Guo force must be bitwise equal to the
+oracle's after any number of coarse steps (FP64, no contraction, IEEE
+division, the reference's operand order on both sides); the diagnostics are
+reduced on the host in the reference's order and are bitwise too.  The
+reference's own octree tests (src/octree/tests.rs) are restated on the GPU
+path.
+"""
+import numpy as np
+import pytest
+
+from cfd_cavity_amd import OctreeCavitySolver
+from cfd_cavity_amd._lib import CavlbmError, E_UNSUPPORTED
+from tests.helpers import bits_equal, first_mismatch
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (("rho", "rho"), ("ux", "ux"), ("uy", "uy"), ("temperature", "temperature"),
+          ("fy_node", "fy"))
+
+
+def pair(oracle, *args, **kw):
+    g = OctreeCavitySolver(*args, **kw)
+    o = oracle.OracleOctree(*args, **kw)
+    return g, o
+
+
+def assert_bitwise(g, o, what=""):
+    assert g.n_grids == len(o.grids)
+    for gi, lv in enumerate(o.grids):
+        assert g.dims[gi] == (lv.nx, lv.ny)
+        for name, oname in FIELDS:
+            got, ref = g.get_field(gi, name), getattr(lv, oname)
+            assert bits_equal(got, ref), f"{what} grid {gi} {name}: {first_mismatch(got, ref)}"
+        for w in ("f", "g"):
+            got, ref = g.get_pops(gi, w), getattr(lv, w)
+            assert bits_equal(got, ref), f"{what} grid {gi} pops {w}: {first_mismatch(got, ref)}"
+
+
+def assert_diag_equal(g, o):
+    d = g.diagnostics()
+    n = o.noise_stats()
+    assert d["nusselt_hot"] == o.nusselt_hot()
+    assert d["nusselt_cold"] == o.nusselt_cold()
+    assert d["nusselt_hot_link0"] == o.nusselt_hot_link0()
+    assert d["nusselt_hot_fine"] == o.nusselt_hot_fine(0, g.nx0 - 1.0)
+    assert d["max_rho_dev"] == n["max_rho_dev"]
+    assert d["max_rho_dev_ifaces"] == n["max_rho_dev_ifaces"]
+    assert d["masse"] == n["masse"]
+    assert d["max_u_all"] == o.max_u_all()
+    assert d["guard"] == o.check_guards()
+    assert d["time_step"] == o.time_step
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_initial_state_and_constants_bitwise(oracle, levels):
+    g, o = pair(oracle, 32, 64, 4, 1.0e6, levels=levels)
+    assert_bitwise(g, o, "init")
+    d, od = g.derived(), o.derived()
+    assert d["nu_lu"] == od["nu_lu"] and d["alpha_lu"] == od["alpha_lu"]
+    assert d["beta_eff_coarse"] == od["beta_eff_coarse"]
+    assert d["beta_eff_fine"] == od["beta_eff_fine"]
+    assert d["spin_up_steps"] == od["spin_up_steps"]
+    assert d["n_nodes_total"] == od["n_nodes_total"]
+    for lvl, name in enumerate(("l0", "l1", "l2")):
+        assert d["tau"][lvl] == [od[f"tau_plus_{name}"], od[f"tau_minus_{name}"],
+                                 od[f"tau_t_{name}"]]
+    assert d["xi_a"] == [od["xi_even_a"], od["xi_odd_a"], od["xi_thermal_a"]]
+    assert d["xi_b"] == [od["xi_even_b"], od["xi_odd_b"], od["xi_thermal_b"]]
+
+
+@pytest.mark.parametrize("args,kw,nsteps", [
+    ((32, 64, 4, 1.0e6), dict(levels=2), 1),
+    ((32, 64, 4, 1.0e6), dict(levels=2), 40),
+    ((32, 64, 4, 1.0e6), dict(levels=3), 25),
+    ((32, 64, 4, 1.0e6), dict(levels=3, odd_exact=True), 10),
+    ((32, 64, 4, 1.0e6), dict(levels=2, nu_start_mult=3.0), 30),
+    ((64, 256, 8, 1.0e6), dict(levels=2), 200),
+    ((64, 256, 8, 1.0e7), dict(levels=3, seed=7), 60),
+    ((32, 64, 4, 1.0e6), dict(levels=2, geq_order=1), 15),
+])
+def test_steps_bitwise(oracle, args, kw, nsteps):
+    g, o = pair(oracle, *args, **kw)
+    g.run(nsteps)
+    o.run(nsteps)
+    assert g.time_step == nsteps
+    assert_bitwise(g, o, f"{args} {kw} after {nsteps}")
+    assert_diag_equal(g, o)
+
+
+def test_single_steps_equal_graph_run(oracle):
+    a = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    b = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    for _ in range(12):
+        a.step_cavity()
+        a.diagnostics()  # interleaved reads do not perturb the state
+    b.run(12)
+    for gi in range(a.n_grids):
+        for w in ("f", "g"):
+            assert bits_equal(a.get_pops(gi, w), b.get_pops(gi, w))
+
+
+def test_random_state_step_bitwise(oracle):
+    """A developed state with random non-equilibrium on every level, loaded
+    into both, then stepped: every phase and transfer sees non-trivial data."""
+    g, o = pair(oracle, 32, 64, 4, 1.0e6, levels=3)
+    g.run(30)
+    o.run(30)
+    rng = np.random.default_rng(21)
+    for gi, lv in enumerate(o.grids):
+        n = lv.nx * lv.ny
+        lv.f[:] = lv.f * (1.0 + 1e-3 * rng.uniform(-1, 1, (n, 9)))
+        lv.g[:] = lv.g + 1e-4 * rng.uniform(-1, 1, (n, 9))
+        lv.ux[:] = lv.ux + 1e-3 * rng.uniform(-1, 1, n)
+        lv.fy[:] = lv.fy * (1.0 + 0.1 * rng.uniform(-1, 1, n))
+        for w in ("f", "g"):
+            g.set_pops(gi, w, getattr(lv, w))
+        for name, oname in FIELDS:
+            g.set_field(gi, name, getattr(lv, oname))
+    assert_bitwise(g, o, "loaded")
+    for _ in range(3):
+        g.step_cavity()
+        o.step()
+    assert_bitwise(g, o, "random state, 3 steps")
+    assert_diag_equal(g, o)
+
+
+def test_set_from_snapshot_bitwise(oracle):
+    g, o = pair(oracle, 32, 64, 4, 1.0e6, levels=3)
+    g.run(20)
+    o.run(20)
+    fl = g.level_fields()
+    rho, ux, uy, t = (np.concatenate([f[2 + q] for f in fl]) for q in range(4))
+    rng = np.random.default_rng(4)
+    ux = ux + 1e-3 * rng.uniform(-1, 1, ux.size)
+    g.set_from_snapshot(rho, ux, uy, t)
+    o.set_from_snapshot(rho, ux, uy, t)
+    assert g.time_step == 0
+    assert_bitwise(g, o, "after set_from_snapshot")
+    g.run(15)
+    o.run(15)
+    assert_bitwise(g, o, "restart + 15")
+
+
+# ---------------------------------------------------------------------------
+# reference tests (src/octree/tests.rs) on the GPU path
+
+def test_v2_mass_conserved_with_guo_force():                 # tests.rs:396-416
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6, 0.71, 0.02, 42, 1, False, 2, 1.0, None)
+    for _ in range(10):
+        s.step_cavity()
+        assert abs(s.diagnostics()["masse"] - 1.0) < 1e-6
+
+
+def test_v2_noise_stats_bounded():                           # tests.rs:418-447
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6)
+    for step in range(20):
+        s.step_cavity()
+        n = s.noise_stats()
+        assert n["max_rho_dev"] >= n["max_rho_dev_ifaces"]
+        if step >= 5:
+            assert n["max_rho_dev_ifaces"] < 1.0
+
+
+def test_v2_nusselt_conductive_initial():                    # tests.rs:449-475
+    d = OctreeCavitySolver(32, 64, 4, 1.0e6).diagnostics()
+    assert abs(d["nusselt_hot"] - 1.0) < 0.05 and abs(d["nusselt_cold"] - 1.0) < 0.05
+
+
+def test_v6_three_levels_stable_short():                     # tests.rs:548-575
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    assert s.dims[3][0] == 7 and s.dims[4][0] == 7
+    for _ in range(10):
+        s.step_cavity()
+        d = s.diagnostics()
+        assert abs(d["masse"] - 1.0) < 1e-6
+        assert d["max_rho_dev"] >= d["max_rho_dev_ifaces"] and d["max_rho_dev_ifaces"] < 1.0
+        assert s.check_guards() is None
+
+
+def test_errors():
+    with pytest.raises(ValueError):
+        OctreeCavitySolver(32, 64, 1, 1.0e6)           # solver.rs:356
+    with pytest.raises(ValueError):
+        OctreeCavitySolver(16, 64, 8, 1.0e6)           # solver.rs:357
+    with pytest.raises(ValueError):
+        OctreeCavitySolver(32, 64, 4, 1.0e6, levels=4)
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6)
+    with pytest.raises(CavlbmError) as e:
+        s.set_dynamic_sgs(True)
+    assert e.value.code == E_UNSUPPORTED
+    s.set_dynamic_sgs(False)
