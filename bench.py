@@ -162,6 +162,68 @@ def copy_probe(mib: int, reps: int = 5):
     print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
 
 
+def dynamic_form(steps: int, warmup: int, cpu_s: float):
+    """The dynamic mixed-scale SGS path (solver.rs:562-828) on the same
+    256x1024 workload, FP64: the model's three passes plus the collision per
+    step, against the oracle's dynamic step on a bounded CPU sample."""
+    from oracle import pyoracle
+
+    s = make_solver("fp64")
+    nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
+    s.set_dynamic_sgs(True)
+    s.run(warmup)
+    ms = s.time_run(steps)
+    out = {"steps": steps, "ms_per_step": ms / steps, "mlups": nodes * steps / ms / 1e3,
+           "cs_dyn_mean": s.dynamic_stats()[0]}
+    if cpu_s > 0:
+        threads = min(os.cpu_count() or 1, 16)
+        w = WORKLOAD
+        o = pyoracle.OracleSolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"],
+                                  w["utarget"], w["seed"], threads)
+        o.set_dynamic_sgs(True)
+        o.run(1)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_s:
+            o.run(1)
+            n += 1
+        dt = time.perf_counter() - t0
+        out["cpu_mlups"] = nodes * n / dt / 1e6
+        out["cpu_sample"] = f"{n} dynamic steps, {threads} threads (collision), model serial"
+        out["speedup_vs_cpu"] = out["mlups"] / out["cpu_mlups"]
+    return out
+
+
+def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8)):
+    """The octree cavity (src/octree, the reference CLI's default 64x256 base
+    grid with bl_width 8, Ra=1e6) for 2 or 3 levels: coarse steps per second
+    and node updates per second (a coarse step updates L0 once, the L1 blocks
+    twice and the L2 strips four times), against the octree oracle (one
+    thread) on a bounded CPU sample."""
+    from cfd_cavity_amd import OctreeCavitySolver
+    from oracle import pyoracle
+
+    nx0, ny0, bl = cfg
+    g = OctreeCavitySolver(nx0, ny0, bl, 1.0e6, levels=levels)
+    upd = sum(nx * ny * (1, 2, 2, 4, 4)[gi] for gi, (nx, ny) in enumerate(g.dims))
+    g.run(20)
+    ms = g.time_run(steps)
+    out = {"config": f"{nx0}x{ny0}, bl_width {bl}, Ra=1e6, levels {levels}",
+           "us_per_coarse_step": ms / steps * 1e3, "node_updates_per_step": upd,
+           "mnups": upd * steps / ms / 1e3}
+    if cpu_s > 0:
+        o = pyoracle.OracleOctree(nx0, ny0, bl, 1.0e6, levels=levels)
+        o.run(1)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_s:
+            o.run(5)
+            n += 5
+        dt = time.perf_counter() - t0
+        out["cpu_mnups"] = upd * n / dt / 1e6
+        out["cpu_sample"] = f"{n} coarse steps, oracle/octree_oracle.c, 1 thread"
+        out["speedup_vs_cpu"] = out["mnups"] / out["cpu_mnups"]
+    return out
+
+
 def cpu_baseline(budget_s: float):
     """The oracle (row-band OpenMP restatement of the rayon path) on a bounded
     sample of the same workload: steps from the reference init until the time
@@ -196,6 +258,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the dynamic-SGS and octree measurements")
+    ap.add_argument("--octree-levels", default="2,3", help="octree configurations to measure")
+    ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree",
+                    help="paths to measure (a subset prints a partial line, for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -207,11 +274,16 @@ def main():
               "layout is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
+    paths = set(a.paths.split(","))
+    if "uniform" not in paths:
+        a.forms = ""
+    if "batched" not in paths:
+        a.no_batch = True
     forms = {}
-    for storage in a.forms.split(","):
+    for storage in filter(None, a.forms.split(",")):
         print(f"[bench] {storage}: warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
         forms[storage] = gpu_form(storage, a.steps, a.warmup, a.kernel_launches)
-    main_form = forms.get("fp64") or next(iter(forms.values()))
+    main_form = forms.get("fp64") or next(iter(forms.values()), None)
     ref_nu = forms["fp64"]["nusselt_hot"] if "fp64" in forms else None
     for k, v in forms.items():
         if ref_nu is not None:
@@ -224,10 +296,26 @@ def main():
         print(f"[bench] batched: 64 cavities, {a.batch_steps} steps", file=sys.stderr, flush=True)
         batched = batched_form(a.batch_steps, a.warmup)
 
+    extra = None
+    if not a.no_extra:
+        cs = 0.0 if a.no_cpu else 3.0
+        extra = {}
+        if "dynamic_sgs" in paths:
+            print("[bench] dynamic SGS 256x1024", file=sys.stderr, flush=True)
+            extra["dynamic_sgs"] = dynamic_form(500, 20, cs)
+        if "octree" in paths:
+            for lv in (int(x) for x in a.octree_levels.split(",")):
+                print(f"[bench] octree, {lv} levels", file=sys.stderr, flush=True)
+                extra[f"octree_levels{lv}"] = octree_form(lv, 1000, cs)
+
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and main_form is not None:
         print(f"[bench] cpu baseline ({a.cpu_seconds} s)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(a.cpu_seconds)
+    if main_form is None:  # a partial run for profiling: no headline figures
+        print(json.dumps({"paths": sorted(paths), "batched": batched,
+                          "other_paths": extra}), flush=True)
+        return
 
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     b = main_form["bytes_per_lup"]
@@ -259,6 +347,9 @@ def main():
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
         "batched": batched,
+        "other_paths": ({k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
+                             for kk, vv in v.items()} for k, v in extra.items()}
+                        if extra else None),
     }
     print(json.dumps(line), flush=True)
 

@@ -54,6 +54,7 @@ struct cavlbm_solver {
   bool c_valid = false, m_valid = false;
   uint64_t time_step = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer parity
+  hipGraphExec_t dgraph[2] = {nullptr, nullptr};  // dynamic-SGS steps, by parity
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
@@ -151,6 +152,11 @@ int materialize(cavlbm_solver* s) {
 
 void drop_graphs(cavlbm_solver* s) {
   for (auto& g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  for (auto& g : s->dgraph)
     if (g) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
@@ -259,9 +265,54 @@ hipGraphExec_t capture(cavlbm_solver* s, int par) {
   return ex;
 }
 
+// Captures kGraphSteps dynamic-SGS steps (materialize, the model's three
+// passes, the collision with per-node values) starting from parity `par`.
+hipGraphExec_t capture_dyn(cavlbm_solver* s, int par) {
+  const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);
+  const DynSgsArgs args = dyn_args(s, ctx);
+  const int saved = s->cur;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  s->cur = par;
+  bool ok = true;
+  for (int i = 0; i < kGraphSteps && ok; ++i) {
+    ok = launch(s, MODE_MATERIALIZE, ctx, 0) == hipSuccess &&
+         launch_dynamic_sgs(s->geo, args, s->st) == hipSuccess &&
+         launch(s, MODE_FIRST_PRE, ctx, 0) == hipSuccess;
+    s->cur ^= 1;
+  }
+  s->cur = saved;
+  if (hipStreamEndCapture(s->st, &g) != hipSuccess || !ok) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  return ex;
+}
+
 int run_steps(cavlbm_solver* s, uint64_t n) {
   uint64_t left = n;
   while (left > 0) {
+    // dynamic-SGS steps after the spin-up: graphs of kGraphSteps steps
+    if (s->dyn_on && s->c_valid && !s->graph_failed && s->time_step >= s->d.spin_up_steps &&
+        left > (uint64_t)kGraphSteps) {
+      hipGraphExec_t& g = s->dgraph[s->cur];
+      if (!g) g = capture_dyn(s, s->cur);
+      if (!g) {
+        s->graph_failed = true;
+        continue;
+      }
+      HIP_TRY(s, hipGraphLaunch(g, s->st));
+      s->time_step += kGraphSteps;
+      s->m_valid = false;
+      s->dyn_pre = true;
+      left -= kGraphSteps;
+      continue;
+    }
     // Steady part: whole graphs of kGraphSteps fused steps (constant per-step
     // scalars after spin-up), keeping the final step direct so that it writes
     // nu_t.
@@ -724,6 +775,11 @@ int cavlbm_set_dynamic_sgs(cavlbm_solver* s, int32_t enable, double smag_alpha,
     HIP_TRY(s, hipMemsetAsync(s->dyn, 0, bytes, s->st));
     HIP_TRY(s, hipStreamSynchronize(s->st));
   }
+  for (auto& g : s->dgraph)  // captured with the previous model parameters
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   s->dyn_on = enable != 0;
   s->dyn_alpha = smag_alpha;
   s->dyn_min = nu_sm_min;

@@ -223,15 +223,21 @@ OctCollide collide_params(double beta, double nu, double alpha, const Relax& r, 
   return p;
 }
 
-// One level sub-step: collide_cavity, then stream + BCs + moments + walls.
-hipError_t substep(cavlbm_octree* o, int gi, const OctCollide& c) {
-  hipError_t e = launch_oct_collide(o->lv[gi], c, o->geq, o->st);
+// One sub-step of `count` levels of the same kind (the two blocks, or the two
+// strips): collide_cavity, then stream + BCs + moments + walls.
+hipError_t substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
+  OctLevel L[2] = {o->lv[g0], o->lv[g0 + (count > 1 ? 1 : 0)]};
+  OctStream S[2] = {stream_rules(o, g0), stream_rules(o, g0 + (count > 1 ? 1 : 0))};
+  hipError_t e = launch_oct_collide(L, count, c, o->geq, o->st);
   if (e != hipSuccess) return e;
-  return launch_oct_stream(o->lv[gi], stream_rules(o, gi), o->geq, o->st);
+  return launch_oct_stream(L, S, count, o->geq, o->st);
 }
 
 // step_cavity (solver.rs:647-947) with the dynamic SGS off: enqueues the
-// launches of the coarse step `step`.
+// launches of the coarse step `step`.  The reference walks the hot block and
+// then the cold one; the two touch disjoint fine levels and disjoint coarse
+// nodes after the coarse-to-fine injection, so both advance in the same
+// launches.
 hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   double nu_eff, alpha_eff;
   ramp(o, step, nu_eff, alpha_eff);
@@ -245,27 +251,26 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   const OctCollide c0 = collide_params(beta_c, nu_eff, alpha_eff, tc, false);
   const OctCollide c1 = collide_params(beta_f, 2.0 * nu_eff, 2.0 * alpha_eff, tf, true);
   const OctCollide c2 = collide_params(beta_s, 4.0 * nu_eff, 4.0 * alpha_eff, ts, true);
+  const OctLevel coarse2[2] = {o->lv[0], o->lv[0]};
+  const OctLevel blocks[2] = {o->lv[1], o->lv[2]};
   hipError_t e;
-#define OCT_Q(x)              \
-  do {                        \
-    e = (x);                  \
+#define OCT_Q(x)                   \
+  do {                             \
+    e = (x);                       \
     if (e != hipSuccess) return e; \
   } while (0)
-  OCT_Q(substep(o, 0, c0));
-  for (int b = 1; b <= 2; ++b)
-    OCT_Q(launch_oct_coarse_to_fine(o->lv[0], o->lv[b], o->gt[b], o->xi_a, o->geq, o->st));
-  for (int b = 1; b <= 2; ++b) {
-    OCT_Q(substep(o, b, c1));
-    if (o->levels == 3) {
-      const int s = b + 2;
-      OCT_Q(launch_oct_coarse_to_fine(o->lv[b], o->lv[s], o->gt[s], o->xi_b, o->geq, o->st));
-      OCT_Q(substep(o, s, c2));
-      OCT_Q(substep(o, s, c2));
-      OCT_Q(launch_oct_fine_to_coarse(o->lv[s], o->lv[b], o->ct[s], o->xi_b, o->geq, o->st));
-    }
-    OCT_Q(substep(o, b, c1));
-    OCT_Q(launch_oct_fine_to_coarse(o->lv[b], o->lv[0], o->ct[b], o->xi_a, o->geq, o->st));
+  OCT_Q(substep(o, 0, 1, c0));
+  OCT_Q(launch_oct_coarse_to_fine(coarse2, blocks, &o->gt[1], 2, o->xi_a, o->geq, o->st));
+  OCT_Q(substep(o, 1, 2, c1));
+  if (o->levels == 3) {
+    const OctLevel strips[2] = {o->lv[3], o->lv[4]};
+    OCT_Q(launch_oct_coarse_to_fine(blocks, strips, &o->gt[3], 2, o->xi_b, o->geq, o->st));
+    OCT_Q(substep(o, 3, 2, c2));
+    OCT_Q(substep(o, 3, 2, c2));
+    OCT_Q(launch_oct_fine_to_coarse(strips, blocks, &o->ct[3], 2, o->xi_b, o->geq, o->st));
   }
+  OCT_Q(substep(o, 1, 2, c1));
+  OCT_Q(launch_oct_fine_to_coarse(blocks, coarse2, &o->ct[1], 2, o->xi_a, o->geq, o->st));
 #undef OCT_Q
   return hipSuccess;
 }

@@ -23,6 +23,32 @@ __device__ __forceinline__ int node_id(long long n) {
   return k < n ? (int)k : -1;
 }
 
+// Up to two independent items (levels or interfaces) per launch: workgroups
+// [0, nblk0) serve item 0, the rest item 1.
+struct OctGroup {
+  OctLevel L[2];
+  OctStream S[2];
+  int nblk0;
+};
+struct OctXferGroup {
+  OctLevel Lc[2], Lf[2];
+  OctGhostTab gt[2];
+  OctCoincTab ct[2];
+  int nblk0;
+};
+
+__device__ __forceinline__ int group_item(int nblk0, int& local_block) {
+  const int b = (int)blockIdx.x;
+  const int w = b >= nblk0 ? 1 : 0;
+  local_block = w ? b - nblk0 : b;
+  return w;
+}
+
+__device__ __forceinline__ int local_id(int local_block, long long n) {
+  const long long k = (long long)local_block * kOctBlock + threadIdx.x;
+  return k < n ? (int)k : -1;
+}
+
 // Sums of update_macroscopic_guo / refresh_moments (cavity.rs:296-315,
 // interface.rs:213-229): products by 0 and -1 are exact and the zero ones
 // leave the +0-started accumulators unchanged.
@@ -79,8 +105,10 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_walls(OctLevel L, OctStream s
 // bitwise the level's RelaxTimes.  Nodes outside the window keep their
 // populations (ghost layers carry the injected values).
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctLevel L, OctCollide p) {
-  const int k = node_id(L.plane);
+__global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctGroup grp, OctCollide p) {
+  int lb;
+  const OctLevel& L = grp.L[group_item(grp.nblk0, lb)];
+  const int k = local_id(lb, L.plane);
   if (k < 0) return;
   const long long P = L.plane;
   const int ix = k % L.nx, iy = k / L.nx;
@@ -118,8 +146,12 @@ __device__ __forceinline__ bool bounce(const OctLevel& L, const OctStream& s, in
 }
 
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctLevel L, OctStream s) {
-  const int k = node_id(L.plane);
+__global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
+  int lb;
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& L = grp.L[item];
+  const OctStream& s = grp.S[item];
+  const int k = local_id(lb, L.plane);
   if (k < 0) return;
   const long long P = L.plane;
   const int nx = L.nx, ny = L.ny;
@@ -200,9 +232,13 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctLevel L, OctStream 
 // of the coarse level, rescaled by xi, injected on the ghost nodes, whose
 // moments are then recomputed from the injected populations.
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctLevel Lc, OctLevel Lf, OctGhostTab t,
-                                                       OctXi xi) {
-  const int j = node_id(t.n);
+__global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi xi) {
+  int lb;
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& Lc = grp.Lc[item];
+  const OctLevel& Lf = grp.Lf[item];
+  const OctGhostTab& t = grp.gt[item];
+  const int j = local_id(lb, t.n);
   if (j < 0) return;
   const long long Pc = Lc.plane, Pf = Lf.plane;
   const double* cm = Lc.m;
@@ -256,9 +292,13 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctLevel Lc, OctLevel Lf,
 // grid.rs:178-196), then refresh_moments_nodes (solver.rs:1188-1206) of the
 // restricted node.
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctLevel Lf, OctLevel Lc, OctCoincTab t,
-                                                       OctXi xi) {
-  const int j = node_id(t.n);
+__global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi xi) {
+  int lb;
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& Lf = grp.Lf[item];
+  const OctLevel& Lc = grp.Lc[item];
+  const OctCoincTab& t = grp.ct[item];
+  const int j = local_id(lb, t.n);
   if (j < 0) return;
   const long long Pc = Lc.plane, Pf = Lf.plane;
   const double* fm = Lf.m;
@@ -325,41 +365,83 @@ hipError_t launch_oct_walls(const OctLevel& L, const OctStream& s, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_oct_collide(const OctLevel& L, const OctCollide& p, int geq, hipStream_t st) {
+namespace {
+
+inline int nblk(long long n) { return (int)((n + kOctBlock - 1) / kOctBlock); }
+
+}  // namespace
+
+hipError_t launch_oct_collide(const OctLevel* L, int count, const OctCollide& p, int geq,
+                              hipStream_t st) {
+  OctGroup g{};
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    g.L[i] = L[i];
+    if (i == 0) g.nblk0 = nblk(L[i].plane);
+    total += nblk(L[i].plane);
+  }
+  if (count == 1) g.nblk0 = total;
   if (geq == 1)
-    hipLaunchKernelGGL(k_oct_collide<1>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, p);
+    hipLaunchKernelGGL(k_oct_collide<1>, dim3(total), dim3(kOctBlock), 0, st, g, p);
   else
-    hipLaunchKernelGGL(k_oct_collide<2>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, p);
+    hipLaunchKernelGGL(k_oct_collide<2>, dim3(total), dim3(kOctBlock), 0, st, g, p);
   return hipGetLastError();
 }
 
-hipError_t launch_oct_stream(const OctLevel& L, const OctStream& s, int geq, hipStream_t st) {
+hipError_t launch_oct_stream(const OctLevel* L, const OctStream* s, int count, int geq,
+                             hipStream_t st) {
+  OctGroup g{};
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    g.L[i] = L[i];
+    g.S[i] = s[i];
+    if (i == 0) g.nblk0 = nblk(L[i].plane);
+    total += nblk(L[i].plane);
+  }
   if (geq == 1)
-    hipLaunchKernelGGL(k_oct_stream<1>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, s);
+    hipLaunchKernelGGL(k_oct_stream<1>, dim3(total), dim3(kOctBlock), 0, st, g);
   else
-    hipLaunchKernelGGL(k_oct_stream<2>, grid_for(L.plane), dim3(kOctBlock), 0, st, L, s);
+    hipLaunchKernelGGL(k_oct_stream<2>, dim3(total), dim3(kOctBlock), 0, st, g);
   return hipGetLastError();
 }
 
-hipError_t launch_oct_coarse_to_fine(const OctLevel& coarse, const OctLevel& fine,
-                                     const OctGhostTab& t, const OctXi& xi, int geq,
+hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
+                                     const OctGhostTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st) {
-  if (t.n == 0) return hipSuccess;
+  OctXferGroup g{};
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    g.Lc[i] = coarse[i];
+    g.Lf[i] = fine[i];
+    g.gt[i] = t[i];
+    if (i == 0) g.nblk0 = nblk(t[i].n);
+    total += nblk(t[i].n);
+  }
+  if (total == 0) return hipSuccess;
   if (geq == 1)
-    hipLaunchKernelGGL(k_oct_c2f<1>, grid_for(t.n), dim3(kOctBlock), 0, st, coarse, fine, t, xi);
+    hipLaunchKernelGGL(k_oct_c2f<1>, dim3(total), dim3(kOctBlock), 0, st, g, xi);
   else
-    hipLaunchKernelGGL(k_oct_c2f<2>, grid_for(t.n), dim3(kOctBlock), 0, st, coarse, fine, t, xi);
+    hipLaunchKernelGGL(k_oct_c2f<2>, dim3(total), dim3(kOctBlock), 0, st, g, xi);
   return hipGetLastError();
 }
 
-hipError_t launch_oct_fine_to_coarse(const OctLevel& fine, const OctLevel& coarse,
-                                     const OctCoincTab& t, const OctXi& xi, int geq,
+hipError_t launch_oct_fine_to_coarse(const OctLevel* fine, const OctLevel* coarse,
+                                     const OctCoincTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st) {
-  if (t.n == 0) return hipSuccess;
+  OctXferGroup g{};
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    g.Lf[i] = fine[i];
+    g.Lc[i] = coarse[i];
+    g.ct[i] = t[i];
+    if (i == 0) g.nblk0 = nblk(t[i].n);
+    total += nblk(t[i].n);
+  }
+  if (total == 0) return hipSuccess;
   if (geq == 1)
-    hipLaunchKernelGGL(k_oct_f2c<1>, grid_for(t.n), dim3(kOctBlock), 0, st, fine, coarse, t, xi);
+    hipLaunchKernelGGL(k_oct_f2c<1>, dim3(total), dim3(kOctBlock), 0, st, g, xi);
   else
-    hipLaunchKernelGGL(k_oct_f2c<2>, grid_for(t.n), dim3(kOctBlock), 0, st, fine, coarse, t, xi);
+    hipLaunchKernelGGL(k_oct_f2c<2>, dim3(total), dim3(kOctBlock), 0, st, g, xi);
   return hipGetLastError();
 }
 

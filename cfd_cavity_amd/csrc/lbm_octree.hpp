@@ -55,15 +55,20 @@ struct OctXi {
   double even, odd, thermal;
 };
 
+// Launches over `count` (1 or 2) independent levels or interfaces at once:
+// the hot and cold blocks (or strips) of a level take one launch, their
+// workgroups split between the two.
 hipError_t launch_oct_init_eq(const OctLevel& L, int geq, hipStream_t st);
 hipError_t launch_oct_walls(const OctLevel& L, const OctStream& s, hipStream_t st);
-hipError_t launch_oct_collide(const OctLevel& L, const OctCollide& p, int geq, hipStream_t st);
-hipError_t launch_oct_stream(const OctLevel& L, const OctStream& s, int geq, hipStream_t st);
-hipError_t launch_oct_coarse_to_fine(const OctLevel& coarse, const OctLevel& fine,
-                                     const OctGhostTab& t, const OctXi& xi, int geq,
+hipError_t launch_oct_collide(const OctLevel* L, int count, const OctCollide& p, int geq,
+                              hipStream_t st);
+hipError_t launch_oct_stream(const OctLevel* L, const OctStream* s, int count, int geq,
+                             hipStream_t st);
+hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
+                                     const OctGhostTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st);
-hipError_t launch_oct_fine_to_coarse(const OctLevel& fine, const OctLevel& coarse,
-                                     const OctCoincTab& t, const OctXi& xi, int geq,
+hipError_t launch_oct_fine_to_coarse(const OctLevel* fine, const OctLevel* coarse,
+                                     const OctCoincTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st);
 
 }  // namespace cavlbm

@@ -187,3 +187,23 @@ def test_unsupported_combinations():
     with pytest.raises(CavlbmError) as e:
         g.phase("collide", g.time_step)
     assert e.value.code == E_UNSUPPORTED
+
+
+def test_graph_replayed_dynamic_steps_equal_single_steps():
+    """After the spin-up, runs of dynamic steps replay captured graphs (64
+    steps each); they must equal the same steps issued one by one."""
+    runs = []
+    for single in (False, True):
+        s = CavitySolver(*args_of(dict(SMALL, nx=32, ny=128, ra=1e6)))
+        s.run(250)
+        s.set_dynamic_sgs(True, 0.3, -0.1, 3.0)
+        if single:
+            for _ in range(130):
+                s.step()
+        else:
+            s.run(130)
+        runs.append(s)
+    a, b = runs
+    for w in ("f", "g"):
+        assert bits_equal(a.get_pops(w), b.get_pops(w))
+    assert bits_equal(a.nu_t_pre, b.nu_t_pre) and a.dynamic_stats() == b.dynamic_stats()
