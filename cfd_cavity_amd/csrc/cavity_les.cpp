@@ -1,17 +1,20 @@
-// cavity_les.cpp — the uniform-mode driver of the reference (bbrangeo/cfd_cavity
-// src/main.rs:42-443 `run_uniform`, flags of src/cli.rs:19-384) on the GPU
-// solver of libcavlbm, through the C++ mirror in include/cavlbm.hpp.
+// cavity_les.cpp — the drivers of the reference (bbrangeo/cfd_cavity
+// src/main.rs:42-443 `run_uniform`, :445-738 `run_octree`, flags of
+// src/cli.rs:19-384) on the GPU solvers of libcavlbm, through the C++ mirror
+// in include/cavlbm.hpp.
 //
 // Same flags (--key val or --key=val), defaults, validation messages' intent,
 // progress lines, abort guards and exit codes:
 //   0 ok / help, 1 CLI error, 2 abort (max_u > 0.10 or non-finite),
 //   3 no averaging samples, 4 output directory, 5 write failure,
 //   6 snapshot read / restart failure.
-// Outputs in --out: progress.log, snapshot.bin (CAVSNAP1), summary.json.  The
-// CSV / VTK field exporters of post.rs are not reproduced.
+// Outputs in --out: progress.log, snapshot.bin (CAVSNAP1; OCTSNAP1 with
+// --mode octree), summary.json (uniform).  The CSV / VTK field exporters of
+// post.rs are not reproduced; the octree's dynamic SGS is not on the GPU path.
 // Extra flags: --storage fp64|fp32|fp16_offset, --geq-order 1|2, --device N.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -39,6 +42,9 @@ struct Args {
   size_t snapshot_every = 0;
   bool dynamic_smag = false;
   double smag_alpha = 0.5, nu_sm_min = -0.1, eta_test = 2.0;
+  bool octree = false;  // --mode octree
+  size_t nx0 = 64, ny0 = 256, bl_width = 8, levels = 2;
+  bool odd_exact = false;
   int storage = CAVLBM_STORAGE_FP64, geq_order = 2, device = 0;
 };
 
@@ -46,11 +52,12 @@ constexpr double kMaxUAbort = 0.10;  // src/main.rs:46
 
 void usage() {
   std::printf(
-      "Usage: cavity_les [OPTIONS]   (uniform mode of bbrangeo/cfd_cavity on an MI355X)\n"
+      "Usage: cavity_les [OPTIONS]   (uniform and octree modes of bbrangeo/cfd_cavity, MI355X)\n"
       "  --nx --ny --ra --pr --cs --pr_t --steps --sample-start --sample-every --out\n"
       "  --utarget --seed --progress-every --jobs --alpha-mult --nu-start-mult --restart\n"
       "  --nu --snapshot-every --dynamic-smag --smag-alpha --nu-sm-min --eta-test\n"
-      "  --storage fp64|fp32|fp16_offset --geq-order 1|2 --device N\n");
+      "  --storage fp64|fp32|fp16_offset --geq-order 1|2 --device N\n"
+      "  --mode uniform|octree; octree: --nx0 --ny0 --bl-width --levels 2|3 --odd-exact\n");
 }
 
 bool parse_size(const std::string& v, size_t& out) {
@@ -91,15 +98,25 @@ std::string parse(int argc, char** argv, Args& a, bool& help) {
         return "--dynamic-smag : booléen attendu, lu '" + val + "'";
       continue;
     }
+    if (key == "odd-exact") {  // cli.rs:213-223
+      a.odd_exact = !inl || val == "true" || val == "1";
+      if (inl && val != "true" && val != "1" && val != "false" && val != "0")
+        return "--odd-exact : booléen attendu, lu '" + val + "'";
+      continue;
+    }
     if (!inl) {
       if (i + 1 >= argc) return "valeur manquante pour --" + key;
       val = argv[++i];
     }
     bool ok = true;
     if (key == "mode") {
-      if (val == "octree") return "--mode octree : non disponible sur ce solveur GPU (uniforme seul)";
-      ok = val == "uniform";
-    } else if (key == "nx") ok = parse_size(val, a.nx);
+      ok = val == "uniform" || val == "octree";
+      a.octree = val == "octree";
+    } else if (key == "nx0") ok = parse_size(val, a.nx0);
+    else if (key == "ny0") ok = parse_size(val, a.ny0);
+    else if (key == "bl-width") ok = parse_size(val, a.bl_width);
+    else if (key == "levels") ok = parse_size(val, a.levels);
+    else if (key == "nx") ok = parse_size(val, a.nx);
     else if (key == "ny") ok = parse_size(val, a.ny);
     else if (key == "ra") ok = parse_f64(val, a.ra);
     else if (key == "pr") ok = parse_f64(val, a.pr);
@@ -159,6 +176,19 @@ std::string parse(int argc, char** argv, Args& a, bool& help) {
     return "smag-alpha doit être ∈ [0, 1] (0 = Smagorinsky pur, 1 = similarité pure)";
   if (!(a.eta_test >= 1.0) || !std::isfinite(a.eta_test))
     return "eta-test doit être ≥ 1 (ratio filtre test/grille)";
+  if (a.octree) {  // cli.rs:362-381
+    if (a.nx0 < 16 || a.ny0 < 16) return "nx0 et ny0 doivent être ≥ 16";
+    if (a.ny0 <= a.nx0) return "cavité haute : ny0 doit être > nx0 (Az ≈ 4)";
+    if (!(a.bl_width >= 2 && 2 * a.bl_width < a.nx0))
+      return "bl-width doit être ≥ 2 et 2·bl-width < nx0 (noyau grossier requis)";
+    if (!(a.levels == 2 || a.levels == 3))
+      return "levels doit valoir 2 (config V2) ou 3 (escalier tanh V6)";
+    if (a.levels == 3 && a.bl_width != 4)
+      std::fprintf(stderr,
+                   "Note : escalier V6 conçu pour bl-width = 4 cellules grossières (frontières "
+                   "tanh §0.1) ; lu %zu\n",
+                   a.bl_width);
+  }
   if (a.progress_every == 0) a.progress_every = 1;
   return "";
 }
@@ -249,6 +279,226 @@ bool make_dirs(const std::string& path) {
 
 }  // namespace
 
+std::string fixed(double v, int prec) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*f", prec, v);
+  return buf;
+}
+
+// run_octree (main.rs:445-738): progress lines with the interface noise and
+// the Nusselt estimators, abort guards, sampled Nusselt means, OCTSNAP1
+// exports and restart, and the V2 verdict against the uniform references.
+int run_octree(const Args& a) {
+  std::printf("=== Cavité DHC 2D octree (Étape B, porte V2) ===\n");
+  std::printf("Grossier : %zu×%zu | BL fine : %zu cellules grossières ×2 parois (équiv. uniforme "
+              "%zu×%zu) | Ra = %s | Pr = %s | jobs = %zu | ξ_odd = %s\n",
+              a.nx0, a.ny0, a.bl_width, 2 * a.bl_width, a.ny0, rust_exp(a.ra, 3).c_str(),
+              rust_display(a.pr).c_str(), a.jobs, a.odd_exact ? "exact (τ⁻)" : "1/2 (approx.)");
+  if (a.nu > 0.0)
+    std::printf("Étape H : nu forcé = %s (bypass calage utarget = %s)\n",
+                rust_display(a.nu).c_str(), rust_display(a.utarget).c_str());
+  else if (a.utarget != 0.02)
+    std::printf("Étape H : calage ν via utarget = %s\n", rust_display(a.utarget).c_str());
+  if (a.nu_start_mult > 1.0) {
+    std::printf("Étape H : rampe de viscosité exponentielle nu-start-mult = %s (tau_ramp = 1000 "
+                "pas, terminée ~5000 pas) + rampe beta linéaire sur %zu pas (spin-up)\n",
+                rust_display(a.nu_start_mult).c_str(), std::max<size_t>(a.ny0 / 10, 2000));
+    if (a.sample_start < 5000)
+      std::fprintf(stderr,
+                   "WARNING : rampe de viscosité terminée ~5000 pas mais sample-start = %zu : "
+                   "augmentez --sample-start (> 5000) pour ne pas moyenner pendant la rampe\n",
+                   a.sample_start);
+  }
+  try {
+    cavlbm::OctreeCavitySolver sim(a.nx0, a.ny0, a.bl_width, a.ra, a.pr, a.utarget, a.seed,
+                                   a.jobs, a.odd_exact, (int)a.levels, a.nu_start_mult, a.nu,
+                                   a.geq_order, a.device);
+    if (a.dynamic_smag) {
+      std::fprintf(stderr, "Erreur : --dynamic-smag en mode octree n'est pas sur le chemin GPU\n");
+      return 1;
+    }
+    const int ng = sim.n_grids();
+    if (!a.restart.empty()) {  // main.rs:526-546
+      uint32_t lv = 0, sng = 0;
+      uint64_t step = 0, dims[10];
+      if (cavlbm_octsnap_read(a.restart.c_str(), &lv, &step, &sng, dims, nullptr, nullptr, nullptr,
+                              nullptr, 0)) {
+        std::fprintf(stderr, "Erreur de lecture du snapshot %s : %s\n", a.restart.c_str(),
+                     cavlbm_snapshot_last_error());
+        return 6;
+      }
+      uint64_t total = 0;
+      for (uint32_t g = 0; g < sng; ++g) total += dims[2 * g] * dims[2 * g + 1];
+      std::vector<double> f[4];
+      for (auto& v : f) v.resize(total);
+      if (cavlbm_octsnap_read(a.restart.c_str(), &lv, &step, &sng, dims, f[0].data(), f[1].data(),
+                              f[2].data(), f[3].data(), total)) {
+        std::fprintf(stderr, "Erreur de lecture du snapshot %s : %s\n", a.restart.c_str(),
+                     cavlbm_snapshot_last_error());
+        return 6;
+      }
+      std::printf("Restart depuis %s : snapshot levels=%u (%u grilles), pas %llu — injection des "
+                  "champs du solveur octree\n",
+                  a.restart.c_str(), lv, sng, (unsigned long long)step);
+      // set_from_snapshot's layout checks (solver.rs:1095-1133)
+      if ((int)lv != (int)a.levels || (int)sng != ng) {
+        std::fprintf(stderr, "Erreur de restart : restart : snapshot levels=%u (%u grilles) "
+                     "incompatible avec le solveur levels=%zu (%d grilles attendues)\n",
+                     lv, sng, a.levels, ng);
+        return 6;
+      }
+      for (int g = 0; g < ng; ++g)
+        if (dims[2 * g] != sim.dims()[g].first || dims[2 * g + 1] != sim.dims()[g].second) {
+          std::fprintf(stderr, "Erreur de restart : restart : grille %d du snapshot %llux%llu ≠ "
+                       "solveur %zux%zu (même layout requis — pas d'interpolation inter-grille "
+                       "en Étape G)\n",
+                       g, (unsigned long long)dims[2 * g], (unsigned long long)dims[2 * g + 1],
+                       sim.dims()[g].first, sim.dims()[g].second);
+          return 6;
+        }
+      sim.set_from_snapshot(f[0], f[1], f[2], f[3]);
+    }
+    const cavlbm_octree_derived& d = sim.derived();
+    const size_t n0 = a.nx0 * a.ny0;
+    std::printf("Unités réseau : nu = %.6f | alpha = %.6f | tau+ = %.4f/%.4f%s | beta_eff = "
+                "%s/%s%s | %zu niveaux | %llu nœuds (grossier %zu + fins %llu)\n",
+                d.nu_lu, d.alpha_lu, d.tau[0][0], d.tau[1][0],
+                a.levels == 3 ? ("/" + fixed(d.tau[2][0], 4)).c_str() : "",
+                rust_exp(d.beta_eff_coarse, 4).c_str(), rust_exp(d.beta_eff_fine, 4).c_str(),
+                a.levels == 3 ? ("/" + rust_exp(0.25 * d.beta_eff_coarse, 4)).c_str() : "",
+                a.levels, (unsigned long long)d.n_nodes_total, n0,
+                (unsigned long long)(d.n_nodes_total - n0));
+    const double lz = (double)(a.ny0 - 1);
+    std::printf("Ra vérifié depuis unités réseau (grossier) : %s (cible %s)\n",
+                rust_exp(d.beta_eff_coarse * (lz * lz * lz) / (d.nu_lu * d.alpha_lu), 4).c_str(),
+                rust_exp(a.ra, 4).c_str());
+    if (!make_dirs(a.out)) {
+      std::fprintf(stderr, "Impossible de créer %s\n", a.out.c_str());
+      return 4;
+    }
+    std::vector<std::string> log_lines;
+    double nuc_sum = 0.0, nuf_sum = 0.0, nucl0_sum = 0.0, nufine_sum = 0.0;
+    size_t nuc_n = 0;
+    double noise_max = 0.0, noise_iface_max = 0.0;
+    const std::string snap_path = a.out + "/snapshot.bin";
+    auto write_snap = [&](uint64_t step) {
+      std::vector<double> r, u, v, t;
+      sim.level_fields(r, u, v, t);
+      std::vector<uint64_t> dims;
+      for (auto& p : sim.dims()) {
+        dims.push_back(p.first);
+        dims.push_back(p.second);
+      }
+      return cavlbm_octsnap_write(snap_path.c_str(), (uint32_t)a.levels, step, (uint32_t)ng,
+                                  dims.data(), r.data(), u.data(), v.data(), t.data(), 1);
+    };
+    auto write_log = [&]() {
+      std::string text;
+      for (size_t i = 0; i < log_lines.size(); ++i) text += log_lines[i] + "\n";
+      write_text(a.out + "/progress.log", text);
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t done = 0;
+    auto is_progress = [&](size_t s) { return s % a.progress_every == 0 || s + 1 == a.steps; };
+    auto is_sample = [&](size_t s) {
+      return s >= a.sample_start && (s - a.sample_start) % a.sample_every == 0;
+    };
+    auto is_snap = [&](size_t s) { return a.snapshot_every > 0 && (s + 1) % a.snapshot_every == 0; };
+    for (size_t step = 0; step < a.steps; ++step) {
+      if (!(is_progress(step) || is_sample(step) || is_snap(step))) continue;
+      sim.run(step + 1 - done);
+      done = step + 1;
+      cavlbm_octree_diag dg{};
+      if (is_sample(step) || is_progress(step)) dg = sim.diagnostics();
+      if (is_sample(step)) {
+        nuc_sum += dg.nusselt_hot;
+        nuf_sum += dg.nusselt_cold;
+        nucl0_sum += dg.nusselt_hot_link0;
+        nufine_sum += dg.nusselt_hot_fine;
+        ++nuc_n;
+      }
+      if (is_progress(step)) {
+        noise_max = std::fmax(noise_max, dg.max_rho_dev);
+        noise_iface_max = std::fmax(noise_iface_max, dg.max_rho_dev_ifaces);
+        char line[512];
+        std::snprintf(line, sizeof line,
+                      "step %7zu | max_rho_dev = %s | max_rho_dev_ifaces = %s | masse = %.6f | "
+                      "Nuc = %.4f | Nuc_link0 = %.4f | Nuf = %.4f | Nu_fin = %.4f | max_u = %.5f",
+                      step, rust_exp(dg.max_rho_dev, 3).c_str(),
+                      rust_exp(dg.max_rho_dev_ifaces, 3).c_str(), dg.masse, dg.nusselt_hot,
+                      dg.nusselt_hot_link0, dg.nusselt_cold, dg.nusselt_hot_fine, dg.max_u_all);
+        std::printf("%s\n", line);
+        log_lines.push_back(line);
+        if (dg.guard != 0) {  // check_guards (solver.rs:1018-1041)
+          char m[256];
+          if (dg.guard == 1)
+            std::snprintf(m, sizeof m,
+                          "AVORT : max_u = %.5f dépasse le garde-fou 0.10 (Ma_local = max_u·√3 "
+                          "≈ %.3f)",
+                          dg.max_u_all, dg.max_u_all * std::sqrt(3.0));
+          else
+            std::snprintf(m, sizeof m, "DIVERGENCE détectée (valeurs non finies).");
+          std::fprintf(stderr, "%s\n", m);
+          log_lines.push_back(m);
+          write_log();
+          return 2;
+        }
+      }
+      if (is_snap(step)) {
+        if (write_snap(step)) {
+          std::fprintf(stderr, "Erreur d'export du snapshot au pas %zu : %s\n", step,
+                       cavlbm_snapshot_last_error());
+          return 5;
+        }
+        std::printf("snapshot exporté au pas %zu\n", step);
+      }
+    }
+    if (done < a.steps) sim.run(a.steps - done);
+    if (a.snapshot_every > 0) {
+      if (write_snap(a.steps)) {
+        std::fprintf(stderr, "Erreur d'export du snapshot final : %s\n",
+                     cavlbm_snapshot_last_error());
+        return 5;
+      }
+      std::printf("snapshot final exporté au pas %zu\n", a.steps - 1);
+    }
+    const double el =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Simulation terminée en %.1f s (%.1fk pas/s)\n", el, (double)a.steps / el / 1000.0);
+    if (nuc_n == 0) {
+      std::fprintf(stderr, "Aucun échantillon moyenné : sample_start ≥ steps ?\n");
+      return 3;
+    }
+    const double nuc_mean = nuc_sum / (double)nuc_n, nuf_mean = nuf_sum / (double)nuc_n;
+    const double nucl0_mean = nucl0_sum / (double)nuc_n, nufine_mean = nufine_sum / (double)nuc_n;
+    const double nuc_ref = 2.3585, nucl0_ref = 1.3660;  // main.rs:697-700
+    const double ecart = std::fabs((nuc_mean - nuc_ref) / nuc_ref) * 100.0;
+    char summ[1024];
+    std::snprintf(summ, sizeof summ,
+                  "--- Bilan (moyenné sur %zu échantillons, fenêtre %zu→%zu) ---\n"
+                  "Nuc (2ᵉ liaison) = %.4f | écart like-for-like vs %s = %.1f %%\n"
+                  "Nuc_link0 (1ʳᵉ liaison) = %.4f | Nuf = %.4f | Nu_fin (bloc chaud) = %.4f\n"
+                  "Références uniforme Ra=1e6 : 2ᵉ liaison = %s | 1ʳᵉ liaison = %s\n"
+                  "Bruit acoustique (max sur la run) : max|ρ−ρ̄| = %s | interfaces = %s\n"
+                  "PORTE V2 (écart Nuc ≤ 5 %%) : %s",
+                  nuc_n, a.sample_start, a.steps, nuc_mean, rust_display(nuc_ref).c_str(), ecart,
+                  nucl0_mean, nuf_mean, nufine_mean, rust_display(nuc_ref).c_str(),
+                  rust_display(nucl0_ref).c_str(), rust_exp(noise_max, 3).c_str(),
+                  rust_exp(noise_iface_max, 3).c_str(), ecart <= 5.0 ? "PASS" : "FAIL");
+    std::printf("%s\n", summ);
+    log_lines.push_back(summ);
+    write_log();
+    std::printf("Artefacts écrits dans %s\n", a.out.c_str());
+  } catch (const std::invalid_argument& e) {
+    std::fprintf(stderr, "Erreur : %s\n", e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "Erreur : %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   Args a;
   bool help = false;
@@ -262,6 +512,7 @@ int main(int argc, char** argv) {
     usage();
     return 1;
   }
+  if (a.octree) return run_octree(a);  // main.rs:34-37
   std::printf("=== Cavité différentiellement chauffée 2D (LBM D2Q9 + Smagorinsky, MI355X) ===\n");
   std::printf("Grille : %zux%zu (Az = %.2f) | Ra = %.3e | Pr = %g | Cs = %g | Pr_t = %g | jobs = %zu\n",
               a.nx, a.ny, (double)(a.ny - 1) / (double)(a.nx - 1), a.ra, a.pr, a.cs, a.pr_t, a.jobs);

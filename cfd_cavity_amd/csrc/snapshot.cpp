@@ -148,6 +148,132 @@ int cavlbm_interpolate_field(const double* src, uint64_t nx0, uint64_t ny0, doub
   return 0;
 }
 
+// ---- OCTSNAP1: multi-level snapshots of the octree solver
+// (src/octree/snapshot.rs:1-170).  Header: magic "OCTSNAP1", version u32 = 1,
+// levels u32, step u64, n_grids u32, then nx, ny (u64) of every grid; then
+// for every grid in the same order its rho, ux, uy, T (f64), all
+// little-endian.  The field arrays here are the grids' fields concatenated in
+// grid order.
+
+namespace {
+
+const char kOctMagic[8] = {'O', 'C', 'T', 'S', 'N', 'A', 'P', '1'};
+
+void put_u32(std::vector<unsigned char>& b, uint32_t v) {
+  for (int i = 0; i < 4; ++i) b.push_back((unsigned char)(v >> (8 * i)));
+}
+
+uint32_t get_u32(const unsigned char* p) {
+  return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+int write_file(const std::string& path, const std::vector<unsigned char>& b) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return io_fail("cannot create " + path);
+  const size_t w = std::fwrite(b.data(), 1, b.size(), f);
+  const int fl = std::fflush(f);
+  const int c = std::fclose(f);
+  if (w != b.size() || fl != 0 || c != 0) return io_fail("write failed: " + path);
+  return 0;
+}
+
+}  // namespace
+
+// encode (snapshot.rs:57-79) + write_snapshot_atomic (snapshot.rs:140-161)
+int cavlbm_octsnap_write(const char* path, uint32_t levels, uint64_t step, uint32_t n_grids,
+                         const uint64_t* dims, const double* rho, const double* ux,
+                         const double* uy, const double* t, int32_t atomic) {
+  if (!path || !dims || !rho || !ux || !uy || !t) return io_fail("null argument");
+  std::vector<unsigned char> b;
+  b.insert(b.end(), kOctMagic, kOctMagic + 8);
+  put_u32(b, 1);
+  put_u32(b, levels);
+  put_u64(b, step);
+  put_u32(b, n_grids);
+  for (uint32_t g = 0; g < n_grids; ++g) {
+    put_u64(b, dims[2 * g]);
+    put_u64(b, dims[2 * g + 1]);
+  }
+  uint64_t off = 0;
+  for (uint32_t g = 0; g < n_grids; ++g) {
+    const uint64_t n = dims[2 * g] * dims[2 * g + 1];
+    for (const double* a : {rho, ux, uy, t})
+      for (uint64_t k = 0; k < n; ++k) put_f64(b, a[off + k]);
+    off += n;
+  }
+  if (!atomic) return write_file(path, b);
+  const std::string tmp = std::string(path) + ".tmp";
+  int rc = write_file(tmp, b);
+  if (rc) {
+    std::remove(tmp.c_str());
+    return rc;
+  }
+  if (std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return io_fail(std::string("rename failed: ") + path);
+  }
+  return 0;
+}
+
+// read_snapshot + decode (snapshot.rs:83-137, :164-170): the exact length is
+// checked.  With null field pointers only the header (levels, step,
+// n_grids, dims: room for 2*5 values) is returned; otherwise each field
+// buffer must hold `capacity` >= the sum of nx*ny values.
+int cavlbm_octsnap_read(const char* path, uint32_t* levels, uint64_t* step, uint32_t* n_grids,
+                        uint64_t* dims, double* rho, double* ux, double* uy, double* t,
+                        uint64_t capacity) {
+  if (!path || !levels || !step || !n_grids || !dims) return io_fail("null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return io_fail(std::string("cannot open ") + path);
+  std::vector<unsigned char> data;
+  unsigned char buf[1 << 16];
+  size_t r;
+  while ((r = std::fread(buf, 1, sizeof(buf), f)) > 0) data.insert(data.end(), buf, buf + r);
+  std::fclose(f);
+  if (data.size() < 28 || std::memcmp(data.data(), kOctMagic, 8) != 0)
+    return io_fail("snapshot octree : magic invalide ou fichier trop court");
+  const uint32_t version = get_u32(data.data() + 8);
+  if (version != 1)
+    return io_fail("snapshot octree : version " + std::to_string(version) +
+                   " non supportée (attendu 1)");
+  const uint32_t lv = get_u32(data.data() + 12);
+  const uint64_t st = get_u64(data.data() + 16);
+  const uint32_t ng = get_u32(data.data() + 24);
+  if (!(lv == 2 || lv == 3) || ng != (lv == 2 ? 3u : 5u))
+    return io_fail("snapshot octree : incohérence levels=" + std::to_string(lv) +
+                   " / n_grids=" + std::to_string(ng) + " (attendu 3/5)");
+  const size_t dims_end = 28 + (size_t)ng * 16;
+  if (data.size() < dims_end) return io_fail("snapshot octree : header dims tronqué");
+  uint64_t expect = dims_end, total = 0;
+  uint64_t d[10];
+  for (uint32_t g = 0; g < ng; ++g) {
+    d[2 * g] = get_u64(data.data() + 28 + 16 * g);
+    d[2 * g + 1] = get_u64(data.data() + 36 + 16 * g);
+    const uint64_t n = d[2 * g] * d[2 * g + 1];
+    expect += 4 * n * 8;
+    total += n;
+  }
+  if (data.size() != expect)
+    return io_fail("snapshot octree : attendu " + std::to_string(expect) + " octets, lu " +
+                   std::to_string(data.size()));
+  *levels = lv;
+  *step = st;
+  *n_grids = ng;
+  std::memcpy(dims, d, sizeof(uint64_t) * 2 * ng);
+  if (!rho && !ux && !uy && !t) return 0;
+  if (!rho || !ux || !uy || !t || capacity < total)
+    return io_fail("snapshot octree : output buffers too small");
+  const unsigned char* p = data.data() + dims_end;
+  uint64_t off = 0;
+  for (uint32_t g = 0; g < ng; ++g) {
+    const uint64_t n = d[2 * g] * d[2 * g + 1];
+    for (double* a : {rho, ux, uy, t})
+      for (uint64_t k = 0; k < n; ++k, p += 8) a[off + k] = get_f64(p);
+    off += n;
+  }
+  return 0;
+}
+
 const char* cavlbm_snapshot_last_error(void) { return g_io_error.c_str(); }
 
 }  // extern "C"

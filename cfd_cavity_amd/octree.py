@@ -148,6 +148,33 @@ class OctreeCavitySolver:
         self._check(self._L.cavlbm_octree_set_from_snapshot(self._h, *[_dp(a) for a in arrs],
                                                             arrs[0].size))
 
+    def write_snapshot(self, path, step, atomic=True):
+        """OCTSNAP1 export of the level fields (solver.rs:1068-1075,
+        snapshot.rs:140-161)."""
+        from .snapshot import write_octree_snapshot
+
+        write_octree_snapshot(path, self.levels, step, self.level_fields(), atomic)
+
+    def restart(self, path):
+        """--restart of run_octree (main.rs:526-546): read an OCTSNAP1 file and
+        inject its fields (set_from_snapshot, which requires the same layout,
+        solver.rs:1095-1133).  Returns the snapshot's step."""
+        from .snapshot import read_octree_snapshot
+
+        levels, step, grids = read_octree_snapshot(path)
+        n_expected = 3 if self.levels == 2 else 5
+        if levels != self.levels or len(grids) != n_expected:
+            raise ValueError(
+                f"restart : snapshot levels={levels} ({len(grids)} grilles) incompatible avec "
+                f"le solveur levels={self.levels} ({n_expected} grilles attendues)")
+        for gi, (g, d) in enumerate(zip(grids, self.dims)):
+            if tuple(g[:2]) != tuple(d):
+                raise ValueError(
+                    f"restart : grille {gi} du snapshot {g[0]}x{g[1]} ≠ solveur {d[0]}x{d[1]} "
+                    "(même layout requis — pas d'interpolation inter-grille en Étape G)")
+        self.set_from_snapshot(*(np.concatenate([g[2 + q] for g in grids]) for q in range(4)))
+        return step
+
     # --- diagnostics (cavity.rs:672-790, solver.rs:949-1041)
     def diagnostics(self):
         d = _lib.OctreeDiag()

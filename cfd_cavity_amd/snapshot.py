@@ -69,3 +69,39 @@ def save(solver, path):
     write_snapshot_atomic(path, solver.nx(), solver.ny(), solver.get_field("rho"),
                           solver.get_field("ux"), solver.get_field("uy"),
                           solver.get_field("temperature"))
+
+
+# ---------------------------------------------------------------------------
+# OCTSNAP1: multi-level snapshots of the octree solver (src/octree/snapshot.rs)
+
+def write_octree_snapshot(path, levels, step, grids, atomic=True):
+    """``grids``: list of (nx, ny, rho, ux, uy, T) in grid order
+    (snapshot.rs:57-79, atomic write snapshot.rs:140-161)."""
+    dims = (C.c_uint64 * (2 * len(grids)))(*[v for g in grids for v in g[:2]])
+    fields = [np.concatenate([np.asarray(g[2 + q], dtype=np.float64).ravel() for g in grids])
+              for q in range(4)]
+    _check(lib().cavlbm_octsnap_write(os.fsencode(path), levels, step, len(grids), dims,
+                                      *[_dp(x) for x in fields], 1 if atomic else 0))
+
+
+def read_octree_snapshot(path):
+    """(levels, step, grids) with grids = [(nx, ny, rho, ux, uy, T), ...]
+    (snapshot.rs:83-137, :164-170)."""
+    L = lib()
+    lv, ng = C.c_uint32(), C.c_uint32()
+    st = C.c_uint64()
+    dims = (C.c_uint64 * 10)()
+    p = os.fsencode(path)
+    _check(L.cavlbm_octsnap_read(p, C.byref(lv), C.byref(st), C.byref(ng), dims, None, None,
+                                 None, None, 0))
+    d = [(dims[2 * g], dims[2 * g + 1]) for g in range(ng.value)]
+    total = sum(nx * ny for nx, ny in d)
+    out = [np.empty(total) for _ in range(4)]
+    _check(L.cavlbm_octsnap_read(p, C.byref(lv), C.byref(st), C.byref(ng), dims,
+                                 *[_dp(x) for x in out], total))
+    grids, off = [], 0
+    for nx, ny in d:
+        n = nx * ny
+        grids.append((nx, ny) + tuple(a[off:off + n] for a in out))
+        off += n
+    return lv.value, st.value, grids

@@ -369,6 +369,23 @@ int cavlbm_octree_set_dynamic_sgs(cavlbm_octree *o, int32_t enable, double smag_
                                   double nu_sm_min, double eta_test);
 const char *cavlbm_octree_last_error(const cavlbm_octree *o);
 
+/* OCTSNAP1 multi-level snapshots (src/octree/snapshot.rs:1-170), host-side
+ * file I/O like the CAVSNAP1 functions (errors: CAVLBM_E_IO, message in
+ * cavlbm_snapshot_last_error()).  dims holds nx, ny of each grid (2*n_grids
+ * values); the field arrays are the grids' fields concatenated in grid
+ * order.  write: encode + (atomic != 0) temporary file and rename
+ * (snapshot.rs:57-79, :140-161). */
+int cavlbm_octsnap_write(const char *path, uint32_t levels, uint64_t step, uint32_t n_grids,
+                         const uint64_t *dims, const double *rho, const double *ux,
+                         const double *uy, const double *t, int32_t atomic);
+/* read_snapshot + decode (snapshot.rs:83-137, :164-170), exact length
+ * checked.  With null field pointers only the header is returned (dims needs
+ * room for 10 values); else each field buffer holds `capacity` >= the total
+ * node count. */
+int cavlbm_octsnap_read(const char *path, uint32_t *levels, uint64_t *step, uint32_t *n_grids,
+                        uint64_t *dims, double *rho, double *ux, double *uy, double *t,
+                        uint64_t capacity);
+
 /* Message of the last failure on this handle (or of cavlbm_create when s is
  * NULL).  Valid until the next call on the same handle / thread. */
 const char *cavlbm_last_error(const cavlbm_solver *s);

@@ -127,4 +127,92 @@ class CavitySolver {
   size_t nx_ = 0, ny_ = 0;
 };
 
+// Octree CavitySolver (src/octree/solver.rs:118-1206) over cavlbm_octree_*.
+class OctreeCavitySolver {
+ public:
+  // CavitySolver::new (solver.rs:340-353); nu_override < 0 means None.
+  OctreeCavitySolver(size_t nx0, size_t ny0, size_t bl_width, double ra, double pr,
+                     double utarget, uint64_t seed, size_t jobs, bool odd_exact, int levels,
+                     double nu_start_mult, double nu_override = -1.0, int geq_order = 2,
+                     int device = 0) {
+    cavlbm_octree_params p;
+    cavlbm_octree_params_default(&p);
+    p.nx0 = nx0;
+    p.ny0 = ny0;
+    p.bl_width = bl_width;
+    p.ra = ra;
+    p.pr = pr;
+    p.utarget = utarget;
+    p.seed = seed;
+    p.jobs = jobs;
+    p.odd_exact = odd_exact ? 1 : 0;
+    p.levels = levels;
+    p.nu_start_mult = nu_start_mult;
+    p.has_nu_override = nu_override >= 0.0 ? 1 : 0;
+    p.nu_override = nu_override >= 0.0 ? nu_override : 0.0;
+    p.geq_order = geq_order;
+    p.device = device;
+    const int rc = cavlbm_octree_create(&p, &o_);
+    if (rc == CAVLBM_E_PARAM) throw std::invalid_argument(cavlbm_octree_last_error(nullptr));
+    if (rc) throw std::runtime_error(cavlbm_octree_last_error(nullptr));
+    check(cavlbm_octree_derived_params(o_, &d_));
+    for (int g = 0; g < d_.n_grids; ++g) {
+      uint64_t nx = 0, ny = 0;
+      check(cavlbm_octree_grid_dims(o_, g, &nx, &ny));
+      dims_.push_back({(size_t)nx, (size_t)ny});
+    }
+  }
+  ~OctreeCavitySolver() { cavlbm_octree_destroy(o_); }
+  OctreeCavitySolver(const OctreeCavitySolver&) = delete;
+  OctreeCavitySolver& operator=(const OctreeCavitySolver&) = delete;
+
+  void step_cavity() { check(cavlbm_octree_step(o_)); }          // solver.rs:647
+  void run(uint64_t n) { check(cavlbm_octree_run(o_, n)); }
+  const cavlbm_octree_derived& derived() const { return d_; }
+  const std::vector<std::pair<size_t, size_t>>& dims() const { return dims_; }
+  int n_grids() const { return d_.n_grids; }
+  cavlbm_octree_diag diagnostics() {
+    cavlbm_octree_diag d;
+    check(cavlbm_octree_diagnostics(o_, &d));
+    return d;
+  }
+  std::vector<double> field(int grid, int which) {
+    std::vector<double> v(dims_[grid].first * dims_[grid].second);
+    check(cavlbm_octree_get_field(o_, grid, which, v.data(), v.size()));
+    return v;
+  }
+  // level_fields (solver.rs:1048-1066), concatenated in grid order
+  void level_fields(std::vector<double>& rho, std::vector<double>& ux, std::vector<double>& uy,
+                    std::vector<double>& t) {
+    rho.clear();
+    ux.clear();
+    uy.clear();
+    t.clear();
+    for (int g = 0; g < n_grids(); ++g) {
+      for (auto pr : {std::make_pair(&rho, CAVLBM_FIELD_RHO), std::make_pair(&ux, CAVLBM_FIELD_UX),
+                      std::make_pair(&uy, CAVLBM_FIELD_UY), std::make_pair(&t, CAVLBM_FIELD_T)}) {
+        const auto v = field(g, pr.second);
+        pr.first->insert(pr.first->end(), v.begin(), v.end());
+      }
+    }
+  }
+  void set_from_snapshot(const std::vector<double>& rho, const std::vector<double>& ux,
+                         const std::vector<double>& uy, const std::vector<double>& t) {
+    check(cavlbm_octree_set_from_snapshot(o_, rho.data(), ux.data(), uy.data(), t.data(),
+                                          rho.size()));
+  }
+  void set_dynamic_sgs(bool on, double a = 0.5, double m = -0.1, double e = 2.0) {
+    check(cavlbm_octree_set_dynamic_sgs(o_, on ? 1 : 0, a, m, e));
+  }
+  cavlbm_octree* raw() { return o_; }
+
+ private:
+  void check(int rc) const {
+    if (rc) throw std::runtime_error(cavlbm_octree_last_error(o_));
+  }
+  cavlbm_octree* o_ = nullptr;
+  cavlbm_octree_derived d_{};
+  std::vector<std::pair<size_t, size_t>> dims_;
+};
+
 }  // namespace cavlbm

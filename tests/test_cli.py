@@ -27,7 +27,12 @@ def run(*args, timeout=120):
     (["--nu", "0.01"], "0.002"),
     (["--nx"], "manquante"),
     (["--bogus", "1"], "inconnue"),
-    (["--mode", "octree"], "octree"),
+    (["--mode", "quadtree"], "mode"),
+    (["--mode", "octree", "--nx0", "10"], "nx0 et ny0"),
+    (["--mode", "octree", "--nx0", "64", "--ny0", "64"], "ny0 doit être > nx0"),
+    (["--mode", "octree", "--bl-width", "40"], "bl-width"),
+    (["--mode", "octree", "--levels", "4"], "levels doit valoir"),
+    (["--mode", "octree", "--odd-exact=maybe"], "booléen"),
     (["--smag-alpha", "1.5"], "smag-alpha"),
     (["--eta-test", "0.5"], "eta-test"),
     (["--dynamic-smag=maybe"], "booléen"),
@@ -100,3 +105,36 @@ def test_cli_dynamic_smag(tmp_path):
     assert dj["dynamic_smag"] is True and dj["eta_test"] == 3 and dj["smag_alpha"] == 0.5
     assert 0.0 <= dj["cs_dyn_max"] <= 0.04 and dj["cs_dyn_mean"] <= dj["cs_dyn_max"]
     assert "LES dynamique : cs_dyn_mean = " in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_octree_run_snapshot_and_restart(tmp_path):
+    """--mode octree (main.rs:445-738): progress lines, OCTSNAP1 exports, the
+    V2 summary, and a restart from the exported snapshot."""
+    from cfd_cavity_amd import OctreeCavitySolver
+    from cfd_cavity_amd.snapshot import read_octree_snapshot
+
+    out = tmp_path / "oct"
+    args = ["--mode", "octree", "--nx0", 32, "--ny0", 64, "--bl-width", 4, "--levels", 3,
+            "--sample-start", 100, "--sample-every", 50, "--progress-every", 100,
+            "--snapshot-every", 100]
+    r = run(*args, "--steps", 300, "--out", out)
+    assert r.returncode == 0, r.stderr
+    lines = (out / "progress.log").read_text().splitlines()
+    assert [ln.split("|")[0].strip() for ln in lines[:4]] == ["step       0", "step     100",
+                                                             "step     200", "step     299"]
+    assert "max_rho_dev_ifaces = " in lines[0] and "Nu_fin = " in lines[0]
+    assert "PORTE V2" in r.stdout and "--- Bilan (moyenné sur 4 échantillons" in r.stdout
+    levels, step, grids = read_octree_snapshot(out / "snapshot.bin")
+    assert (levels, step, len(grids)) == (3, 300, 5)
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    s.run(300)
+    for gi, g in enumerate(grids):
+        assert np.array_equal(g[5], s.get_field(gi, "temperature"))
+    r2 = run(*args, "--steps", 50, "--sample-start", 0, "--restart", out / "snapshot.bin",
+             "--out", tmp_path / "oct2")
+    assert r2.returncode == 0, r2.stderr
+    assert "Restart depuis" in r2.stdout and "pas 300" in r2.stdout
+    bad = run("--mode", "octree", "--nx0", 32, "--ny0", 64, "--bl-width", 6, "--steps", 5,
+              "--restart", out / "snapshot.bin", "--out", tmp_path / "oct3")
+    assert bad.returncode == 6 and "incompatible" in bad.stderr

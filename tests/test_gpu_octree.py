@@ -191,3 +191,44 @@ def test_errors():
         s.set_dynamic_sgs(True)
     assert e.value.code == E_UNSUPPORTED
     s.set_dynamic_sgs(False)
+
+
+# ---------------------------------------------------------------------------
+# OCTSNAP1 restart (src/octree/tests.rs:795-965)
+
+def test_snapshot_restart_restores_distributions(oracle, tmp_path):   # tests.rs:837-945
+    sim = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    sim.run(5)
+    path = tmp_path / "snap.bin"
+    sim.write_snapshot(path, 5)
+    restored = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    assert restored.restart(path) == 5
+    assert restored.time_step == 0
+    for gi, (nx, ny, rho, ux, uy, t) in enumerate(sim.level_fields()):
+        assert bits_equal(restored.get_field(gi, "rho"), rho)
+        assert bits_equal(restored.get_field(gi, "temperature"), t)
+        f = restored.get_pops(gi, "f")
+        k = (ny // 2) * nx + nx // 2  # an interior node: its velocity is the exported one
+        assert bits_equal(restored.get_field(gi, "ux")[k], ux[k])
+        assert bits_equal(f[k], np.array([oracle.feq(i, rho[k], ux[k], uy[k]) for i in range(9)]))
+    sim.step_cavity()
+    restored.step_cavity()
+    a, b = sim.diagnostics(), restored.diagnostics()
+    assert abs(a["masse"] - b["masse"]) < 5e-6
+    assert abs(a["max_rho_dev"] - b["max_rho_dev"]) < 5e-3
+    assert abs(a["nusselt_hot"] - b["nusselt_hot"]) < 5e-2
+    ra, rb = sim.get_field(0, "rho"), restored.get_field(0, "rho")
+    assert np.max(np.abs(ra - rb)) < 5e-4
+
+
+def test_snapshot_restart_layout_mismatch_is_error(tmp_path):        # tests.rs:947-965
+    sim = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    sim.run(2)
+    path = tmp_path / "snap.bin"
+    sim.write_snapshot(path, 2)
+    other = OctreeCavitySolver(32, 64, 6, 1.0e6, levels=3)
+    with pytest.raises(ValueError, match="layout"):
+        other.restart(path)
+    two = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=2)
+    with pytest.raises(ValueError, match="incompatible"):
+        two.restart(path)
