@@ -228,51 +228,83 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
   M[kOctT * P + k] = mo.t;
 }
 
+// The interface transfers, one workgroup lane per (entry, stencil node):
+// each lane forms its node's weighted moments and weighted non-equilibrium
+// parts, and the entry's first lane adds them in stencil order (the order of
+// the reference's accumulation loops) and injects the result.  Splitting the
+// stencil over lanes shortens each thread's dependent chain, which bounds
+// these small launches.
+constexpr int kXferVals = 22;  // w*rho, w*ux, w*uy, w*T, 9 f and 9 g parts
+
+template <int NS>
+struct XferShape {
+  static constexpr int kEntries = kOctBlock / NS;  // entries per workgroup
+};
+
+template <int GEQ, int NS>
+__device__ __forceinline__ void xfer_lane(const OctLevel& Ls, int ks, double w,
+                                          double (&c)[kXferVals]) {
+  const long long P = Ls.plane;
+  const double* m = Ls.m;
+  const double r = m[kOctRho * P + ks], u = m[kOctUx * P + ks], v = m[kOctUy * P + ks],
+               t = m[kOctT * P + ks];
+  c[0] = w * r;
+  c[1] = w * u;
+  c[2] = w * v;
+  c[3] = w * t;
+  EqTerms<double> e;
+  eq_terms<double>(u, v, e);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    c[4 + i] = w * (m[i * P + ks] - eq_quad<double>(i, r, e));
+    c[13 + i] = w * (m[(9 + i) * P + ks] - eq_g<double, GEQ>(i, t, e));
+  }
+}
+
 // coarse_to_fine (interface.rs:164-195): bilinear moments and non-equilibrium
 // of the coarse level, rescaled by xi, injected on the ghost nodes, whose
 // moments are then recomputed from the injected populations.
 template <int GEQ>
 __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi xi) {
+  constexpr int NS = 4, EPB = XferShape<NS>::kEntries;
+  __shared__ double sh[EPB * NS][kXferVals];
   int lb;
   const int item = group_item(grp.nblk0, lb);
   const OctLevel& Lc = grp.Lc[item];
   const OctLevel& Lf = grp.Lf[item];
   const OctGhostTab& t = grp.gt[item];
-  const int j = local_id(lb, t.n);
-  if (j < 0) return;
-  const long long Pc = Lc.plane, Pf = Lf.plane;
-  const double* cm = Lc.m;
-  double rho = 0.0, ux = 0.0, uy = 0.0, temp = 0.0;
-  double fneq[9], gneq[9];
+  const int e = (int)threadIdx.x / NS, q = (int)threadIdx.x % NS;
+  const int j = lb * EPB + e;
+  const bool valid = e < EPB && j < t.n;
+  if (valid) {
+    double c[kXferVals];
+    xfer_lane<GEQ, NS>(Lc, t.kc[4 * j + q], t.w[4 * j + q], c);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) fneq[i] = gneq[i] = 0.0;
-  for (int q = 0; q < 4; ++q) {
-    const int kc = t.kc[4 * j + q];
-    const double w = t.w[4 * j + q];
-    const double rc = cm[kOctRho * Pc + kc], uc = cm[kOctUx * Pc + kc],
-                 vc = cm[kOctUy * Pc + kc], tc = cm[kOctT * Pc + kc];
-    rho += w * rc;
-    ux += w * uc;
-    uy += w * vc;
-    temp += w * tc;
-    EqTerms<double> e;
-    eq_terms<double>(uc, vc, e);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      fneq[i] += w * (cm[i * Pc + kc] - eq_quad<double>(i, rc, e));
-      gneq[i] += w * (cm[(9 + i) * Pc + kc] - eq_g<double, GEQ>(i, tc, e));
-    }
+    for (int v = 0; v < kXferVals; ++v) sh[threadIdx.x][v] = c[v];
   }
-  EqTerms<double> e;
-  eq_terms<double>(ux, uy, e);
+  __syncthreads();
+  if (!valid || q != 0) return;
+  double s[kXferVals];
+#pragma unroll
+  for (int v = 0; v < kXferVals; ++v) {
+    double a = 0.0;
+    for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
+    s[v] = a;
+  }
+  const long long Pf = Lf.plane;
+  const double rho = s[0], ux = s[1], uy = s[2], temp = s[3];
+  const double* fneq = s + 4;
+  const double* gneq = s + 13;
+  EqTerms<double> eq;
+  eq_terms<double>(ux, uy, eq);
   double f[9], g[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     const int o = kOpp[i];
     const double even = 0.5 * (fneq[i] + fneq[o]);
     const double odd = 0.5 * (fneq[i] - fneq[o]);
-    f[i] = eq_quad<double>(i, rho, e) + (xi.even * even + xi.odd * odd);
-    g[i] = eq_g<double, GEQ>(i, temp, e) + xi.thermal * gneq[i];
+    f[i] = eq_quad<double>(i, rho, eq) + (xi.even * even + xi.odd * odd);
+    g[i] = eq_g<double, GEQ>(i, temp, eq) + xi.thermal * gneq[i];
   }
   const int kf = t.k_fine[j];
   double* fm = Lf.m;
@@ -293,46 +325,45 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi x
 // restricted node.
 template <int GEQ>
 __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi xi) {
+  constexpr int NS = 9, EPB = XferShape<NS>::kEntries;
+  __shared__ double sh[EPB * NS][kXferVals];
   int lb;
   const int item = group_item(grp.nblk0, lb);
   const OctLevel& Lf = grp.Lf[item];
   const OctLevel& Lc = grp.Lc[item];
   const OctCoincTab& t = grp.ct[item];
-  const int j = local_id(lb, t.n);
-  if (j < 0) return;
-  const long long Pc = Lc.plane, Pf = Lf.plane;
-  const double* fm = Lf.m;
-  double rho = 0.0, ux = 0.0, uy = 0.0, temp = 0.0;
-  double fneq[9], gneq[9];
+  const int e = (int)threadIdx.x / NS, q = (int)threadIdx.x % NS;
+  const int j = lb * EPB + e;
+  const bool valid = e < EPB && j < t.n;
+  if (valid) {
+    double c[kXferVals];
+    xfer_lane<GEQ, NS>(Lf, t.kf[9 * j + q], t.w[9 * j + q], c);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) fneq[i] = gneq[i] = 0.0;
-  for (int q = 0; q < 9; ++q) {
-    const int kf = t.kf[9 * j + q];
-    const double w = t.w[9 * j + q];
-    const double rf = fm[kOctRho * Pf + kf], uf = fm[kOctUx * Pf + kf],
-                 vf = fm[kOctUy * Pf + kf], tf = fm[kOctT * Pf + kf];
-    rho += w * rf;
-    ux += w * uf;
-    uy += w * vf;
-    temp += w * tf;
-    EqTerms<double> e;
-    eq_terms<double>(uf, vf, e);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      fneq[i] += w * (fm[i * Pf + kf] - eq_quad<double>(i, rf, e));
-      gneq[i] += w * (fm[(9 + i) * Pf + kf] - eq_g<double, GEQ>(i, tf, e));
-    }
+    for (int v = 0; v < kXferVals; ++v) sh[threadIdx.x][v] = c[v];
   }
-  EqTerms<double> e;
-  eq_terms<double>(ux, uy, e);
+  __syncthreads();
+  if (!valid || q != 0) return;
+  double s[kXferVals];
+#pragma unroll
+  for (int v = 0; v < kXferVals; ++v) {
+    double a = 0.0;
+    for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
+    s[v] = a;
+  }
+  const long long Pc = Lc.plane;
+  const double rho = s[0], ux = s[1], uy = s[2], temp = s[3];
+  const double* fneq = s + 4;
+  const double* gneq = s + 13;
+  EqTerms<double> eq;
+  eq_terms<double>(ux, uy, eq);
   double f[9], g[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     const int o = kOpp[i];
     const double even = 0.5 * (fneq[i] + fneq[o]);
     const double odd = 0.5 * (fneq[i] - fneq[o]);
-    f[i] = eq_quad<double>(i, rho, e) + (even / xi.even + odd / xi.odd);
-    g[i] = eq_g<double, GEQ>(i, temp, e) + gneq[i] / xi.thermal;
+    f[i] = eq_quad<double>(i, rho, eq) + (even / xi.even + odd / xi.odd);
+    g[i] = eq_g<double, GEQ>(i, temp, eq) + gneq[i] / xi.thermal;
   }
   const int kc = t.k_coarse[j];
   double* cm = Lc.m;
@@ -414,8 +445,9 @@ hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fin
     g.Lc[i] = coarse[i];
     g.Lf[i] = fine[i];
     g.gt[i] = t[i];
-    if (i == 0) g.nblk0 = nblk(t[i].n);
-    total += nblk(t[i].n);
+    const int nb = (t[i].n + XferShape<4>::kEntries - 1) / XferShape<4>::kEntries;
+    if (i == 0) g.nblk0 = nb;
+    total += nb;
   }
   if (total == 0) return hipSuccess;
   if (geq == 1)
@@ -434,8 +466,9 @@ hipError_t launch_oct_fine_to_coarse(const OctLevel* fine, const OctLevel* coars
     g.Lf[i] = fine[i];
     g.Lc[i] = coarse[i];
     g.ct[i] = t[i];
-    if (i == 0) g.nblk0 = nblk(t[i].n);
-    total += nblk(t[i].n);
+    const int nb = (t[i].n + XferShape<9>::kEntries - 1) / XferShape<9>::kEntries;
+    if (i == 0) g.nblk0 = nb;
+    total += nb;
   }
   if (total == 0) return hipSuccess;
   if (geq == 1)
