@@ -50,6 +50,9 @@ OctXi rescaling(const Relax& c, const Relax& f, int substeps, bool odd_exact) {
   x.even = dt * f.tp / c.tp;
   x.odd = odd_exact ? dt * f.tm / c.tm : dt;
   x.thermal = dt * f.tt / c.tt;
+  x.inv_even = 1.0 / x.even;
+  x.inv_odd = 1.0 / x.odd;
+  x.inv_thermal = 1.0 / x.thermal;
   return x;
 }
 
@@ -233,6 +236,20 @@ hipError_t substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
   return launch_oct_stream(L, S, count, o->geq, o->st);
 }
 
+// Two sub-steps with no transfer in between: collide, the fused stream of
+// the first and collision of the second (C-form -> second C-form), and the
+// stream of the second from that second C-form.
+hipError_t double_substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
+  OctLevel L[2] = {o->lv[g0], o->lv[g0 + (count > 1 ? 1 : 0)]};
+  OctStream S[2] = {stream_rules(o, g0), stream_rules(o, g0 + (count > 1 ? 1 : 0))};
+  hipError_t e = launch_oct_collide(L, count, c, o->geq, o->st);
+  if (e == hipSuccess) e = launch_oct_stream_collide(L, S, count, c, o->geq, o->st);
+  if (e != hipSuccess) return e;
+  OctLevel L2[2] = {L[0], L[1]};
+  for (auto& l : L2) std::swap(l.c, l.c2);
+  return launch_oct_stream(L2, S, count, o->geq, o->st);
+}
+
 // step_cavity (solver.rs:647-947) with the dynamic SGS off: enqueues the
 // launches of the coarse step `step`.  The reference walks the hot block and
 // then the cold one; the two touch disjoint fine levels and disjoint coarse
@@ -261,15 +278,16 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   } while (0)
   OCT_Q(substep(o, 0, 1, c0));
   OCT_Q(launch_oct_coarse_to_fine(coarse2, blocks, &o->gt[1], 2, o->xi_a, o->geq, o->st));
-  OCT_Q(substep(o, 1, 2, c1));
   if (o->levels == 3) {
     const OctLevel strips[2] = {o->lv[3], o->lv[4]};
+    OCT_Q(substep(o, 1, 2, c1));
     OCT_Q(launch_oct_coarse_to_fine(blocks, strips, &o->gt[3], 2, o->xi_b, o->geq, o->st));
-    OCT_Q(substep(o, 3, 2, c2));
-    OCT_Q(substep(o, 3, 2, c2));
+    OCT_Q(double_substep(o, 3, 2, c2));
     OCT_Q(launch_oct_fine_to_coarse(strips, blocks, &o->ct[3], 2, o->xi_b, o->geq, o->st));
+    OCT_Q(substep(o, 1, 2, c1));
+  } else {
+    OCT_Q(double_substep(o, 1, 2, c1));
   }
-  OCT_Q(substep(o, 1, 2, c1));
   OCT_Q(launch_oct_fine_to_coarse(blocks, coarse2, &o->ct[1], 2, o->xi_a, o->geq, o->st));
 #undef OCT_Q
   return hipSuccess;
@@ -456,7 +474,8 @@ int cavlbm_octree_create(const cavlbm_octree_params* pin, cavlbm_octree** out) {
     L.plane = (long long)L.nx * L.ny;
     L.m = dalloc<double>(o, (size_t)kOctM * L.plane);
     L.c = dalloc<double>(o, (size_t)18 * L.plane);
-    if (!L.m || !L.c) return fail_free("device allocation failed");
+    L.c2 = dalloc<double>(o, (size_t)18 * L.plane);
+    if (!L.m || !L.c || !L.c2) return fail_free("device allocation failed");
     if (hipMemsetAsync(L.m, 0, (size_t)kOctM * L.plane * sizeof(double), o->st) != hipSuccess)
       return fail_free("device memset failed");
   }

@@ -145,19 +145,14 @@ __device__ __forceinline__ bool bounce(const OctLevel& L, const OctStream& s, in
          (sy == L.ny - 1 && s.wt);
 }
 
+// stream + BCs + moments + walls of one node, from the post-collision set C.
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
-  int lb;
-  const int item = group_item(grp.nblk0, lb);
-  const OctLevel& L = grp.L[item];
-  const OctStream& s = grp.S[item];
-  const int k = local_id(lb, L.plane);
-  if (k < 0) return;
+__device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& s, const double* C,
+                                            int k, int ix, int iy, double (&f)[9],
+                                            double (&g)[9], double& rho, double& ux,
+                                            double& uy, double& temp) {
   const long long P = L.plane;
   const int nx = L.nx, ny = L.ny;
-  const int ix = k % nx, iy = k / nx;
-  const double* C = L.c;
-  double f[9], g[9];
 
   // f: pull; the block form leaves its ghost layers as they are
   const bool interior = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
@@ -207,7 +202,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
 
   // update_macroscopic_guo (cavity.rs:296-315), then the wall velocities
   const Moments mo = moments(f, g);
-  double ux = 0.0, uy = 0.0;
+  ux = 0.0;
+  uy = 0.0;
   if (mo.r > kF64Eps) {
     ux = mo.mx / mo.r;
     uy = mo.my / mo.r + 0.5 * L.m[kOctFy * P + k] / mo.r;
@@ -216,16 +212,60 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
     ux = 0.0;
     uy = 0.0;
   }
+  rho = mo.r;
+  temp = mo.t;
+}
+
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
+  int lb;
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& L = grp.L[item];
+  const int k = local_id(lb, L.plane);
+  if (k < 0) return;
+  const long long P = L.plane;
+  double f[9], g[9], rho, ux, uy, temp;
+  stream_node<GEQ>(L, grp.S[item], L.c, k, k % L.nx, k / L.nx, f, g, rho, ux, uy, temp);
   double* M = L.m;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     M[i * P + k] = f[i];
     M[(9 + i) * P + k] = g[i];
   }
-  M[kOctRho * P + k] = mo.r;
+  M[kOctRho * P + k] = rho;
   M[kOctUx * P + k] = ux;
   M[kOctUy * P + k] = uy;
-  M[kOctT * P + k] = mo.t;
+  M[kOctT * P + k] = temp;
+}
+
+// Two consecutive sub-steps of a level with no transfer between them: the
+// first sub-step's stream + BCs + moments + walls and the second sub-step's
+// collision in one pass, C-form (c) -> C-form (c2), exactly the two separate
+// launches' arithmetic.  The M-form is not needed in between; the following
+// stream launch reads c2.
+template <int GEQ>
+__global__ __launch_bounds__(kOctBlock) void k_oct_stream_collide(OctGroup grp, OctCollide p) {
+  int lb;
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& L = grp.L[item];
+  const int k = local_id(lb, L.plane);
+  if (k < 0) return;
+  const long long P = L.plane;
+  const int ix = k % L.nx, iy = k / L.nx;
+  double f[9], g[9], rho, ux, uy, temp;
+  stream_node<GEQ>(L, grp.S[item], L.c, k, ix, iy, f, g, rho, ux, uy, temp);
+  const bool in = !p.interior_only ||
+                  (ix >= 1 && ix <= L.nx - 2 && iy >= 1 && iy <= L.ny - 2);
+  if (in) {
+    double fy, nut;
+    collide_node<double, GEQ>(f, g, rho, ux, uy, temp, p.ctx, 0.0, fy, nut);
+    L.m[kOctFy * P + k] = fy;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    L.c2[i * P + k] = f[i];
+    L.c2[(9 + i) * P + k] = g[i];
+  }
 }
 
 // The interface transfers, one workgroup lane per (entry, stencil node):
@@ -283,14 +323,28 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi x
     for (int v = 0; v < kXferVals; ++v) sh[threadIdx.x][v] = c[v];
   }
   __syncthreads();
+  // the entry's lanes add the 22 quantities over the stencil, each in
+  // stencil order, into the first row of the entry's block of sh
+  double part[(kXferVals + NS - 1) / NS];
+#pragma unroll
+  for (int u = 0; u < (kXferVals + NS - 1) / NS; ++u) {
+    const int v = q + u * NS;
+    double a = 0.0;
+    if (valid && v < kXferVals)
+      for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
+    part[u] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < (kXferVals + NS - 1) / NS; ++u) {
+    const int v = q + u * NS;
+    if (valid && v < kXferVals) sh[e * NS][v] = part[u];
+  }
+  __syncthreads();
   if (!valid || q != 0) return;
   double s[kXferVals];
 #pragma unroll
-  for (int v = 0; v < kXferVals; ++v) {
-    double a = 0.0;
-    for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
-    s[v] = a;
-  }
+  for (int v = 0; v < kXferVals; ++v) s[v] = sh[e * NS][v];
   const long long Pf = Lf.plane;
   const double rho = s[0], ux = s[1], uy = s[2], temp = s[3];
   const double* fneq = s + 4;
@@ -342,14 +396,28 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
     for (int v = 0; v < kXferVals; ++v) sh[threadIdx.x][v] = c[v];
   }
   __syncthreads();
+  // the entry's lanes add the 22 quantities over the stencil, each in
+  // stencil order, into the first row of the entry's block of sh
+  double part[(kXferVals + NS - 1) / NS];
+#pragma unroll
+  for (int u = 0; u < (kXferVals + NS - 1) / NS; ++u) {
+    const int v = q + u * NS;
+    double a = 0.0;
+    if (valid && v < kXferVals)
+      for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
+    part[u] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < (kXferVals + NS - 1) / NS; ++u) {
+    const int v = q + u * NS;
+    if (valid && v < kXferVals) sh[e * NS][v] = part[u];
+  }
+  __syncthreads();
   if (!valid || q != 0) return;
   double s[kXferVals];
 #pragma unroll
-  for (int v = 0; v < kXferVals; ++v) {
-    double a = 0.0;
-    for (int r = 0; r < NS; ++r) a += sh[e * NS + r][v];
-    s[v] = a;
-  }
+  for (int v = 0; v < kXferVals; ++v) s[v] = sh[e * NS][v];
   const long long Pc = Lc.plane;
   const double rho = s[0], ux = s[1], uy = s[2], temp = s[3];
   const double* fneq = s + 4;
@@ -362,8 +430,13 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
     const int o = kOpp[i];
     const double even = 0.5 * (fneq[i] + fneq[o]);
     const double odd = 0.5 * (fneq[i] - fneq[o]);
-    f[i] = eq_quad<double>(i, rho, eq) + (even / xi.even + odd / xi.odd);
-    g[i] = eq_g<double, GEQ>(i, temp, eq) + gneq[i] / xi.thermal;
+    // a / xi as divr(a, xi, RN(1/xi)): the correctly rounded quotient
+    // (lbm_device.hpp); a zero dividend only changes the sign of a zero
+    // that is added to the nonzero equilibrium or to +0
+    f[i] = eq_quad<double>(i, rho, eq) + (Arith<double>::divr(even, xi.even, xi.inv_even) +
+                                          Arith<double>::divr(odd, xi.odd, xi.inv_odd));
+    g[i] = eq_g<double, GEQ>(i, temp, eq) +
+           Arith<double>::divr(gneq[i], xi.thermal, xi.inv_thermal);
   }
   const int kc = t.k_coarse[j];
   double* cm = Lc.m;
@@ -433,6 +506,23 @@ hipError_t launch_oct_stream(const OctLevel* L, const OctStream* s, int count, i
     hipLaunchKernelGGL(k_oct_stream<1>, dim3(total), dim3(kOctBlock), 0, st, g);
   else
     hipLaunchKernelGGL(k_oct_stream<2>, dim3(total), dim3(kOctBlock), 0, st, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_oct_stream_collide(const OctLevel* L, const OctStream* s, int count,
+                                     const OctCollide& p, int geq, hipStream_t st) {
+  OctGroup g{};
+  int total = 0;
+  for (int i = 0; i < count; ++i) {
+    g.L[i] = L[i];
+    g.S[i] = s[i];
+    if (i == 0) g.nblk0 = nblk(L[i].plane);
+    total += nblk(L[i].plane);
+  }
+  if (geq == 1)
+    hipLaunchKernelGGL(k_oct_stream_collide<1>, dim3(total), dim3(kOctBlock), 0, st, g, p);
+  else
+    hipLaunchKernelGGL(k_oct_stream_collide<2>, dim3(total), dim3(kOctBlock), 0, st, g, p);
   return hipGetLastError();
 }
 

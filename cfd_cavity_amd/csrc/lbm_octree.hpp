@@ -21,6 +21,7 @@ struct OctLevel {
   long long plane;  // nx * ny
   double* m;
   double* c;
+  double* c2;  // second C-form, for two sub-steps fused across the stream
 };
 
 // Relaxation and force of one collision pass (RelaxTimes of the level,
@@ -53,6 +54,7 @@ struct OctCoincTab {
 };
 struct OctXi {
   double even, odd, thermal;
+  double inv_even, inv_odd, inv_thermal;  // RN(1/xi), for the restriction's quotients
 };
 
 // Launches over `count` (1 or 2) independent levels or interfaces at once:
@@ -64,6 +66,8 @@ hipError_t launch_oct_collide(const OctLevel* L, int count, const OctCollide& p,
                               hipStream_t st);
 hipError_t launch_oct_stream(const OctLevel* L, const OctStream* s, int count, int geq,
                              hipStream_t st);
+hipError_t launch_oct_stream_collide(const OctLevel* L, const OctStream* s, int count,
+                                     const OctCollide& p, int geq, hipStream_t st);
 hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
                                      const OctGhostTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st);
