@@ -5,7 +5,9 @@ Workload (one GPU): D2Q9 f+g double-distribution cavity 256x1024, Ra=1e10,
 Pr=0.71, Pr_t=0.9, Cs=0.10, utarget=0.005, seed 42, the reference's init
 (linear T + xorshift64* noise of +-1e-3).  A step is one fused
 stream+BC+moments+collide pass over the whole lattice; inputs stay resident
-in HBM.  Prints ONE JSON line.
+in HBM.  The run loop advances two steps per kernel launch (temporal
+blocking, bitwise equal to single steps); the "fp64_1step" form times the
+single-step kernel beside it.  Prints ONE JSON line.
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
 rayon path) timed on this host's cores on a bounded sample of the same
@@ -24,9 +26,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# Algorithmic bytes per lattice update: read + write 9 f and 9 g populations
-# and the per-node Guo force fy_node (FP16-offset: plus the FP32 per-node
-# temperature plane its g offsets refer to).
+# Algorithmic bytes per node and step-kernel launch: read + write 9 f and 9 g
+# populations and the per-node Guo force fy_node (FP16-offset: plus the FP32
+# per-node temperature plane its g offsets refer to).  A launch advances one
+# step, or two with temporal blocking, so bytes per lattice update are these
+# divided by the steps per launch.
 BYTES_PER_LUP = {"fp64": 2 * (18 * 8 + 8), "fp32": 2 * (18 * 4 + 4),
                  "fp16_offset": 2 * (18 * 2 + 4 + 4)}
 DTYPE = {"fp64": "f64", "fp32": "f32", "fp16_offset": "f16-offset storage, f32 arithmetic"}
@@ -36,18 +40,24 @@ METRIC = "MLUPS & %HBM peak at 256x1024; mixed vs FP64 at rel dNu<=1e-4"
 WORKLOAD = dict(nx=256, ny=1024, ra=1.0e10, pr=0.71, cs=0.10, pr_t=0.9, utarget=0.005, seed=42)
 
 
-def make_solver(storage: str):
+def make_solver(storage: str, tblock: int = 2):
     from cfd_cavity_amd import CavitySolver
 
     w = WORKLOAD
-    return CavitySolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"], w["utarget"],
-                        w["seed"], 1, storage=storage)
+    s = CavitySolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"], w["utarget"],
+                     w["seed"], 1, storage=storage)
+    s.time_blocking = tblock
+    return s
 
 
-def gpu_form(storage: str, steps: int, warmup: int, kernel_launches: int):
+def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
+    """One storage form; a name ending in "_1step" runs the same storage with
+    one step per launch (the single-step kernel) for comparison."""
     import torch
 
-    s = make_solver(storage)
+    storage = form[:-len("_1step")] if form.endswith("_1step") else form
+    s = make_solver(storage, 1 if form.endswith("_1step") else 2)
+    spl = s.steps_per_launch
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     s.run(warmup)
     s.synchronize()
@@ -60,19 +70,23 @@ def gpu_form(storage: str, steps: int, warmup: int, kernel_launches: int):
     d = s.diagnostics()
     kern_ms = s.time_step_kernel(kernel_launches) if kernel_launches > 0 else float("nan")
     mlups = nodes * steps / wall / 1e6
+    # algorithmic bytes per node and launch: one read and one write of the
+    # state (a two-step launch crosses HBM once for both steps)
     b = BYTES_PER_LUP[storage]
-    # average fused-kernel launch duration over the timed region, from HIP
-    # events recorded on the solver's stream (one launch per step)
-    launch_ms = ev_ms / steps
+    # average step-kernel launch duration over the timed region, from HIP
+    # events recorded on the solver's stream (steps / spl launches)
+    launch_ms = ev_ms / (steps / spl)
     achieved = nodes * b / (launch_ms * 1e-3) / 1e9
     res = {
+        "steps_per_launch": spl,
         "mlups": mlups,
         "ms_per_step": wall * 1e3 / steps,
         "ms_per_step_events": ev_ms / steps,
         "kernel_ms_isolated": kern_ms,
         "launch_ms": launch_ms,
-        "bytes_per_lup": b,
-        "hbm_frac_whole_job": mlups * 1e6 * b / (HBM_PEAK_GBS * 1e9),
+        "bytes_per_launch_per_node": b,
+        "bytes_per_lup": b / spl,
+        "hbm_frac_whole_job": mlups * 1e6 * (b / spl) / (HBM_PEAK_GBS * 1e9),
         "kernel_gbs": achieved,
         "kernel_hbm_frac": achieved / HBM_PEAK_GBS,
         "nusselt_hot": d["nusselt_hot"],
@@ -253,7 +267,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--kernel-launches", type=int, default=200)
-    ap.add_argument("--forms", default="fp64,fp32,fp16_offset")
+    ap.add_argument("--forms", default="fp64,fp32,fp16_offset,fp64_1step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000)
@@ -318,7 +332,7 @@ def main():
         return
 
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
-    b = main_form["bytes_per_lup"]
+    b = main_form["bytes_per_launch_per_node"]
     line = {
         "metric": METRIC,
         "value": round(main_form["mlups"], 2),
@@ -342,6 +356,7 @@ def main():
                      "frac": round(main_form["kernel_hbm_frac"], 4),
                      "traffic": measured_traffic(),
                      "bytes_per_launch": nodes * b, "launch_ms": main_form["launch_ms"],
+                     "steps_per_launch": main_form["steps_per_launch"],
                      "timing": "HIP events around the timed steps on the solver stream"},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}

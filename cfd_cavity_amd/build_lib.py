@@ -3,8 +3,11 @@
 Each translation unit is compiled by hipcc with its own floating-point
 contract: the FP64 kernels, the auxiliary kernels and the host API with
 ``-ffp-contract=off`` (bitwise FP64 parity with an evaluation in the
-reference's operand order), the FP32-arithmetic kernels with contraction
-allowed.  Packed FP32 instructions are disabled for all device code.
+reference's operand order), the FP32-arithmetic step kernels with
+contraction inside each source expression only (``-ffp-contract=on``: the FMA
+formation follows the source, so the single-step and two-step kernels round
+alike), the batched FP32 kernel with contraction allowed across
+expressions.  Packed FP32 instructions are disabled for all device code.
 """
 from __future__ import annotations
 
@@ -26,7 +29,7 @@ COMMON = [
 ]
 UNITS = {
     "lbm_fp64.hip": ["-ffp-contract=off"],
-    "lbm_lowp.hip": ["-ffp-contract=fast"],
+    "lbm_lowp.hip": ["-ffp-contract=on"],
     "lbm_aux.hip": ["-ffp-contract=off"],
     "lbm_adjoint.hip": ["-ffp-contract=off"],
     "lbm_tangent.hip": ["-ffp-contract=off"],

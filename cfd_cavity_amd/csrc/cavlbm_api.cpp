@@ -47,19 +47,25 @@ struct cavlbm_solver {
   double* fld[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, T
   double* scratch_field = nullptr;                        // FP64 conversion buffer
   void* fy = nullptr;
+  void* fy_alt = nullptr;  // second fy plane of the two-step kernel (pointers swap)
   void* nut = nullptr;
   double* theta = nullptr;  // ny values, always present (0.5 when no profile)
   double* red_part = nullptr;
   double* red_out = nullptr;
   bool c_valid = false, m_valid = false;
   uint64_t time_step = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer parity
+  // by starting buffer parity cur | (fy parity << 1)
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  int fy_par = 0;  // swaps of fy / fy_alt so far, mod 2
   hipGraphExec_t dgraph[2] = {nullptr, nullptr};  // dynamic-SGS steps, by parity
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
   uint64_t avg_n = 0;
   bool in_batch = false;  // owned by a cavlbm_batch (stepped by the batched kernel)
+  int tblock = 2;         // steps per launch of the fused run loop (1 or 2)
+  Geom geo2{};            // tile grid of the two-step kernel
+  bool fits2 = false;     // the lattice and storage have a two-step form
   // Dynamic mixed-scale SGS (solver.rs:76-103).  dyn_on: compute nu_t/alpha_t
   // before every collision; dyn_pre: the per-node values exist and are used by
   // the collision (they persist, as the reference's Option does, after the
@@ -148,6 +154,27 @@ int materialize(cavlbm_solver* s) {
   HIP_TRY(s, launch(s, MODE_MATERIALIZE, make_ctx(s, s->time_step), 0));
   s->m_valid = true;
   return 0;
+}
+
+// Two fused steps in one launch (lbm_step2.hpp): storage equal to the
+// arithmetic type, a lattice that fits the two-step tiling, C-form state and
+// no dynamic model.
+bool pair_fits(const cavlbm_solver* s) {
+  return s->tblock == 2 && s->fits2 && !s->in_batch && !s->dyn_on && !s->dyn_pre;
+}
+bool pair_ok(const cavlbm_solver* s) { return s->c_valid && pair_fits(s); }
+
+// collide(c1) + stream + collide(c2) from cpop[cur] to cpop[cur ^ 1]; the
+// caller advances cur once and swaps the fy planes.
+hipError_t launch_pair(cavlbm_solver* s, const StepCtx& c1, const StepCtx& c2, int write_aux) {
+  const Walls w{s->theta};
+  const void* src = s->cpop[s->cur];
+  void* dst = s->cpop[s->cur ^ 1];
+  if (s->storage == ST_FP64)
+    return launch_step2_fp64(s->geq, s->geo2, c1, c2, w, src, dst, s->fy, s->fy_alt, s->nut,
+                             write_aux, s->st);
+  return launch_step2_lowp(s->storage, s->geq, s->geo2, c1, c2, w, src, dst, s->fy, s->fy_alt,
+                           s->nut, write_aux, s->st);
 }
 
 void drop_graphs(cavlbm_solver* s) {
@@ -240,20 +267,52 @@ int one_step(cavlbm_solver* s, int write_aux) {
   return 0;
 }
 
-// Captures kGraphSteps fused steps starting from buffer parity `par`.
-hipGraphExec_t capture(cavlbm_solver* s, int par) {
+void swap_fy(cavlbm_solver* s) {
+  std::swap(s->fy, s->fy_alt);
+  s->fy_par ^= 1;
+}
+
+// Two steps in one launch: collide(time_step), stream, collide(time_step + 1).
+int pair_step(cavlbm_solver* s, int write_aux) {
+  const StepCtx c1 = make_ctx(s, s->time_step);
+  const StepCtx c2 = make_ctx(s, s->time_step + 1);
+  HIP_TRY(s, launch_pair(s, c1, c2, write_aux));
+  s->cur ^= 1;
+  swap_fy(s);
+  s->m_valid = false;
+  s->time_step += 2;
+  return 0;
+}
+
+// Captures kGraphSteps fused steps from the current buffers (cur, fy): pairs
+// of steps per launch where the two-step kernel applies, single steps
+// otherwise.  kGraphSteps is a multiple of 4, so the replay ends on the
+// buffers it started from.
+hipGraphExec_t capture(cavlbm_solver* s) {
+  static_assert(kGraphSteps % 4 == 0, "graph must end on its starting buffers");
   const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);  // constant after spin-up
   const int saved = s->cur;
+  void* const fy_saved = s->fy;
+  void* const fy_alt_saved = s->fy_alt;
+  const int fy_par_saved = s->fy_par;
+  const bool pairs = pair_ok(s);
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-  s->cur = par;
   bool ok = true;
-  for (int i = 0; i < kGraphSteps && ok; ++i) {
-    ok = launch(s, MODE_FUSED, ctx, 0) == hipSuccess;
+  for (int i = 0; i < kGraphSteps && ok; i += pairs ? 2 : 1) {
+    if (pairs) {
+      ok = launch_pair(s, ctx, ctx, 0) == hipSuccess;
+      swap_fy(s);
+    } else {
+      ok = launch(s, MODE_FUSED, ctx, 0) == hipSuccess;
+    }
     s->cur ^= 1;
   }
   s->cur = saved;
+  s->fy = fy_saved;
+  s->fy_alt = fy_alt_saved;
+  s->fy_par = fy_par_saved;
   if (hipStreamEndCapture(s->st, &g) != hipSuccess || !ok) {
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
@@ -318,8 +377,8 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
     // nu_t.
     if (s->c_valid && !s->graph_failed && !s->dyn_on && !s->dyn_pre &&
         s->time_step >= s->d.spin_up_steps && left > (uint64_t)kGraphSteps) {
-      hipGraphExec_t& g = s->graph[s->cur];
-      if (!g) g = capture(s, s->cur);
+      hipGraphExec_t& g = s->graph[s->cur | (s->fy_par << 1)];
+      if (!g) g = capture(s);
       if (!g) {
         s->graph_failed = true;
         continue;
@@ -328,6 +387,12 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
       s->time_step += kGraphSteps;
       s->m_valid = false;
       left -= kGraphSteps;
+      continue;
+    }
+    if (pair_ok(s) && left >= 2) {
+      int rc = pair_step(s, left == 2 ? 1 : 0);
+      if (rc) return rc;
+      left -= 2;
       continue;
     }
     int rc = one_step(s, left == 1 ? 1 : 0);
@@ -349,6 +414,7 @@ void free_all(cavlbm_solver* s) {
   for (auto* p : s->fld) f(p);
   f(s->scratch_field);
   f(s->fy);
+  f(s->fy_alt);
   f(s->nut);
   f(s->theta);
   f(s->red_part);
@@ -493,6 +559,8 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   g.nbx = (g.nx + kTileX - 1) / kTileX;
   g.nby = (g.ny + kTileY - 1) / kTileY;
   g.plane = (long long)g.pitch * g.ny;
+  s->geo2 = g;
+  s->fits2 = step2_grid(p.storage, s->geo2);
   if (18 * g.plane >= (1LL << 31)) {
     delete s;
     return fail(nullptr, CAVLBM_E_PARAM, "lattice too large: 18*pitch*ny must be < 2^31");
@@ -515,6 +583,7 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   for (auto& f : s->fld) ok &= hipMalloc((void**)&f, P * sizeof(double)) == hipSuccess;
   ok &= hipMalloc((void**)&s->scratch_field, P * sizeof(double)) == hipSuccess;
   ok &= hipMalloc(&s->fy, P * s->r_bytes) == hipSuccess;
+  ok &= hipMalloc(&s->fy_alt, P * s->r_bytes) == hipSuccess;
   ok &= hipMalloc(&s->nut, P * s->r_bytes) == hipSuccess;
   ok &= hipMalloc((void**)&s->theta, p.ny * sizeof(double)) == hipSuccess;
   ok &= hipMalloc((void**)&s->red_part, (size_t)kDiagSlots * diag_blocks(g) * sizeof(double)) ==
@@ -524,6 +593,7 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   ok &= hipEventCreate(&s->ev1) == hipSuccess;
   if (!ok) return fail_free("device allocation failed");
   ok &= hipMemsetAsync(s->fy, 0, P * s->r_bytes, s->st) == hipSuccess;
+  ok &= hipMemsetAsync(s->fy_alt, 0, P * s->r_bytes, s->st) == hipSuccess;
   ok &= hipMemsetAsync(s->nut, 0, P * s->r_bytes, s->st) == hipSuccess;
   ok &= hipMemsetAsync(s->mpops, 0, 18 * P * sizeof(double), s->st) == hipSuccess;
   for (auto& f : s->fld) ok &= hipMemsetAsync(f, 0, P * sizeof(double), s->st) == hipSuccess;
@@ -885,7 +955,8 @@ int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms
   double total = 0.0;
   for (uint64_t i = 0; i < launches; ++i) {
     HIP_TRY(s, hipEventRecord(s->ev0, s->st));
-    int rc = one_step(s, i + 1 == launches ? 1 : 0);
+    const int aux = i + 1 == launches ? 1 : 0;
+    int rc = pair_ok(s) ? pair_step(s, aux) : one_step(s, aux);
     if (rc) return rc;
     HIP_TRY(s, hipEventRecord(s->ev1, s->st));
     HIP_TRY(s, hipEventSynchronize(s->ev1));
@@ -894,6 +965,24 @@ int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms
     total += t;
   }
   *mean_ms = total / (double)launches;
+  return 0;
+}
+
+int cavlbm_set_time_blocking(cavlbm_solver* s, int32_t steps) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (steps != 1 && steps != 2) return fail(s, CAVLBM_E_PARAM, "time blocking must be 1 or 2 steps");
+  if (s->tblock != steps) {
+    HIP_TRY(s, hipSetDevice(s->p.device));
+    drop_graphs(s);
+    s->graph_failed = false;
+    s->tblock = steps;
+  }
+  return 0;
+}
+
+int cavlbm_steps_per_launch(const cavlbm_solver* s, int32_t* out) {
+  if (!s || !out) return CAVLBM_E_PARAM;
+  *out = pair_fits(s) ? 2 : 1;
   return 0;
 }
 

@@ -335,6 +335,100 @@ __device__ __forceinline__ void tile_of_block(const Geom& geo, int& bx, int& by)
 }
 
 // ---------------------------------------------------------------------------
+// g of the isothermal columns (solver.rs:951-970): the equilibrium at rest at
+// the wall temperature; the hot column takes the profile value of its row,
+// the corner rows that of rows 1 / ny-2.
+template <class R, int GEQ>
+__device__ __forceinline__ void isothermal_g(const Walls& walls, int ix, int iy, int ny,
+                                             R (&g)[9]) {
+  R th;
+  if (ix == 0) {
+    const int iyc = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+    th = walls.theta_hot ? R(walls.theta_hot[iyc]) : R(kTHot);
+  } else {
+    th = R(kTCold);
+  }
+  EqTerms<R> e0;
+  eq_terms<R>(R(0.0), R(0.0), e0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) g[i] = eq_g<R, GEQ>(i, th, e0);
+}
+
+// update_macroscopic with the Guo correction (solver.rs:319-345) and
+// enforce_wall_velocity (solver.rs:992-1009) on the pulled populations.
+// Sums in the reference's order j = 0..8; zero products dropped and products
+// by -1 turned into subtractions (exact, accumulators from +0).
+template <class R>
+__device__ __forceinline__ void node_moments(const R (&f)[9], const R (&g)[9], R fyp,
+                                             bool wall, R& rho, R& ux, R& uy, R& temp) {
+  using A = Arith<R>;
+  const R r = R(0.0) + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  R un = R(0.0) + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
+  R vn = R(0.0) + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
+  const R tn = R(0.0) + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+  if (r > R(kF64Eps)) {
+    const R ir = A::rcp(r);
+    un = A::divr(un, r, ir);
+    vn = A::divr(vn, r, ir);
+    vn += A::divr(R(0.5) * fyp, r, ir);
+  }
+  if (wall) {
+    un = R(0.0);
+    vn = R(0.0);
+  }
+  rho = r;
+  ux = un;
+  uy = vn;
+  temp = tn;
+}
+
+// Streaming of one node from a C-form population set in HBM: pull with
+// full-way bounce-back for f (solver.rs:890-912), pull for g followed by the
+// thermal boundary conditions (solver.rs:951-988: isothermal columns, then
+// adiabatic rows copying rows 1 / ny-2), then the node's moments.
+template <int S, int GEQ>
+__device__ __forceinline__ void pull_node(const Geom& geo, const Walls& walls,
+                                          const typename StorageT<S>::T* src,
+                                          typename StorageT<S>::R fyp, int ix, int iy,
+                                          typename StorageT<S>::R (&f)[9],
+                                          typename StorageT<S>::R (&g)[9],
+                                          typename StorageT<S>::R& rho,
+                                          typename StorageT<S>::R& ux,
+                                          typename StorageT<S>::R& uy,
+                                          typename StorageT<S>::R& temp) {
+  using ST = StorageT<S>;
+  using R = typename ST::R;
+  const int nx = geo.nx, ny = geo.ny;
+  const unsigned P32 = (unsigned)geo.plane;  // 18*plane < 2^31 (checked on the host)
+  const int pitch = geo.pitch;
+  const int k = iy * pitch + ix;
+  // element offsets from the start of the population set fit in 32 bits
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int sx = ix - kCx[i], sy = iy - kCy[i];
+    const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
+    const unsigned o = in ? (unsigned)i * P32 + (unsigned)(sy * pitch + sx)
+                          : (unsigned)kOpp[i] * P32 + (unsigned)k;
+    f[i] = ST::load_f(src, o, in ? i : kOpp[i]);
+  }
+  if (ix == 0 || ix == nx - 1) {
+    isothermal_g<R, GEQ>(walls, ix, iy, ny, g);
+  } else {
+    const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+    const float* tplane =
+        reinterpret_cast<const float*>(src + (unsigned)18 * P32);  // used by FP16 only
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const unsigned nj = (unsigned)((gy - kCy[i]) * pitch + (ix - kCx[i]));
+      const R tj = ST::kTPlane ? R(tplane[nj]) : R(0.0);
+      g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 + nj, i, tj);
+    }
+  }
+  const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+  node_moments<R>(f, g, fyp, wall, rho, ux, uy, temp);
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel: one thread per lattice node.
 // The work of one lattice node (ix, iy) inside [0, nx) x [0, ny) for one
 // step kernel mode.
@@ -345,77 +439,17 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   using ST = StorageT<S>;
   using T = typename ST::T;
   using R = typename ST::R;
-  using A = Arith<R>;
-  const int nx = geo.nx, ny = geo.ny;
   const long long P = geo.plane;
-  const unsigned P32 = (unsigned)geo.plane;  // 18*plane < 2^31 (checked on the host)
-  const int pitch = geo.pitch;
-  const int k = iy * pitch + ix;
-  const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+  const unsigned P32 = (unsigned)geo.plane;
+  const int k = iy * geo.pitch + ix;
 
   R f[9], g[9];
   R rho, ux, uy, temp;
 
   if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE) {
-    const T* src = static_cast<const T*>(b.src);
-    // pull streaming with full-way bounce-back for f (solver.rs:890-912);
-    // element offsets from the start of the population set fit in 32 bits.
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      const int sx = ix - kCx[i], sy = iy - kCy[i];
-      const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
-      const unsigned o = in ? (unsigned)i * P32 + (unsigned)(sy * pitch + sx)
-                            : (unsigned)kOpp[i] * P32 + (unsigned)k;
-      f[i] = ST::load_f(src, o, in ? i : kOpp[i]);
-    }
-    // g: pull, then the thermal boundary conditions (solver.rs:951-988):
-    // isothermal columns, then adiabatic rows copying rows 1 / ny-2.
-    if (ix == 0 || ix == nx - 1) {
-      R th;
-      if (ix == 0) {
-        const int iyc = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
-        th = walls.theta_hot ? R(walls.theta_hot[iyc]) : R(kTHot);
-      } else {
-        th = R(kTCold);
-      }
-      EqTerms<R> e0;
-      eq_terms<R>(R(0.0), R(0.0), e0);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) g[i] = eq_g<R, GEQ>(i, th, e0);
-    } else {
-      const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
-      const float* tplane =
-          reinterpret_cast<const float*>(src + (unsigned)18 * P32);  // used by FP16 only
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const unsigned nj = (unsigned)((gy - kCy[i]) * pitch + (ix - kCx[i]));
-        const R tj = ST::kTPlane ? R(tplane[nj]) : R(0.0);
-        g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 + nj, i, tj);
-      }
-    }
-    // update_macroscopic with the Guo correction (solver.rs:319-345)
-    // sums in the reference's order j = 0..8; zero products dropped and
-    // products by -1 turned into subtractions (exact, accumulators from +0)
-    const R r = R(0.0) + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-    R un = R(0.0) + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
-    R vn = R(0.0) + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
-    const R tn = R(0.0) + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
     const R fyp = static_cast<const R*>(b.fy)[k];
-    if (r > R(kF64Eps)) {
-      const R ir = A::rcp(r);
-      un = A::divr(un, r, ir);
-      vn = A::divr(vn, r, ir);
-      vn += A::divr(R(0.5) * fyp, r, ir);
-    }
-    // enforce_wall_velocity (solver.rs:992-1009)
-    if (wall) {
-      un = R(0.0);
-      vn = R(0.0);
-    }
-    rho = r;
-    ux = un;
-    uy = vn;
-    temp = tn;
+    pull_node<S, GEQ>(geo, walls, static_cast<const T*>(b.src), fyp, ix, iy, f, g, rho, ux, uy,
+                      temp);
     if (MODE == MODE_MATERIALIZE) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {

@@ -124,6 +124,7 @@ class CavitySolver:
         self._nx, self._ny = nx, ny
         self.storage = storage
         self._owned = owned
+        self._tblock = 2
         self.lbm = _LbmView(self)
 
     @classmethod
@@ -401,6 +402,25 @@ class CavitySolver:
         return ms.value
 
     def time_step_kernel(self, launches):
+        """Mean duration (ms) of one launch of the run loop's step kernel
+        (two steps per launch when ``steps_per_launch`` is 2)."""
         ms = C.c_double()
         self._check(self._L.cavlbm_time_step_kernel(self._h, int(launches), C.byref(ms)))
         return ms.value
+
+    @property
+    def time_blocking(self):
+        """Steps per launch requested for the fused run loop (1 or 2)."""
+        return self._tblock
+
+    @time_blocking.setter
+    def time_blocking(self, steps):
+        self._check(self._L.cavlbm_set_time_blocking(self._h, int(steps)))
+        self._tblock = int(steps)
+
+    @property
+    def steps_per_launch(self):
+        """Steps per launch the run loop uses for this lattice and storage."""
+        out = C.c_int32()
+        self._check(self._L.cavlbm_steps_per_launch(self._h, C.byref(out)))
+        return out.value

@@ -186,6 +186,17 @@ int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
  * step kernels with its own event pair and returns the mean kernel time. */
 int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
 int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
+/* Temporal blocking of the fused run loop (no reference counterpart; a
+ * performance setting of this implementation): steps = 2 (the default)
+ * advances two steps per kernel launch wherever the storage is FP64 or FP32,
+ * the lattice fits the two-step tiling ((ny - 1) % 12 != 0 in FP64,
+ * (ny - 1) % 8 != 0 in FP32) and no dynamic
+ * model is active; steps = 1 launches one step at a time.  Both give the same
+ * populations bit for bit.  cavlbm_steps_per_launch reports what the run loop
+ * uses for this handle in its steady state (1 or 2); cavlbm_time_step_kernel
+ * times launches of that kind. */
+int cavlbm_set_time_blocking(cavlbm_solver *s, int32_t steps);
+int cavlbm_steps_per_launch(const cavlbm_solver *s, int32_t *out);
 
 /* ---- time averaging and snapshots (the run loop's helpers) ---------------
  * TimeAverager (post.rs:11-57) on the device: FP64 sums of ux, uy, T, nu_t and
