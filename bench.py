@@ -148,14 +148,17 @@ def batched_form(steps: int, warmup: int, storage: str = "fp32"):
             "hbm_frac": mlups * 1e6 * BYTES_PER_LUP[storage] / (HBM_PEAK_GBS * 1e9)}
 
 
-def measured_traffic():
-    """HBM bytes per fused FP64 launch at 256x1024 from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy
-    kernel), or None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_step_fused_256x1024.md")
+def measured_traffic(steps_per_launch: int):
+    """HBM bytes per FP64 step-kernel launch at 256x1024 (the two-step kernel
+    or the single-step one) from the committed rocprofv3 counter summaries
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy kernel), or None
+    when the summary is absent."""
+    name, key = (("k_step2_256x1024.md", "TRAFFIC_FP64_STEP2_BYTES_PER_LAUNCH")
+                 if steps_per_launch == 2 else
+                 ("k_step_fused_256x1024.md", "TRAFFIC_FP64_BYTES_PER_LAUNCH"))
     try:
-        for line in open(path):
-            if line.startswith("TRAFFIC_FP64_BYTES_PER_LAUNCH"):
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith(key):
                 return int(line.split("=")[1].strip())
     except OSError:
         pass
@@ -354,7 +357,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(main_form["kernel_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(main_form["kernel_hbm_frac"], 4),
-                     "traffic": measured_traffic(),
+                     "traffic": measured_traffic(main_form["steps_per_launch"]),
                      "bytes_per_launch": nodes * b, "launch_ms": main_form["launch_ms"],
                      "steps_per_launch": main_form["steps_per_launch"],
                      "timing": "HIP events around the timed steps on the solver stream"},
