@@ -5,8 +5,9 @@ Workload (one GPU): D2Q9 f+g double-distribution cavity 256x1024, Ra=1e10,
 Pr=0.71, Pr_t=0.9, Cs=0.10, utarget=0.005, seed 42, the reference's init
 (linear T + xorshift64* noise of +-1e-3).  A step is one fused
 stream+BC+moments+collide pass over the whole lattice; inputs stay resident
-in HBM.  The run loop advances two steps per kernel launch (temporal
-blocking, bitwise equal to single steps); the "fp64_1step" form times the
+in HBM.  The run loop advances several steps per kernel launch (temporal
+blocking, bitwise equal to single steps: 4 in FP64, 3 in FP32); the
+"fp64_2step" and "fp64_1step" forms time two steps per launch and the
 single-step kernel beside it.  Prints ONE JSON line.
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
@@ -40,23 +41,28 @@ METRIC = "MLUPS & %HBM peak at 256x1024; mixed vs FP64 at rel dNu<=1e-4"
 WORKLOAD = dict(nx=256, ny=1024, ra=1.0e10, pr=0.71, cs=0.10, pr_t=0.9, utarget=0.005, seed=42)
 
 
-def make_solver(storage: str, tblock: int = 2):
+def make_solver(storage: str, tblock=None):
     from cfd_cavity_amd import CavitySolver
 
     w = WORKLOAD
     s = CavitySolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"], w["utarget"],
                      w["seed"], 1, storage=storage)
-    s.time_blocking = tblock
+    if tblock is not None:
+        s.time_blocking = tblock
     return s
 
 
 def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
-    """One storage form; a name ending in "_1step" runs the same storage with
-    one step per launch (the single-step kernel) for comparison."""
+    """One storage form with the library's default steps per launch; a name
+    ending in "_<n>step" runs the same storage with at most n steps per
+    launch (n = 1: the single-step kernel), for comparison."""
     import torch
 
-    storage = form[:-len("_1step")] if form.endswith("_1step") else form
-    s = make_solver(storage, 1 if form.endswith("_1step") else 2)
+    storage, tblock = form, None
+    if form.endswith("step") and "_" in form:
+        storage, suffix = form.rsplit("_", 1)
+        tblock = int(suffix[:-len("step")])
+    s = make_solver(storage, tblock)
     spl = s.steps_per_launch
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     s.run(warmup)
@@ -149,12 +155,12 @@ def batched_form(steps: int, warmup: int, storage: str = "fp32"):
 
 
 def measured_traffic(steps_per_launch: int):
-    """HBM bytes per FP64 step-kernel launch at 256x1024 (the two-step kernel
+    """HBM bytes per FP64 step-kernel launch at 256x1024 (the N-step kernel
     or the single-step one) from the committed rocprofv3 counter summaries
     (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy kernel), or None
     when the summary is absent."""
-    name, key = (("k_step2_256x1024.md", "TRAFFIC_FP64_STEP2_BYTES_PER_LAUNCH")
-                 if steps_per_launch == 2 else
+    name, key = (("k_stepn_256x1024.md", f"TRAFFIC_FP64_STEP{steps_per_launch}_BYTES_PER_LAUNCH")
+                 if steps_per_launch > 1 else
                  ("k_step_fused_256x1024.md", "TRAFFIC_FP64_BYTES_PER_LAUNCH"))
     try:
         for line in open(os.path.join(ROOT, "profiles", name)):
@@ -270,7 +276,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--kernel-launches", type=int, default=200)
-    ap.add_argument("--forms", default="fp64,fp32,fp16_offset,fp64_1step")
+    ap.add_argument("--forms", default="fp64,fp32,fp16_offset,fp64_2step,fp64_1step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000)

@@ -29,7 +29,9 @@ namespace {
 
 thread_local std::string g_create_error;
 
-constexpr int kGraphSteps = 64;  // steps per captured graph (even: keeps buffer parity)
+// Steps per captured graph: a multiple of every steps-per-launch 1..4 with an
+// even launch count, so a replay ends on the buffers it started from.
+constexpr int kGraphSteps = 48;
 
 }  // namespace
 
@@ -47,7 +49,7 @@ struct cavlbm_solver {
   double* fld[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, T
   double* scratch_field = nullptr;                        // FP64 conversion buffer
   void* fy = nullptr;
-  void* fy_alt = nullptr;  // second fy plane of the two-step kernel (pointers swap)
+  void* fy_alt = nullptr;  // second fy plane of the N-step kernels (pointers swap)
   void* nut = nullptr;
   double* theta = nullptr;  // ny values, always present (0.5 when no profile)
   double* red_part = nullptr;
@@ -63,9 +65,9 @@ struct cavlbm_solver {
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
   uint64_t avg_n = 0;
   bool in_batch = false;  // owned by a cavlbm_batch (stepped by the batched kernel)
-  int tblock = 2;         // steps per launch of the fused run loop (1 or 2)
-  Geom geo2{};            // tile grid of the two-step kernel
-  bool fits2 = false;     // the lattice and storage have a two-step form
+  int tblock = 1;         // most steps per launch of the fused run loop (1..4)
+  Geom geon[kMaxBlockSteps + 1] = {};   // tile grid of the N-step kernel, by N
+  bool fitsn[kMaxBlockSteps + 1] = {};  // the lattice and storage have an N-step form
   // Dynamic mixed-scale SGS (solver.rs:76-103).  dyn_on: compute nu_t/alpha_t
   // before every collision; dyn_pre: the per-node values exist and are used by
   // the collision (they persist, as the reference's Option does, after the
@@ -156,24 +158,34 @@ int materialize(cavlbm_solver* s) {
   return 0;
 }
 
-// Two fused steps in one launch (lbm_step2.hpp): storage equal to the
-// arithmetic type, a lattice that fits the two-step tiling, C-form state and
-// no dynamic model.
-bool pair_fits(const cavlbm_solver* s) {
-  return s->tblock == 2 && s->fits2 && !s->in_batch && !s->dyn_on && !s->dyn_pre;
+// Steps per launch for a block of at most `left` steps (lbm_stepn.hpp): the
+// largest n <= min(tblock, left) whose N-step tiling fits the lattice, for
+// storage equal to the arithmetic type, outside a batch and without the
+// dynamic model; 1 (the single-step kernel) otherwise.  `steady` ignores
+// whether the C-form is current (what the run loop settles on).
+int block_steps(const cavlbm_solver* s, uint64_t left, bool steady = false) {
+  if ((!s->c_valid && !steady) || s->in_batch || s->dyn_on || s->dyn_pre) return 1;
+  const int top = (int)std::min<uint64_t>((uint64_t)s->tblock, left);
+  for (int n = top; n >= 2; --n)
+    if (s->fitsn[n]) return n;
+  return 1;
 }
-bool pair_ok(const cavlbm_solver* s) { return s->c_valid && pair_fits(s); }
 
-// collide(c1) + stream + collide(c2) from cpop[cur] to cpop[cur ^ 1]; the
-// caller advances cur once and swaps the fy planes.
-hipError_t launch_pair(cavlbm_solver* s, const StepCtx& c1, const StepCtx& c2, int write_aux) {
+// collide(time_step + j) with stream in between, j = 0..n-1, from cpop[cur]
+// to cpop[cur ^ 1] in one launch; `steady` uses the constant post-spin-up
+// scalars (graph capture).  The caller advances cur once and swaps the fy
+// planes.
+hipError_t launch_block(cavlbm_solver* s, int n, bool steady, int write_aux) {
+  StepCtxN cn{};
+  for (int j = 0; j < n; ++j)
+    cn.c[j] = make_ctx(s, steady ? s->d.spin_up_steps : s->time_step + (uint64_t)j);
   const Walls w{s->theta};
   const void* src = s->cpop[s->cur];
   void* dst = s->cpop[s->cur ^ 1];
   if (s->storage == ST_FP64)
-    return launch_step2_fp64(s->geq, s->geo2, c1, c2, w, src, dst, s->fy, s->fy_alt, s->nut,
+    return launch_stepn_fp64(s->geq, n, s->geon[n], cn, w, src, dst, s->fy, s->fy_alt, s->nut,
                              write_aux, s->st);
-  return launch_step2_lowp(s->storage, s->geq, s->geo2, c1, c2, w, src, dst, s->fy, s->fy_alt,
+  return launch_stepn_lowp(s->storage, s->geq, n, s->geon[n], cn, w, src, dst, s->fy, s->fy_alt,
                            s->nut, write_aux, s->st);
 }
 
@@ -272,37 +284,34 @@ void swap_fy(cavlbm_solver* s) {
   s->fy_par ^= 1;
 }
 
-// Two steps in one launch: collide(time_step), stream, collide(time_step + 1).
-int pair_step(cavlbm_solver* s, int write_aux) {
-  const StepCtx c1 = make_ctx(s, s->time_step);
-  const StepCtx c2 = make_ctx(s, s->time_step + 1);
-  HIP_TRY(s, launch_pair(s, c1, c2, write_aux));
+// n steps in one launch: collide(time_step + j), stream, ... (n >= 2).
+int block_step(cavlbm_solver* s, int n, int write_aux) {
+  HIP_TRY(s, launch_block(s, n, false, write_aux));
   s->cur ^= 1;
   swap_fy(s);
   s->m_valid = false;
-  s->time_step += 2;
+  s->time_step += (uint64_t)n;
   return 0;
 }
 
-// Captures kGraphSteps fused steps from the current buffers (cur, fy): pairs
-// of steps per launch where the two-step kernel applies, single steps
-// otherwise.  kGraphSteps is a multiple of 4, so the replay ends on the
-// buffers it started from.
+// Captures kGraphSteps fused steps from the current buffers (cur, fy): n
+// steps per launch (the run loop's steady choice), kGraphSteps / n launches,
+// an even count, so the replay ends on the buffers it started from.
 hipGraphExec_t capture(cavlbm_solver* s) {
-  static_assert(kGraphSteps % 4 == 0, "graph must end on its starting buffers");
   const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);  // constant after spin-up
   const int saved = s->cur;
   void* const fy_saved = s->fy;
   void* const fy_alt_saved = s->fy_alt;
   const int fy_par_saved = s->fy_par;
-  const bool pairs = pair_ok(s);
+  const int n = block_steps(s, kGraphSteps);
+  if (kGraphSteps % n != 0 || (kGraphSteps / n) % 2 != 0) return nullptr;
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   bool ok = true;
-  for (int i = 0; i < kGraphSteps && ok; i += pairs ? 2 : 1) {
-    if (pairs) {
-      ok = launch_pair(s, ctx, ctx, 0) == hipSuccess;
+  for (int i = 0; i < kGraphSteps && ok; i += n) {
+    if (n > 1) {
+      ok = launch_block(s, n, true, 0) == hipSuccess;
       swap_fy(s);
     } else {
       ok = launch(s, MODE_FUSED, ctx, 0) == hipSuccess;
@@ -389,10 +398,11 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
       left -= kGraphSteps;
       continue;
     }
-    if (pair_ok(s) && left >= 2) {
-      int rc = pair_step(s, left == 2 ? 1 : 0);
+    const int n = block_steps(s, left);
+    if (n > 1) {
+      int rc = block_step(s, n, left == (uint64_t)n ? 1 : 0);
       if (rc) return rc;
-      left -= 2;
+      left -= (uint64_t)n;
       continue;
     }
     int rc = one_step(s, left == 1 ? 1 : 0);
@@ -559,8 +569,12 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
   g.nbx = (g.nx + kTileX - 1) / kTileX;
   g.nby = (g.ny + kTileY - 1) / kTileY;
   g.plane = (long long)g.pitch * g.ny;
-  s->geo2 = g;
-  s->fits2 = step2_grid(p.storage, s->geo2);
+  for (int n = 2; n <= kMaxBlockSteps; ++n) {
+    s->geon[n] = g;
+    s->fitsn[n] = stepn_grid((int)p.storage, n, s->geon[n]);
+  }
+  s->tblock = p.storage == ST_FP64 ? kDefaultBlockFp64
+                                   : (p.storage == ST_FP32 ? kDefaultBlockFp32 : 1);
   if (18 * g.plane >= (1LL << 31)) {
     delete s;
     return fail(nullptr, CAVLBM_E_PARAM, "lattice too large: 18*pitch*ny must be < 2^31");
@@ -956,7 +970,8 @@ int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms
   for (uint64_t i = 0; i < launches; ++i) {
     HIP_TRY(s, hipEventRecord(s->ev0, s->st));
     const int aux = i + 1 == launches ? 1 : 0;
-    int rc = pair_ok(s) ? pair_step(s, aux) : one_step(s, aux);
+    const int n = block_steps(s, kGraphSteps);
+    int rc = n > 1 ? block_step(s, n, aux) : one_step(s, aux);
     if (rc) return rc;
     HIP_TRY(s, hipEventRecord(s->ev1, s->st));
     HIP_TRY(s, hipEventSynchronize(s->ev1));
@@ -970,7 +985,8 @@ int cavlbm_time_step_kernel(cavlbm_solver* s, uint64_t launches, double* mean_ms
 
 int cavlbm_set_time_blocking(cavlbm_solver* s, int32_t steps) {
   if (!s) return CAVLBM_E_PARAM;
-  if (steps != 1 && steps != 2) return fail(s, CAVLBM_E_PARAM, "time blocking must be 1 or 2 steps");
+  if (steps < 1 || steps > kMaxBlockSteps)
+    return fail(s, CAVLBM_E_PARAM, "time blocking must be 1 to 4 steps per launch");
   if (s->tblock != steps) {
     HIP_TRY(s, hipSetDevice(s->p.device));
     drop_graphs(s);
@@ -982,7 +998,7 @@ int cavlbm_set_time_blocking(cavlbm_solver* s, int32_t steps) {
 
 int cavlbm_steps_per_launch(const cavlbm_solver* s, int32_t* out) {
   if (!s || !out) return CAVLBM_E_PARAM;
-  *out = pair_fits(s) ? 2 : 1;
+  *out = block_steps(s, kGraphSteps, true);
   return 0;
 }
 

@@ -2,7 +2,6 @@
 // -ffp-contract=off: no FMA contraction, IEEE division and square root, so the
 // populations are bitwise equal to a CPU evaluation in the reference's order.
 #include "lbm_launch.hpp"
-#include "lbm_step2.hpp"
 
 namespace cavlbm {
 
@@ -33,40 +32,52 @@ hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& c
   return hipErrorInvalidValue;
 }
 
-template <int GEQ>
-static hipError_t go2(const Geom& geo2, const StepCtx& c1, const StepCtx& c2, const Walls& walls,
-                      const void* src, void* dst, const void* fy_in, void* fy_out, void* nut,
-                      int write_aux, hipStream_t st) {
-  hipLaunchKernelGGL((k_step2<ST_FP64, GEQ>), dim3(geo2.nbx * geo2.nby), dim3(Tile2<ST_FP64>::kNT), 0, st,
-                     geo2, c1, c2, walls, static_cast<const double*>(src),
-                     static_cast<double*>(dst), static_cast<const double*>(fy_in),
-                     static_cast<double*>(fy_out), static_cast<double*>(nut), write_aux);
-  return hipGetLastError();
-}
-
-bool step2_grid(int storage, Geom& geo2) {
+bool stepn_grid(int storage, int n, Geom& geo_n) {
   bool ok = false;
   int tx = 1, ty = 1;
-  if (storage == ST_FP64) {
-    ok = step2_fits<ST_FP64>(geo2.nx, geo2.ny);
-    tx = Tile2<ST_FP64>::kTX;
-    ty = Tile2<ST_FP64>::kTY;
-  } else if (storage == ST_FP32) {
-    ok = step2_fits<ST_FP32>(geo2.nx, geo2.ny);
-    tx = Tile2<ST_FP32>::kTX;
-    ty = Tile2<ST_FP32>::kTY;
+#define CAVLBM_TILE(S, N)                        \
+  if (storage == S && n == N) {                  \
+    ok = stepn_fits<S, N>(geo_n.nx, geo_n.ny);   \
+    tx = TileN<S, N>::kTX;                       \
+    ty = TileN<S, N>::kTY;                       \
   }
-  geo2.nbx = (geo2.nx + tx - 1) / tx;
-  geo2.nby = (geo2.ny + ty - 1) / ty;
+  CAVLBM_TILE(ST_FP64, 2)
+  CAVLBM_TILE(ST_FP64, 3)
+  CAVLBM_TILE(ST_FP64, 4)
+  CAVLBM_TILE(ST_FP32, 2)
+  CAVLBM_TILE(ST_FP32, 3)
+  CAVLBM_TILE(ST_FP32, 4)
+#undef CAVLBM_TILE
+  geo_n.nbx = (geo_n.nx + tx - 1) / tx;
+  geo_n.nby = (geo_n.ny + ty - 1) / ty;
   return ok;
 }
 
-hipError_t launch_step2_fp64(int geq, const Geom& geo2, const StepCtx& c1, const StepCtx& c2,
+template <int GEQ, int N>
+static hipError_t gon(const Geom& geo_n, const StepCtxN& cn, const Walls& walls, const void* src,
+                      void* dst, const void* fy_in, void* fy_out, void* nut, int write_aux,
+                      hipStream_t st) {
+  if (!stepn_fits<ST_FP64, N>(geo_n.nx, geo_n.ny)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_stepn<ST_FP64, GEQ, N>), dim3(geo_n.nbx * geo_n.nby),
+                     dim3(TileN<ST_FP64, N>::kNT), 0, st, geo_n, cn, walls,
+                     static_cast<const double*>(src), static_cast<double*>(dst),
+                     static_cast<const double*>(fy_in), static_cast<double*>(fy_out),
+                     static_cast<double*>(nut), write_aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_stepn_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st) {
-  if (!step2_fits<ST_FP64>(geo2.nx, geo2.ny)) return hipErrorInvalidValue;
-  if (geq == 2) return go2<2>(geo2, c1, c2, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
-  if (geq == 1) return go2<1>(geo2, c1, c2, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
+#define CAVLBM_CASE(G, N) \
+  if (geq == G && n == N) return gon<G, N>(geo_n, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
+  CAVLBM_CASE(2, 2)
+  CAVLBM_CASE(2, 3)
+  CAVLBM_CASE(2, 4)
+  CAVLBM_CASE(1, 2)
+  CAVLBM_CASE(1, 3)
+  CAVLBM_CASE(1, 4)
+#undef CAVLBM_CASE
   return hipErrorInvalidValue;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "lbm_device.hpp"
+#include "lbm_stepn.hpp"
 
 namespace cavlbm {
 
@@ -16,22 +17,21 @@ hipError_t launch_step_lowp(int storage, int geq, int mode, const Geom& geo,
                             const StepCtx& ctx, const Walls& walls, const StepBufs& b,
                             int write_aux, hipStream_t st);
 
-// Two consecutive fused steps (collide with c1, then stream and collide with
-// c2) per launch, see lbm_step2.hpp.  geo2 carries the two-step tile grid of
-// the storage type (step2_grid); fy is read from fy_in and
-// written to fy_out (distinct planes); nu_t of the second step is written when
-// write_aux is set.  hipErrorInvalidValue when the lattice does not fit the
-// two-step tiling or the storage has no two-step form.
-// step2_grid: whether the lattice fits the storage's two-step tiling, and the
-// tile grid (nbx, nby) written into geo2.
-bool step2_grid(int storage, Geom& geo2);
-hipError_t launch_step2_fp64(int geq, const Geom& geo2, const StepCtx& c1, const StepCtx& c2,
+// N consecutive fused steps per launch (N = 2..kMaxBlockSteps, collisions
+// with cn.c[0..N-1]), see lbm_stepn.hpp.  geo_n carries the tile grid of the
+// storage type and N (stepn_grid); fy is read from fy_in and written to fy_out
+// (distinct planes); nu_t of the last step is written when write_aux is set.
+// hipErrorInvalidValue when the lattice does not fit the tiling or the storage
+// has no N-step form.
+hipError_t launch_stepn_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st);
-hipError_t launch_step2_lowp(int storage, int geq, const Geom& geo2, const StepCtx& c1,
-                             const StepCtx& c2, const Walls& walls, const void* src, void* dst,
-                             const void* fy_in, void* fy_out, void* nut, int write_aux,
-                             hipStream_t st);
+hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+                             const Walls& walls, const void* src, void* dst, const void* fy_in,
+                             void* fy_out, void* nut, int write_aux, hipStream_t st);
+// stepn_grid: whether the lattice fits the N-step tiling of the storage type,
+// and that tiling's grid (nbx, nby) written into geo_n.
+bool stepn_grid(int storage, int n, Geom& geo_n);
 
 // f = feq / g = geq from the M-form fields, then u = 0 on the walls.
 hipError_t launch_init_eq(const Geom& geo, int geq, double* mpops, const double* rho,

@@ -4,7 +4,6 @@
 // and the single-step and two-step kernels round alike; the precision forms
 // are checked against FP64 with stated tolerances.
 #include "lbm_launch.hpp"
-#include "lbm_step2.hpp"
 
 namespace cavlbm {
 
@@ -45,26 +44,34 @@ hipError_t launch_step_lowp(int storage, int geq, int mode, const Geom& geo,
   return hipErrorInvalidValue;
 }
 
-template <int GEQ>
-static hipError_t go2(const Geom& geo2, const StepCtx& c1, const StepCtx& c2, const Walls& walls,
-                      const void* src, void* dst, const void* fy_in, void* fy_out, void* nut,
-                      int write_aux, hipStream_t st) {
-  hipLaunchKernelGGL((k_step2<ST_FP32, GEQ>), dim3(geo2.nbx * geo2.nby), dim3(Tile2<ST_FP32>::kNT), 0, st,
-                     geo2, c1, c2, walls, static_cast<const float*>(src),
-                     static_cast<float*>(dst), static_cast<const float*>(fy_in),
-                     static_cast<float*>(fy_out), static_cast<float*>(nut), write_aux);
+template <int GEQ, int N>
+static hipError_t gon(const Geom& geo_n, const StepCtxN& cn, const Walls& walls, const void* src,
+                      void* dst, const void* fy_in, void* fy_out, void* nut, int write_aux,
+                      hipStream_t st) {
+  if (!stepn_fits<ST_FP32, N>(geo_n.nx, geo_n.ny)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_stepn<ST_FP32, GEQ, N>), dim3(geo_n.nbx * geo_n.nby),
+                     dim3(TileN<ST_FP32, N>::kNT), 0, st, geo_n, cn, walls,
+                     static_cast<const float*>(src), static_cast<float*>(dst),
+                     static_cast<const float*>(fy_in), static_cast<float*>(fy_out),
+                     static_cast<float*>(nut), write_aux);
   return hipGetLastError();
 }
 
-// Two fused steps per launch for FP32 storage (FP16-offset storage rounds the
-// populations between steps and keeps the single-step kernel).
-hipError_t launch_step2_lowp(int storage, int geq, const Geom& geo2, const StepCtx& c1,
-                             const StepCtx& c2, const Walls& walls, const void* src, void* dst,
-                             const void* fy_in, void* fy_out, void* nut, int write_aux,
-                             hipStream_t st) {
-  if (storage != ST_FP32 || !step2_fits<ST_FP32>(geo2.nx, geo2.ny)) return hipErrorInvalidValue;
-  if (geq == 2) return go2<2>(geo2, c1, c2, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
-  if (geq == 1) return go2<1>(geo2, c1, c2, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
+// Several fused steps per launch for FP32 storage (FP16-offset storage rounds
+// the populations between steps and keeps the single-step kernel).
+hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+                             const Walls& walls, const void* src, void* dst, const void* fy_in,
+                             void* fy_out, void* nut, int write_aux, hipStream_t st) {
+  if (storage != ST_FP32) return hipErrorInvalidValue;
+#define CAVLBM_CASE(G, N) \
+  if (geq == G && n == N) return gon<G, N>(geo_n, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
+  CAVLBM_CASE(2, 2)
+  CAVLBM_CASE(2, 3)
+  CAVLBM_CASE(2, 4)
+  CAVLBM_CASE(1, 2)
+  CAVLBM_CASE(1, 3)
+  CAVLBM_CASE(1, 4)
+#undef CAVLBM_CASE
   return hipErrorInvalidValue;
 }
 

@@ -1,0 +1,174 @@
+// lbm_stepn.hpp — several consecutive fused steps per launch (temporal
+// blocking).
+//
+// The single-step kernel reads and writes the whole C-form state once per
+// step, so it is bound by HBM bandwidth with the collision's arithmetic using
+// about a third of the time.  Here one workgroup advances a tile of kTX x kTY
+// nodes by N steps.  Sub-step 1 runs on the tile grown by N-1 nodes on every
+// side, pulling from HBM; sub-step s (2..N) runs on the tile grown by N-s
+// nodes, pulling the populations of sub-step s-1 from LDS; sub-step N's
+// populations are stored.  The state crosses HBM once per N steps; the halo is
+// recomputed by the neighbouring tiles.
+//
+// Every node's arithmetic is the single-step kernel's (pull_node /
+// node_moments / collide_node), and the values a sub-step pulls are the ones
+// the previous step would have stored, so the result equals N launches of the
+// single-step kernel bit for bit for storage types equal to the arithmetic
+// type (FP64, FP32).
+//
+// fy is read at the halo nodes, which other tiles write in the same launch,
+// so it is double-buffered: the launch reads fy_in and writes fy_out.
+#pragma once
+
+#include "lbm_device.hpp"
+
+namespace cavlbm {
+
+constexpr int kMaxBlockSteps = 4;
+
+// Per-step scalars of the N collisions of one launch.
+struct StepCtxN {
+  StepCtx c[kMaxBlockSteps];
+};
+
+// Tile shapes per storage type and steps per launch: output tile kTX x kTY,
+// first sub-step on (kTX + 2(N-1)) x (kTY + 2(N-1)) nodes, kNT threads.
+// Measured at 256x1024 (shapes compared within one run):
+//   FP64: N=2 32x12 (against 32x16, 32x10, 24x16, 64x4, 32x6, 16x14, 32x4),
+//         N=3 16x16 (against 16x12, 16x20, 32x8, 32x10, 32x12, 64x4),
+//         N=4 16x16 (against 16x12, 32x8) -- the fastest, 16 % below N=2;
+//   FP32: N=2 64x8 (against 32x12..32x28, 64x12), N=3 32x16 (against 32x12,
+//         64x8, 16x16) -- the fastest, 15 % below N=2 -- and N=4 16x16
+//         (against 32x16).
+// The FP64 kernels take 106-120 VGPRs, so two workgroups of at most 8 waves
+// share a CU; the FP32 ones about 60.
+template <int S, int N> struct TileN;
+template <> struct TileN<ST_FP64, 2> { static constexpr int kTX = 32, kTY = 12, kNT = 512; };
+template <> struct TileN<ST_FP64, 3> { static constexpr int kTX = 16, kTY = 16, kNT = 448; };
+template <> struct TileN<ST_FP64, 4> { static constexpr int kTX = 16, kTY = 16, kNT = 512; };
+template <> struct TileN<ST_FP32, 2> { static constexpr int kTX = 64, kTY = 8, kNT = 704; };
+template <> struct TileN<ST_FP32, 3> { static constexpr int kTX = 32, kTY = 16, kNT = 768; };
+template <> struct TileN<ST_FP32, 4> { static constexpr int kTX = 16, kTY = 16, kNT = 512; };
+
+// Steps per launch the run loop uses by default, per storage type.
+constexpr int kDefaultBlockFp64 = 4;
+constexpr int kDefaultBlockFp32 = 3;
+
+// The adiabatic top row copies the pulls of row ny-2, which reach row ny-3.
+// For sub-steps s < N the grown tile's first row lies below the top row; for
+// the last one that row is inside the previous region unless the top row is
+// the first row of its output tile ((ny - 1) % kTY == 0).  The host uses
+// fewer steps per launch for such heights.
+template <int S, int N>
+__host__ __device__ inline bool stepn_fits(int nx, int ny) {
+  return nx >= 3 && ny >= 3 && (ny - 1) % TileN<S, N>::kTY != 0;
+}
+
+template <int S, int GEQ, int N>
+__global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
+    Geom geo, StepCtxN cn, Walls walls, const typename StorageT<S>::T* src,
+    typename StorageT<S>::T* dst, const typename StorageT<S>::R* fy_in,
+    typename StorageT<S>::R* fy_out, typename StorageT<S>::R* nut, int write_aux) {
+  using ST = StorageT<S>;
+  using R = typename ST::R;
+  constexpr int kTX = TileN<S, N>::kTX, kTY = TileN<S, N>::kTY;
+  constexpr int kH = N - 1;                         // halo of sub-step 1
+  constexpr int kE1X = kTX + 2 * kH;
+  constexpr int kE1 = kE1X * (kTY + 2 * kH);        // sub-step 1 nodes
+  static_assert(N >= 2 && N <= kMaxBlockSteps, "steps per launch");
+  static_assert(kE1 <= TileN<S, N>::kNT, "tile does not fit the workgroup");
+  // planes 0..8: f (then g) after a sub-step's collision; plane 9: its fy.
+  // Thread t's values sit at index t.
+  __shared__ R sh[10 * kE1];
+
+  const int nx = geo.nx, ny = geo.ny;
+  int bx, by;
+  tile_of_block(geo, bx, by);
+  const int t = (int)threadIdx.x;
+
+  // ---- sub-step 1: one thread per node of the grown tile, pulls from HBM
+  R f[9], g[9], fyc = R(0.0), nutc = R(0.0), tc = R(0.0);
+  bool have = false;
+  if (t < kE1) {
+    const int ix = bx * kTX - kH + t % kE1X, iy = by * kTY - kH + t / kE1X;
+    if (ix >= 0 && ix < nx && iy >= 0 && iy < ny) {
+      have = true;
+      R rho, ux, uy;
+      pull_node<S, GEQ>(geo, walls, src, fy_in[iy * geo.pitch + ix], ix, iy, f, g, rho, ux, uy,
+                        tc);
+      collide_node<R, GEQ>(f, g, rho, ux, uy, tc, cn.c[0], R(cn.c[0].cs), fyc, nutc);
+    }
+  }
+
+  // ---- sub-steps 2..N: pull the previous sub-step's populations from LDS
+#pragma unroll
+  for (int s = 2; s <= N; ++s) {
+    const int hp = N - (s - 1), hs = N - s;        // halos of the two regions
+    const int px = kTX + 2 * hp;                    // previous region's width
+    const int px0 = bx * kTX - hp, py0 = by * kTY - hp;
+    const int cw = kTX + 2 * hs, ch = kTY + 2 * hs;
+    if (have) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) sh[i * kE1 + t] = f[i];
+      sh[9 * kE1 + t] = fyc;
+    }
+    __syncthreads();
+    const int ix = bx * kTX - hs + t % cw, iy = by * kTY - hs + t / cw;
+    const bool act = t < cw * ch && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+    const int e = (iy - py0) * px + (ix - px0);     // this node in the previous region
+    R f2[9], g2[9], fyp = R(0.0);
+    if (act) {
+      // pull with full-way bounce-back (solver.rs:890-912)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const int sx = ix - kCx[i], sy = iy - kCy[i];
+        const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
+        f2[i] = in ? sh[i * kE1 + e - kCy[i] * px - kCx[i]] : sh[kOpp[i] * kE1 + e];
+      }
+      fyp = sh[9 * kE1 + e];
+    }
+    __syncthreads();
+    if (have) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) sh[i * kE1 + t] = g[i];
+    }
+    __syncthreads();
+    if (act) {
+      // g: pull, then the thermal boundary conditions (solver.rs:951-988)
+      if (ix == 0 || ix == nx - 1) {
+        isothermal_g<R, GEQ>(walls, ix, iy, ny, g2);
+      } else {
+        const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+        const int eg = (gy - py0) * px + (ix - px0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g2[i] = sh[i * kE1 + eg - kCy[i] * px - kCx[i]];
+      }
+      const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+      R rho, ux, uy;
+      node_moments<R>(f2, g2, fyp, wall, rho, ux, uy, tc);
+      collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, cn.c[s - 1], R(cn.c[s - 1].cs), fyc, nutc);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        f[i] = f2[i];
+        g[i] = g2[i];
+      }
+    }
+    have = act;
+    if (s < N) __syncthreads();  // the next sub-step rewrites LDS
+  }
+  if (!have) return;
+
+  // ---- store the output tile (the last sub-step's region)
+  const int ix = bx * kTX + t % kTX, iy = by * kTY + t / kTX;
+  const int k = iy * geo.pitch + ix;
+  const unsigned P32 = (unsigned)geo.plane;
+  fy_out[k] = fyc;
+  if (write_aux) nut[k] = nutc;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
+    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], tc);
+  }
+}
+
+}  // namespace cavlbm

@@ -124,7 +124,7 @@ class CavitySolver:
         self._nx, self._ny = nx, ny
         self.storage = storage
         self._owned = owned
-        self._tblock = 2
+        self._tblock = None
         self.lbm = _LbmView(self)
 
     @classmethod
@@ -410,7 +410,8 @@ class CavitySolver:
 
     @property
     def time_blocking(self):
-        """Steps per launch requested for the fused run loop (1 or 2)."""
+        """Most steps per launch requested for the fused run loop (1..4), or
+        None for the library's default (FP64 4, FP32 3, FP16-offset 1)."""
         return self._tblock
 
     @time_blocking.setter

@@ -187,14 +187,17 @@ int cavlbm_phase(cavlbm_solver *s, int32_t phase, uint64_t step);
 int cavlbm_time_run(cavlbm_solver *s, uint64_t nsteps, double *ms);
 int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms);
 /* Temporal blocking of the fused run loop (no reference counterpart; a
- * performance setting of this implementation): steps = 2 (the default)
- * advances two steps per kernel launch wherever the storage is FP64 or FP32,
- * the lattice fits the two-step tiling ((ny - 1) % 12 != 0 in FP64,
- * (ny - 1) % 8 != 0 in FP32) and no dynamic
- * model is active; steps = 1 launches one step at a time.  Both give the same
- * populations bit for bit.  cavlbm_steps_per_launch reports what the run loop
- * uses for this handle in its steady state (1 or 2); cavlbm_time_step_kernel
- * times launches of that kind. */
+ * performance setting of this implementation): the run loop advances up to
+ * `steps` (1..4) steps per kernel launch.  The defaults are 4 for FP64
+ * storage, 3 for FP32 and 1 for FP16-offset, whose populations are rounded
+ * between steps.  A block of n >= 2 steps needs a lattice that fits that
+ * block's tiling: (ny - 1) % 12 != 0 for FP64 n = 2, % 16 for FP64 n = 3, 4,
+ * % 8 for FP32 n = 2, % 16 for FP32 n = 3, 4.  It also needs no dynamic
+ * model and a handle outside a batch.  Otherwise the loop takes the largest
+ * block that fits, down to single steps.  Every choice gives the same
+ * populations bit for bit.  cavlbm_steps_per_launch reports the steps per
+ * launch of the run loop's steady state for this handle;
+ * cavlbm_time_step_kernel times launches of that kind. */
 int cavlbm_set_time_blocking(cavlbm_solver *s, int32_t steps);
 int cavlbm_steps_per_launch(const cavlbm_solver *s, int32_t *out);
 
