@@ -40,8 +40,8 @@ struct StepCtxN {
 //   FP32: N=2 64x8 (against 32x12..32x28, 64x12), N=3 32x16 (against 32x12,
 //         64x8, 16x16) -- the fastest, 15 % below N=2 -- and N=4 16x16
 //         (against 32x16).
-// The FP64 kernels take 106-120 VGPRs, so two workgroups of at most 8 waves
-// share a CU; the FP32 ones about 60.
+// The FP64 kernels take 110-122 VGPRs and 60-74 KB of LDS, so two workgroups
+// of at most 8 waves share a CU; the FP32 ones about 60 VGPRs.
 template <int S, int N> struct TileN;
 template <> struct TileN<ST_FP64, 2> { static constexpr int kTX = 32, kTY = 12, kNT = 512; };
 template <> struct TileN<ST_FP64, 3> { static constexpr int kTX = 16, kTY = 16, kNT = 448; };
@@ -77,9 +77,11 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
   constexpr int kE1 = kE1X * (kTY + 2 * kH);        // sub-step 1 nodes
   static_assert(N >= 2 && N <= kMaxBlockSteps, "steps per launch");
   static_assert(kE1 <= TileN<S, N>::kNT, "tile does not fit the workgroup");
-  // planes 0..8: f (then g) after a sub-step's collision; plane 9: its fy.
-  // Thread t's values sit at index t.
-  __shared__ R sh[10 * kE1];
+  // After a sub-step's collision: planes 0..8 f, plane 9 fy, planes 10..18 g.
+  // Thread t's values sit at index t.  One exchange of all 19 planes per
+  // sub-step (two barriers) measured 8-15 % faster than exchanging f and g in
+  // turn through 10 planes; FP64 tiles take 60-74 KB, two workgroups per CU.
+  __shared__ R sh[19 * kE1];
 
   const int nx = geo.nx, ny = geo.ny;
   int bx, by;
@@ -109,7 +111,10 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
     const int cw = kTX + 2 * hs, ch = kTY + 2 * hs;
     if (have) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) sh[i * kE1 + t] = f[i];
+      for (int i = 0; i < 9; ++i) {
+        sh[i * kE1 + t] = f[i];
+        sh[(10 + i) * kE1 + t] = g[i];
+      }
       sh[9 * kE1 + t] = fyc;
     }
     __syncthreads();
@@ -126,14 +131,6 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
         f2[i] = in ? sh[i * kE1 + e - kCy[i] * px - kCx[i]] : sh[kOpp[i] * kE1 + e];
       }
       fyp = sh[9 * kE1 + e];
-    }
-    __syncthreads();
-    if (have) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) sh[i * kE1 + t] = g[i];
-    }
-    __syncthreads();
-    if (act) {
       // g: pull, then the thermal boundary conditions (solver.rs:951-988)
       if (ix == 0 || ix == nx - 1) {
         isothermal_g<R, GEQ>(walls, ix, iy, ny, g2);
@@ -141,7 +138,7 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
         const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
         const int eg = (gy - py0) * px + (ix - px0);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g2[i] = sh[i * kE1 + eg - kCy[i] * px - kCx[i]];
+        for (int i = 0; i < 9; ++i) g2[i] = sh[(10 + i) * kE1 + eg - kCy[i] * px - kCx[i]];
       }
       const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
       R rho, ux, uy;
