@@ -386,7 +386,10 @@ __device__ __forceinline__ void node_moments(const R (&f)[9], const R (&g)[9], R
 // full-way bounce-back for f (solver.rs:890-912), pull for g followed by the
 // thermal boundary conditions (solver.rs:951-988: isothermal columns, then
 // adiabatic rows copying rows 1 / ny-2), then the node's moments.
-template <int S, int GEQ>
+// INTERIOR: the caller guarantees 1 <= ix <= nx-2 and 1 <= iy <= ny-2, where
+// every pull is inside the lattice and no boundary condition applies; the
+// same arithmetic without the checks.
+template <int S, int GEQ, bool INTERIOR = false>
 __device__ __forceinline__ void pull_node(const Geom& geo, const Walls& walls,
                                           const typename StorageT<S>::T* src,
                                           typename StorageT<S>::R fyp, int ix, int iy,
@@ -406,15 +409,15 @@ __device__ __forceinline__ void pull_node(const Geom& geo, const Walls& walls,
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     const int sx = ix - kCx[i], sy = iy - kCy[i];
-    const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
+    const bool in = INTERIOR || (sx >= 0 && sx < nx && sy >= 0 && sy < ny);
     const unsigned o = in ? (unsigned)i * P32 + (unsigned)(sy * pitch + sx)
                           : (unsigned)kOpp[i] * P32 + (unsigned)k;
     f[i] = ST::load_f(src, o, in ? i : kOpp[i]);
   }
-  if (ix == 0 || ix == nx - 1) {
+  if (!INTERIOR && (ix == 0 || ix == nx - 1)) {
     isothermal_g<R, GEQ>(walls, ix, iy, ny, g);
   } else {
-    const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+    const int gy = INTERIOR ? iy : (iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy));
     const float* tplane =
         reinterpret_cast<const float*>(src + (unsigned)18 * P32);  // used by FP16 only
 #pragma unroll
@@ -424,7 +427,7 @@ __device__ __forceinline__ void pull_node(const Geom& geo, const Walls& walls,
       g[i] = ST::load_g(src, (unsigned)(9 + i) * P32 + nj, i, tj);
     }
   }
-  const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+  const bool wall = !INTERIOR && (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1);
   node_moments<R>(f, g, fyp, wall, rho, ux, uy, temp);
 }
 

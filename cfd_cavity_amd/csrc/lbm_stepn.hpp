@@ -64,28 +64,25 @@ __host__ __device__ inline bool stepn_fits(int nx, int ny) {
   return nx >= 3 && ny >= 3 && (ny - 1) % TileN<S, N>::kTY != 0;
 }
 
-template <int S, int GEQ, int N>
-__global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
-    Geom geo, StepCtxN cn, Walls walls, const typename StorageT<S>::T* src,
-    typename StorageT<S>::T* dst, const typename StorageT<S>::R* fy_in,
-    typename StorageT<S>::R* fy_out, typename StorageT<S>::R* nut, int write_aux) {
+// One tile's N steps.  INTERIOR: every node of the first sub-step's region
+// lies in [1, nx-2] x [1, ny-2], so every pull is inside the lattice and no
+// boundary condition applies (the same arithmetic without the checks).
+template <int S, int GEQ, int N, bool INTERIOR>
+__device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
+                                           const Walls& walls,
+                                           const typename StorageT<S>::T* src,
+                                           typename StorageT<S>::T* dst,
+                                           const typename StorageT<S>::R* fy_in,
+                                           typename StorageT<S>::R* fy_out,
+                                           typename StorageT<S>::R* nut, int write_aux,
+                                           typename StorageT<S>::R* sh, int bx, int by) {
   using ST = StorageT<S>;
   using R = typename ST::R;
   constexpr int kTX = TileN<S, N>::kTX, kTY = TileN<S, N>::kTY;
   constexpr int kH = N - 1;                         // halo of sub-step 1
   constexpr int kE1X = kTX + 2 * kH;
   constexpr int kE1 = kE1X * (kTY + 2 * kH);        // sub-step 1 nodes
-  static_assert(N >= 2 && N <= kMaxBlockSteps, "steps per launch");
-  static_assert(kE1 <= TileN<S, N>::kNT, "tile does not fit the workgroup");
-  // After a sub-step's collision: planes 0..8 f, plane 9 fy, planes 10..18 g.
-  // Thread t's values sit at index t.  One exchange of all 19 planes per
-  // sub-step (two barriers) measured 8-15 % faster than exchanging f and g in
-  // turn through 10 planes; FP64 tiles take 60-74 KB, two workgroups per CU.
-  __shared__ R sh[19 * kE1];
-
   const int nx = geo.nx, ny = geo.ny;
-  int bx, by;
-  tile_of_block(geo, bx, by);
   const int t = (int)threadIdx.x;
 
   // ---- sub-step 1: one thread per node of the grown tile, pulls from HBM
@@ -93,11 +90,11 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
   bool have = false;
   if (t < kE1) {
     const int ix = bx * kTX - kH + t % kE1X, iy = by * kTY - kH + t / kE1X;
-    if (ix >= 0 && ix < nx && iy >= 0 && iy < ny) {
+    if (INTERIOR || (ix >= 0 && ix < nx && iy >= 0 && iy < ny)) {
       have = true;
       R rho, ux, uy;
-      pull_node<S, GEQ>(geo, walls, src, fy_in[iy * geo.pitch + ix], ix, iy, f, g, rho, ux, uy,
-                        tc);
+      pull_node<S, GEQ, INTERIOR>(geo, walls, src, fy_in[iy * geo.pitch + ix], ix, iy, f, g,
+                                  rho, ux, uy, tc);
       collide_node<R, GEQ>(f, g, rho, ux, uy, tc, cn.c[0], R(cn.c[0].cs), fyc, nutc);
     }
   }
@@ -119,7 +116,7 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
     }
     __syncthreads();
     const int ix = bx * kTX - hs + t % cw, iy = by * kTY - hs + t / cw;
-    const bool act = t < cw * ch && ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+    const bool act = t < cw * ch && (INTERIOR || (ix >= 0 && ix < nx && iy >= 0 && iy < ny));
     const int e = (iy - py0) * px + (ix - px0);     // this node in the previous region
     R f2[9], g2[9], fyp = R(0.0);
     if (act) {
@@ -127,20 +124,20 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         const int sx = ix - kCx[i], sy = iy - kCy[i];
-        const bool in = sx >= 0 && sx < nx && sy >= 0 && sy < ny;
+        const bool in = INTERIOR || (sx >= 0 && sx < nx && sy >= 0 && sy < ny);
         f2[i] = in ? sh[i * kE1 + e - kCy[i] * px - kCx[i]] : sh[kOpp[i] * kE1 + e];
       }
       fyp = sh[9 * kE1 + e];
       // g: pull, then the thermal boundary conditions (solver.rs:951-988)
-      if (ix == 0 || ix == nx - 1) {
+      if (!INTERIOR && (ix == 0 || ix == nx - 1)) {
         isothermal_g<R, GEQ>(walls, ix, iy, ny, g2);
       } else {
-        const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+        const int gy = INTERIOR ? iy : (iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy));
         const int eg = (gy - py0) * px + (ix - px0);
 #pragma unroll
         for (int i = 0; i < 9; ++i) g2[i] = sh[(10 + i) * kE1 + eg - kCy[i] * px - kCx[i]];
       }
-      const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
+      const bool wall = !INTERIOR && (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1);
       R rho, ux, uy;
       node_moments<R>(f2, g2, fyp, wall, rho, ux, uy, tc);
       collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, cn.c[s - 1], R(cn.c[s - 1].cs), fyc, nutc);
@@ -166,6 +163,38 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
     ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
     ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], tc);
   }
+}
+
+template <int S, int GEQ, int N>
+__global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
+    Geom geo, StepCtxN cn, Walls walls, const typename StorageT<S>::T* src,
+    typename StorageT<S>::T* dst, const typename StorageT<S>::R* fy_in,
+    typename StorageT<S>::R* fy_out, typename StorageT<S>::R* nut, int write_aux) {
+  using ST = StorageT<S>;
+  using R = typename ST::R;
+  constexpr int kTX = TileN<S, N>::kTX, kTY = TileN<S, N>::kTY;
+  constexpr int kH = N - 1;                         // halo of sub-step 1
+  constexpr int kE1X = kTX + 2 * kH;
+  constexpr int kE1 = kE1X * (kTY + 2 * kH);        // sub-step 1 nodes
+  static_assert(N >= 2 && N <= kMaxBlockSteps, "steps per launch");
+  static_assert(kE1 <= TileN<S, N>::kNT, "tile does not fit the workgroup");
+  // After a sub-step's collision: planes 0..8 f, plane 9 fy, planes 10..18 g.
+  // Thread t's values sit at index t.  One exchange of all 19 planes per
+  // sub-step (two barriers) measured 8-15 % faster than exchanging f and g in
+  // turn through 10 planes; FP64 tiles take 60-74 KB, two workgroups per CU.
+  __shared__ R sh[19 * kE1];
+
+  int bx, by;
+  tile_of_block(geo, bx, by);
+  // a workgroup-uniform choice of the path without boundary checks
+  const bool interior = bx * kTX - kH >= 1 && (bx + 1) * kTX + kH - 1 <= geo.nx - 2 &&
+                        by * kTY - kH >= 1 && (by + 1) * kTY + kH - 1 <= geo.ny - 2;
+  if (interior)
+    stepn_tile<S, GEQ, N, true>(geo, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, sh, bx,
+                                by);
+  else
+    stepn_tile<S, GEQ, N, false>(geo, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, sh, bx,
+                                 by);
 }
 
 }  // namespace cavlbm
