@@ -27,11 +27,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# Algorithmic bytes per node and step-kernel launch: read + write 9 f and 9 g
-# populations and the per-node Guo force fy_node (FP16-offset: plus the FP32
-# per-node temperature plane its g offsets refer to).  A launch advances one
-# step, or two with temporal blocking, so bytes per lattice update are these
-# divided by the steps per launch.
+# Algorithmic bytes per lattice update: read + write 9 f and 9 g populations
+# and the per-node Guo force fy_node (FP16-offset: plus the FP32 per-node
+# temperature plane its g offsets refer to).  A launch of the N-step kernel
+# processes N lattice updates per node, so its algorithmic bytes are
+# N * nodes * these, while it moves the state across HBM once (the measured
+# traffic, from the committed counter summaries).
 BYTES_PER_LUP = {"fp64": 2 * (18 * 8 + 8), "fp32": 2 * (18 * 4 + 4),
                  "fp16_offset": 2 * (18 * 2 + 4 + 4)}
 DTYPE = {"fp64": "f64", "fp32": "f32", "fp16_offset": "f16-offset storage, f32 arithmetic"}
@@ -76,13 +77,12 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
     d = s.diagnostics()
     kern_ms = s.time_step_kernel(kernel_launches) if kernel_launches > 0 else float("nan")
     mlups = nodes * steps / wall / 1e6
-    # algorithmic bytes per node and launch: one read and one write of the
-    # state (a two-step launch crosses HBM once for both steps)
     b = BYTES_PER_LUP[storage]
     # average step-kernel launch duration over the timed region, from HIP
-    # events recorded on the solver's stream (steps / spl launches)
+    # events recorded on the solver's stream (steps / spl launches), and the
+    # algorithmic bytes of the spl lattice updates per node of one launch
     launch_ms = ev_ms / (steps / spl)
-    achieved = nodes * b / (launch_ms * 1e-3) / 1e9
+    achieved = spl * nodes * b / (launch_ms * 1e-3) / 1e9
     res = {
         "steps_per_launch": spl,
         "mlups": mlups,
@@ -90,9 +90,9 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
         "ms_per_step_events": ev_ms / steps,
         "kernel_ms_isolated": kern_ms,
         "launch_ms": launch_ms,
-        "bytes_per_launch_per_node": b,
-        "bytes_per_lup": b / spl,
-        "hbm_frac_whole_job": mlups * 1e6 * (b / spl) / (HBM_PEAK_GBS * 1e9),
+        "bytes_per_lup": b,
+        "state_bytes_per_launch": nodes * b,
+        "hbm_frac_whole_job": mlups * 1e6 * b / (HBM_PEAK_GBS * 1e9),
         "kernel_gbs": achieved,
         "kernel_hbm_frac": achieved / HBM_PEAK_GBS,
         "nusselt_hot": d["nusselt_hot"],
@@ -341,7 +341,9 @@ def main():
         return
 
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
-    b = main_form["bytes_per_launch_per_node"]
+    b = main_form["bytes_per_lup"]
+    spl = main_form["steps_per_launch"]
+    traffic = measured_traffic(spl)
     line = {
         "metric": METRIC,
         "value": round(main_form["mlups"], 2),
@@ -363,9 +365,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(main_form["kernel_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(main_form["kernel_hbm_frac"], 4),
-                     "traffic": measured_traffic(main_form["steps_per_launch"]),
-                     "bytes_per_launch": nodes * b, "launch_ms": main_form["launch_ms"],
-                     "steps_per_launch": main_form["steps_per_launch"],
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": spl * nodes * b,
+                     "traffic_bytes_per_lup": (round(traffic / (spl * nodes), 1)
+                                               if traffic else None),
+                     "launch_ms": main_form["launch_ms"], "steps_per_launch": spl,
                      "timing": "HIP events around the timed steps on the solver stream"},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
