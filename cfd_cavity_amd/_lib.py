@@ -41,7 +41,8 @@ class Diag(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile_x", C.c_int32), ("tile_y", C.c_int32), ("xcd_remap", C.c_int32),
                 ("candidates", C.c_int32), ("best_ms_per_step", C.c_double),
-                ("first_ms_per_step", C.c_double)]
+                ("first_ms_per_step", C.c_double), ("steps_per_launch", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Derived(C.Structure):

@@ -1065,12 +1065,16 @@ struct cavlbm_batch {
   int storage = 0, geq = 2, device = 0;
   hipStream_t st = nullptr;
   CavDesc* d_desc = nullptr;
-  int* d_tile_cav = nullptr;
+  int* d_tile_cav = nullptr;   // single-step tile -> cavity table
   int total_tiles = 0;
+  int* d_tile_cavn = nullptr;  // N-step tile -> cavity table (nblk >= 2)
+  int total_tiles_n = 0;
   int shape = 0, xcd = 1;
+  int nblk = 1;                // steps per launch of the batched run loop
   uint64_t max_spin = 0;
   uint64_t nodes = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // by starting buffer parity cur | (fy parity << 1)
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -1101,23 +1105,34 @@ void batch_drop_graphs(cavlbm_batch* b) {
     }
 }
 
-// (Re)builds the cavity descriptors and the tile table for the current tile
-// shape; buffer pointers come from each solver's two C-form sets.
+// Whether every cavity of the batch fits the N-step tiling.
+bool batch_fits(const cavlbm_batch* b, int n) {
+  if (n < 2 || n > kMaxBlockSteps) return false;
+  for (const cavlbm_solver* s : b->cav)
+    if (!s->fitsn[n]) return false;
+  return true;
+}
+
+// (Re)builds the cavity descriptors and the tile tables for the current
+// single-step tile shape and steps per launch; buffer pointers come from each
+// solver's two C-form sets and two fy planes.
 int batch_layout(cavlbm_batch* b) {
   const TileShape sh = kBatchShapes[b->shape];
   std::vector<CavDesc> desc(b->cav.size());
-  std::vector<int> tile_cav;
-  int begin = 0;
+  std::vector<int> tile_cav, tile_cavn;
+  int begin = 0, begin_n = 0;
   for (size_t c = 0; c < b->cav.size(); ++c) {
     cavlbm_solver* s = b->cav[c];
     CavDesc& d = desc[c];
     d.geo = s->geo;
     d.geo.nbx = (s->geo.nx + sh.tx - 1) / sh.tx;
     d.geo.nby = (s->geo.ny + sh.ty - 1) / sh.ty;
+    d.geon = b->nblk >= 2 ? s->geon[b->nblk] : s->geo;
     d.ctx = make_ctx(s, s->d.spin_up_steps);
     d.beta_eff = s->d.beta_eff;
     d.spin_up = (int)s->d.spin_up_steps;
     d.tile_begin = begin;
+    d.tile_begin_n = begin_n;
     d.walls = Walls{s->theta};
     for (int par = 0; par < 2; ++par) {
       const int saved = s->cur;
@@ -1125,34 +1140,61 @@ int batch_layout(cavlbm_batch* b) {
       d.bufs[par] = bufs(s);
       s->cur = saved;
     }
+    d.fyp[s->fy_par] = s->fy;
+    d.fyp[s->fy_par ^ 1] = s->fy_alt;
     const int nt = d.geo.nbx * d.geo.nby;
     tile_cav.insert(tile_cav.end(), nt, (int)c);
     begin += nt;
+    if (b->nblk >= 2) {
+      const int ntn = d.geon.nbx * d.geon.nby;
+      tile_cavn.insert(tile_cavn.end(), ntn, (int)c);
+      begin_n += ntn;
+    }
   }
   b->total_tiles = begin;
+  b->total_tiles_n = begin_n;
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_tile_cav) (void)hipFree(b->d_tile_cav);
+  if (b->d_tile_cavn) (void)hipFree(b->d_tile_cavn);
   b->d_desc = nullptr;
   b->d_tile_cav = nullptr;
+  b->d_tile_cavn = nullptr;
   BHIP_TRY(b, hipMalloc((void**)&b->d_desc, desc.size() * sizeof(CavDesc)));
   BHIP_TRY(b, hipMalloc((void**)&b->d_tile_cav, tile_cav.size() * sizeof(int)));
   BHIP_TRY(b, hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(CavDesc),
                              hipMemcpyHostToDevice, b->st));
   BHIP_TRY(b, hipMemcpyAsync(b->d_tile_cav, tile_cav.data(), tile_cav.size() * sizeof(int),
                              hipMemcpyHostToDevice, b->st));
+  if (!tile_cavn.empty()) {
+    BHIP_TRY(b, hipMalloc((void**)&b->d_tile_cavn, tile_cavn.size() * sizeof(int)));
+    BHIP_TRY(b, hipMemcpyAsync(b->d_tile_cavn, tile_cavn.data(), tile_cavn.size() * sizeof(int),
+                               hipMemcpyHostToDevice, b->st));
+  }
   BHIP_TRY(b, hipStreamSynchronize(b->st));
   batch_drop_graphs(b);
   b->graph_failed = false;
   return 0;
 }
 
-hipError_t batch_launch(cavlbm_batch* b, int parity, uint64_t step, int write_aux) {
+// One step of every cavity (single-step tiles).
+hipError_t batch_launch(cavlbm_batch* b, int parity, int fpar, uint64_t step, int write_aux) {
   const int stepi = step > (uint64_t)INT32_MAX ? INT32_MAX : (int)step;
   if (b->storage == ST_FP64)
     return launch_step_batch_fp64(b->geq, b->shape, b->d_desc, b->d_tile_cav, b->total_tiles,
-                                  parity, stepi, write_aux, b->xcd, b->st);
+                                  parity, fpar, stepi, write_aux, b->xcd, b->st);
   return launch_step_batch_lowp(b->storage, b->geq, b->shape, b->d_desc, b->d_tile_cav,
-                                b->total_tiles, parity, stepi, write_aux, b->xcd, b->st);
+                                b->total_tiles, parity, fpar, stepi, write_aux, b->xcd, b->st);
+}
+
+// nblk steps of every cavity (N-step tiles); reads fy plane fpar, writes fpar ^ 1.
+hipError_t batch_launch_n(cavlbm_batch* b, int parity, int fpar, uint64_t step, int write_aux) {
+  const int stepi = step > (uint64_t)INT32_MAX - kMaxBlockSteps ? INT32_MAX - kMaxBlockSteps
+                                                                : (int)step;
+  if (b->storage == ST_FP64)
+    return launch_stepn_batch_fp64(b->geq, b->nblk, b->d_desc, b->d_tile_cavn, b->total_tiles_n,
+                                   parity, fpar, stepi, write_aux, b->xcd, b->st);
+  return launch_stepn_batch_lowp(b->storage, b->geq, b->nblk, b->d_desc, b->d_tile_cavn,
+                                 b->total_tiles_n, parity, fpar, stepi, write_aux, b->xcd, b->st);
 }
 
 // Every cavity in C-form, with a common buffer parity and step count.  A
@@ -1173,7 +1215,7 @@ int batch_prepare(cavlbm_batch* b, int write_aux, uint64_t* consumed) {
   for (cavlbm_solver* s : b->cav) BHIP_TRY(b, hipStreamSynchronize(s->st));
   const cavlbm_solver* s0 = b->cav[0];
   for (cavlbm_solver* s : b->cav)
-    if (s->cur != s0->cur || s->time_step != s0->time_step)
+    if (s->cur != s0->cur || s->fy_par != s0->fy_par || s->time_step != s0->time_step)
       return bfail(b, CAVLBM_E_PARAM, "batch cavities out of lockstep (stepped individually?)");
   return 0;
 }
@@ -1184,22 +1226,35 @@ bool batch_ready(const cavlbm_batch* b) {
   return true;
 }
 
-void batch_advance(cavlbm_batch* b, uint64_t k) {
+// k steps taken by `launches` launches (one buffer flip each); an N-step
+// launch also exchanges the fy planes.
+void batch_advance(cavlbm_batch* b, uint64_t k, uint64_t launches, bool block) {
   for (cavlbm_solver* s : b->cav) {
     s->time_step += k;
-    if (k & 1) s->cur ^= 1;
+    if (launches & 1) {
+      s->cur ^= 1;
+      if (block) swap_fy(s);
+    }
     s->m_valid = false;
   }
 }
 
-hipGraphExec_t batch_capture(cavlbm_batch* b, int par) {
+// kGraphSteps steps from buffers (par, fpar): kGraphSteps / nblk N-step
+// launches, or single steps; an even launch count either way.
+hipGraphExec_t batch_capture(cavlbm_batch* b, int par, int fpar) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   bool ok = true;
-  int p = par;
-  for (int i = 0; i < kGraphSteps && ok; ++i) {
-    ok = batch_launch(b, p, b->max_spin, 0) == hipSuccess;
+  int p = par, q = fpar;
+  const int n = b->nblk;
+  for (int i = 0; i < kGraphSteps && ok; i += n) {
+    if (n >= 2) {
+      ok = batch_launch_n(b, p, q, b->max_spin, 0) == hipSuccess;
+      q ^= 1;
+    } else {
+      ok = batch_launch(b, p, q, b->max_spin, 0) == hipSuccess;
+    }
     p ^= 1;
   }
   if (hipStreamEndCapture(b->st, &g) != hipSuccess || !ok) {
@@ -1222,19 +1277,26 @@ int batch_run_steps(cavlbm_batch* b, uint64_t n) {
   cavlbm_solver* s0 = b->cav[0];
   while (left > 0) {
     if (!b->graph_failed && s0->time_step >= b->max_spin && left > (uint64_t)kGraphSteps) {
-      hipGraphExec_t& g = b->graph[s0->cur];
-      if (!g) g = batch_capture(b, s0->cur);
+      hipGraphExec_t& g = b->graph[s0->cur | (s0->fy_par << 1)];
+      if (!g) g = batch_capture(b, s0->cur, s0->fy_par);
       if (!g) {
         b->graph_failed = true;
         continue;
       }
       BHIP_TRY(b, hipGraphLaunch(g, b->st));
-      batch_advance(b, kGraphSteps);
+      batch_advance(b, kGraphSteps, 0, false);  // an even number of launches
       left -= kGraphSteps;
       continue;
     }
-    BHIP_TRY(b, batch_launch(b, s0->cur, s0->time_step, left == 1 ? 1 : 0));
-    batch_advance(b, 1);
+    const uint64_t n = (uint64_t)b->nblk;
+    if (n >= 2 && left >= n) {
+      BHIP_TRY(b, batch_launch_n(b, s0->cur, s0->fy_par, s0->time_step, left == n ? 1 : 0));
+      batch_advance(b, n, 1, true);
+      left -= n;
+      continue;
+    }
+    BHIP_TRY(b, batch_launch(b, s0->cur, s0->fy_par, s0->time_step, left == 1 ? 1 : 0));
+    batch_advance(b, 1, 1, false);
     --left;
   }
   return 0;
@@ -1271,6 +1333,16 @@ int cavlbm_batch_create(const cavlbm_params* cavities, uint64_t n, cavlbm_batch*
     b->max_spin = std::max<uint64_t>(b->max_spin, s->d.spin_up_steps);
     b->nodes += s->p.nx * s->p.ny;
   }
+  // steps per launch: the storage's default, or the largest that every
+  // cavity's tiling fits
+  const int want = b->storage == ST_FP64 ? kDefaultBlockFp64
+                                         : (b->storage == ST_FP32 ? kDefaultBlockFp32 : 1);
+  b->nblk = 1;
+  for (int n = want; n >= 2; --n)
+    if (batch_fits(b, n)) {
+      b->nblk = n;
+      break;
+    }
   if (hipSetDevice(b->device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
@@ -1293,6 +1365,7 @@ int cavlbm_batch_destroy(cavlbm_batch* b) {
   batch_drop_graphs(b);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_tile_cav) (void)hipFree(b->d_tile_cav);
+  if (b->d_tile_cavn) (void)hipFree(b->d_tile_cavn);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -1358,7 +1431,7 @@ int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_res
     void* fy;
     void* nut;
     size_t pops_bytes, field_bytes;
-    int cur;
+    int cur, fy_par;
     uint64_t t;
   };
   std::vector<Snap> snap;
@@ -1375,6 +1448,7 @@ int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_res
     z.pops_bytes = 18 * P * s->st_bytes + (s->storage == ST_FP16_OFFSET ? P * 4 : 0);
     z.field_bytes = P * s->r_bytes;
     z.cur = s->cur;
+    z.fy_par = s->fy_par;
     z.t = s->time_step;
     bool ok = hipMalloc(&z.pops, z.pops_bytes) == hipSuccess &&
               hipMalloc(&z.fy, z.field_bytes) == hipSuccess &&
@@ -1399,6 +1473,7 @@ int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_res
       cavlbm_solver* s = b->cav[c];
       const Snap& z = snap[c];
       s->cur = z.cur;
+      if (s->fy_par != z.fy_par) swap_fy(s);
       s->time_step = z.t;
       s->c_valid = true;
       s->m_valid = false;
@@ -1412,45 +1487,61 @@ int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_res
     ok &= hipStreamSynchronize(b->st) == hipSuccess;
     return ok;
   };
+  // candidates: every single-step tile shape, then every N-step block the
+  // batch fits, each with and without the XCD remap
+  struct Cand {
+    int shape, nblk, xcd;
+  };
+  std::vector<Cand> cands;
+  for (int shape = 0; shape < kNumBatchShapes; ++shape)
+    for (int xcd = 1; xcd >= 0; --xcd) cands.push_back({shape, 1, xcd});
+  for (int n = 2; n <= kMaxBlockSteps; ++n)
+    if ((b->storage == ST_FP64 || b->storage == ST_FP32) && batch_fits(b, n))
+      for (int xcd = 1; xcd >= 0; --xcd) cands.push_back({b->shape, n, xcd});
   double best = 1e300, first = 0.0;
-  int best_shape = 0, best_xcd = 1, count = 0;
-  for (int shape = 0; shape < kNumBatchShapes; ++shape) {
-    for (int xcd = 1; xcd >= 0; --xcd) {
-      b->shape = shape;
-      b->xcd = xcd;
-      if (batch_layout(b) || !restore()) {
-        free_snap();
-        return CAVLBM_E_HIP;
-      }
-      // one untimed pass (graph capture, caches), then the timed trial
-      double ms = 0.0;
-      if (batch_run_steps(b, trial_steps) || hipStreamSynchronize(b->st) != hipSuccess ||
-          !restore() || cavlbm_batch_time_run(b, trial_steps, &ms)) {
-        free_snap();
-        return CAVLBM_E_HIP;
-      }
-      const double per = ms / (double)trial_steps;
-      if (count == 0) first = per;
-      if (per < best) {
-        best = per;
-        best_shape = shape;
-        best_xcd = xcd;
-      }
-      ++count;
+  Cand best_c = cands[0];
+  int count = 0;
+  for (const Cand& cd : cands) {
+    b->shape = cd.shape;
+    b->nblk = cd.nblk;
+    b->xcd = cd.xcd;
+    if (batch_layout(b) || !restore()) {
+      free_snap();
+      return CAVLBM_E_HIP;
     }
+    // one untimed pass (graph capture, caches), then the timed trial
+    double ms = 0.0;
+    if (batch_run_steps(b, trial_steps) || hipStreamSynchronize(b->st) != hipSuccess ||
+        !restore() || cavlbm_batch_time_run(b, trial_steps, &ms)) {
+      free_snap();
+      return CAVLBM_E_HIP;
+    }
+    const double per = ms / (double)trial_steps;
+    if (count == 0) first = per;
+    if (per < best) {
+      best = per;
+      best_c = cd;
+    }
+    ++count;
   }
-  b->shape = best_shape;
-  b->xcd = best_xcd;
+  b->shape = best_c.shape;
+  b->nblk = best_c.nblk;
+  b->xcd = best_c.xcd;
   const int lrc = batch_layout(b);
   const bool rok = restore();
   free_snap();
   if (lrc || !rok) return bfail(b, CAVLBM_E_HIP, "autotune restore failed");
-  out->tile_x = kBatchShapes[best_shape].tx;
-  out->tile_y = kBatchShapes[best_shape].ty;
-  out->xcd_remap = best_xcd;
+  if (b->nblk >= 2) {
+    stepn_tile_dims(b->storage, b->nblk, &out->tile_x, &out->tile_y);
+  } else {
+    out->tile_x = kBatchShapes[b->shape].tx;
+    out->tile_y = kBatchShapes[b->shape].ty;
+  }
+  out->xcd_remap = b->xcd;
   out->candidates = count;
   out->best_ms_per_step = best;
   out->first_ms_per_step = first;
+  out->steps_per_launch = b->nblk;
   return 0;
 }
 

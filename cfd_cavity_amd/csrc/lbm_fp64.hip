@@ -32,14 +32,12 @@ hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& c
   return hipErrorInvalidValue;
 }
 
-bool stepn_grid(int storage, int n, Geom& geo_n) {
-  bool ok = false;
-  int tx = 1, ty = 1;
-#define CAVLBM_TILE(S, N)                        \
-  if (storage == S && n == N) {                  \
-    ok = stepn_fits<S, N>(geo_n.nx, geo_n.ny);   \
-    tx = TileN<S, N>::kTX;                       \
-    ty = TileN<S, N>::kTY;                       \
+bool stepn_tile_dims(int storage, int n, int* tx, int* ty) {
+#define CAVLBM_TILE(S, N)        \
+  if (storage == S && n == N) {  \
+    *tx = TileN<S, N>::kTX;      \
+    *ty = TileN<S, N>::kTY;      \
+    return true;                 \
   }
   CAVLBM_TILE(ST_FP64, 2)
   CAVLBM_TILE(ST_FP64, 3)
@@ -48,9 +46,16 @@ bool stepn_grid(int storage, int n, Geom& geo_n) {
   CAVLBM_TILE(ST_FP32, 3)
   CAVLBM_TILE(ST_FP32, 4)
 #undef CAVLBM_TILE
+  *tx = *ty = 0;
+  return false;
+}
+
+bool stepn_grid(int storage, int n, Geom& geo_n) {
+  int tx = 0, ty = 0;
+  if (!stepn_tile_dims(storage, n, &tx, &ty)) return false;
   geo_n.nbx = (geo_n.nx + tx - 1) / tx;
   geo_n.nby = (geo_n.ny + ty - 1) / ty;
-  return ok;
+  return geo_n.nx >= 3 && geo_n.ny >= 3 && (geo_n.ny - 1) % ty != 0;
 }
 
 template <int GEQ, int N>

@@ -29,9 +29,12 @@ hipError_t launch_stepn_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& 
 hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st);
-// stepn_grid: whether the lattice fits the N-step tiling of the storage type,
-// and that tiling's grid (nbx, nby) written into geo_n.
+// stepn_grid: whether the lattice fits the N-step tiling of the storage type
+// (the condition of stepn_fits), and that tiling's grid (nbx, nby) written
+// into geo_n; stepn_tile_dims: the output tile of that tiling (false when the
+// storage has no N-step form).
 bool stepn_grid(int storage, int n, Geom& geo_n);
+bool stepn_tile_dims(int storage, int n, int* tx, int* ty);
 
 // f = feq / g = geq from the M-form fields, then u = 0 on the walls.
 hipError_t launch_init_eq(const Geom& geo, int geq, double* mpops, const double* rho,
@@ -109,13 +112,16 @@ hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, hipStream_t 
 // Batched step: many cavities advanced by one launch.  One descriptor per
 // cavity (device memory); `tile_cav` maps every global tile to its cavity.
 struct CavDesc {
-  Geom geo;          // nbx / nby for the tile shape in use
+  Geom geo;          // nbx / nby for the single-step tile shape in use
+  Geom geon;         // nbx / nby for the N-step tiles of the batch's N
   StepCtx ctx;       // scalars after spin-up (beta = beta_eff)
   double beta_eff;   // beta during spin-up: beta_eff * (step / spin_up)
   int spin_up;
-  int tile_begin;    // first global tile of this cavity
+  int tile_begin;    // first global tile of this cavity (single-step table)
+  int tile_begin_n;  // first global tile of this cavity (N-step table)
   Walls walls;
-  StepBufs bufs[2];  // by buffer parity (src/dst exchanged)
+  StepBufs bufs[2];  // by buffer parity (src/dst exchanged); fy comes from fyp
+  void* fyp[2];      // the two fy_node planes, by fy parity
 };
 
 // Tile shapes the batched kernel is instantiated for (autotuner candidates).
@@ -125,12 +131,21 @@ struct TileShape {
 constexpr int kNumBatchShapes = 4;
 constexpr TileShape kBatchShapes[kNumBatchShapes] = {{64, 4}, {64, 2}, {64, 8}, {128, 4}};
 
+// parity: the C-form buffer read; fpar: the fy plane read (and, for N-step
+// launches, fpar ^ 1 the one written).
 hipError_t launch_step_batch_fp64(int geq, int shape, const CavDesc* descs, const int* tile_cav,
-                                  int total_tiles, int parity, int step, int write_aux,
+                                  int total_tiles, int parity, int fpar, int step, int write_aux,
                                   int xcd_remap, hipStream_t st);
 hipError_t launch_step_batch_lowp(int storage, int geq, int shape, const CavDesc* descs,
-                                  const int* tile_cav, int total_tiles, int parity, int step,
-                                  int write_aux, int xcd_remap, hipStream_t st);
+                                  const int* tile_cav, int total_tiles, int parity, int fpar,
+                                  int step, int write_aux, int xcd_remap, hipStream_t st);
+// N steps of every cavity per launch (lbm_stepn.hpp tiles, N-step table).
+hipError_t launch_stepn_batch_fp64(int geq, int n, const CavDesc* descs, const int* tile_cav,
+                                   int total_tiles, int parity, int fpar, int step, int write_aux,
+                                   int xcd_remap, hipStream_t st);
+hipError_t launch_stepn_batch_lowp(int storage, int geq, int n, const CavDesc* descs,
+                                   const int* tile_cav, int total_tiles, int parity, int fpar,
+                                   int step, int write_aux, int xcd_remap, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Adjoint (reverse mode) of the FP64 fused step, see lbm_adjoint.hip.

@@ -267,11 +267,13 @@ int cavlbm_tangent_nu_window(cavlbm_solver *s, const double *theta_hot,
 typedef struct cavlbm_batch cavlbm_batch;
 
 typedef struct {
-  int32_t tile_x, tile_y;   /* chosen workgroup tile (nodes) */
+  int32_t tile_x, tile_y;   /* chosen output tile (nodes) */
   int32_t xcd_remap;        /* 1: tiles remapped so each XCD owns contiguous rows */
   int32_t candidates;       /* configurations timed */
   double best_ms_per_step;  /* of the chosen configuration */
   double first_ms_per_step; /* of the first candidate (64x4, remapped) */
+  int32_t steps_per_launch; /* chosen steps per launch (1: single-step tiles) */
+  int32_t reserved;
 } cavlbm_tune_result;
 
 /* n parameter sets (same storage, geq_order and device; nu_start_mult 1). */
@@ -284,8 +286,12 @@ int cavlbm_batch_synchronize(cavlbm_batch *b);
 /* Borrowed handle of cavity i for reads, diagnostics and setters (valid
  * until cavlbm_batch_destroy; do not destroy it; do not step it alone). */
 int cavlbm_batch_solver(cavlbm_batch *b, uint64_t i, cavlbm_solver **out);
-/* Times every tile shape x block-order candidate for trial_steps steps and
- * keeps the fastest; the cavities' state is restored afterwards.  The batch
+/* Times every single-step tile shape and every N-step block (N = 2..4, FP64
+ * and FP32 storage, when every cavity fits its tiling), each with and
+ * without the XCD remap, for trial_steps steps and keeps the fastest; the
+ * cavities' state is restored afterwards.  A new batch advances the
+ * storage's default steps per launch (4 FP64, 3 FP32) when every cavity
+ * fits, single steps otherwise.  The batch
  * must have been run at least one step (CAVLBM_E_PARAM otherwise); tune after
  * the spin-up so that the trials time the graph-replayed steady steps. */
 int cavlbm_batch_autotune(cavlbm_batch *b, uint64_t trial_steps,

@@ -1,6 +1,9 @@
 """Batched form: every cavity of a batch must equal, bit for bit, the same
 cavity stepped alone (same per-node code), across graph replays and after
-the autotuner (which must leave the state unchanged)."""
+the autotuner (which must leave the state unchanged).  A batch advances
+several steps per launch (the N-step tiles) like a single cavity does; the
+comparisons run each side with its own choice, so they also check the
+batched N-step kernel against the single-cavity kernels."""
 import numpy as np
 import pytest
 
@@ -36,7 +39,7 @@ def same_state(a, b):
 def test_batch_equals_single_cavities(storage):
     batch = CavityBatch(SPECS, storage=storage)
     singles = [single(sp, storage) for sp in SPECS]
-    for chunk in (1, 150, 300):   # spin-up direct launches, then graph replays
+    for chunk in (1, 150, 300, 7, 3):   # spin-up direct launches, graph replays, remainders
         batch.run(chunk)
         for s in singles:
             s.run(chunk)
@@ -53,8 +56,12 @@ def test_autotune_keeps_state_and_picks_a_candidate():
     for s in ref:
         s.run(250)
     res = batch.autotune(trial_steps=130)
-    assert res["candidates"] == 8
-    assert (res["tile_x"], res["tile_y"]) in {(64, 4), (64, 2), (64, 8), (128, 4)}
+    assert res["candidates"] == 8 + 2 * 3   # 4 shapes x 2 orders, N = 2, 3, 4 x 2 orders
+    if res["steps_per_launch"] == 1:
+        assert (res["tile_x"], res["tile_y"]) in {(64, 4), (64, 2), (64, 8), (128, 4)}
+    else:
+        assert (res["steps_per_launch"], res["tile_x"], res["tile_y"]) in {
+            (2, 64, 8), (3, 32, 16), (4, 16, 16)}
     assert res["best_ms_per_step"] <= res["first_ms_per_step"]
     for i, (c, s) in enumerate(zip(batch, ref)):
         assert c.time_step == s.time_step
@@ -66,3 +73,19 @@ def test_autotune_keeps_state_and_picks_a_candidate():
         assert same_state(c, s) is None, f"cavity {i} after tuned run"
     d = batch[1].diagnostics()
     assert d["all_finite"] and d["time_step"] == 450
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+def test_batch_with_a_height_outside_the_tiling(storage):
+    """ny = 49 fits no N-step tiling (48 = 0 mod 8, 12, 16): the whole batch
+    steps one step per launch and still equals the single cavities."""
+    specs = SPECS[:2] + [dict(nx=20, ny=49, ra=1e5, cs=0.1)]
+    batch = CavityBatch(specs, storage=storage)
+    singles = [single(sp, storage) for sp in specs]
+    for chunk in (1, 130, 5):
+        batch.run(chunk)
+        for s in singles:
+            s.run(chunk)
+        for i, (c, s) in enumerate(zip(batch, singles)):
+            bad = same_state(c, s)
+            assert bad is None, f"cavity {i} differs in {bad} after {c.time_step} steps"
