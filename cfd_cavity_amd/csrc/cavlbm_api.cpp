@@ -134,12 +134,21 @@ StepBufs bufs(cavlbm_solver* s) {
   b.temp = s->fld[3];
   b.nut_pre = nullptr;
   b.at_pre = nullptr;
+  b.dc = DynCoef{};
   if (s->dyn) {
     const size_t P = (size_t)s->geo.plane;
     b.nut_pre = s->dyn + 18 * P;
     b.at_pre = s->dyn + 19 * P;
   }
   return b;
+}
+
+hipError_t launch_b(cavlbm_solver* s, int mode, const StepCtx& ctx, const StepBufs& b,
+                    int write_aux) {
+  const Walls w{s->theta};
+  if (s->storage == ST_FP64)
+    return launch_step_fp64(s->geq, mode, s->geo, ctx, w, b, write_aux, s->st);
+  return launch_step_lowp(s->storage, s->geq, mode, s->geo, ctx, w, b, write_aux, s->st);
 }
 
 hipError_t launch(cavlbm_solver* s, int mode, const StepCtx& ctx, int write_aux) {
@@ -246,16 +255,42 @@ DynSgsArgs dyn_args(const cavlbm_solver* s, const StepCtx& ctx) {
   return a;
 }
 
+// The launches of one dynamic step from the C-form: the post-stream f planes
+// and fields the model reads (MODE_MATERIALIZE_F, skipped when the M-form is
+// current), the model's filter and terms, and the collision, which pulls the
+// C-form again, runs the coefficient stage per node and stores its planes
+// with the auxiliary outputs (MODE_FUSED_DYN).
+hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgsArgs& args,
+                              int write_aux) {
+  hipError_t e = hipSuccess;
+  if (!s->m_valid) e = launch(s, MODE_MATERIALIZE_F, ctx, 0);
+  if (e == hipSuccess) e = launch_dynamic_sgs(s->geo, args, false, s->st);
+  if (e != hipSuccess) return e;
+  StepBufs b = bufs(s);
+  b.dc = dyn_coef_of(args);
+  return launch_b(s, MODE_FUSED_DYN, ctx, b, write_aux);
+}
+
 // A step with the per-node SGS values (solver.rs:279-293 with
-// compute_dynamic_sgs first): the model reads the post-stream state, so the
-// M-form is made current, the model runs when enabled, and the collision
-// takes nu_t / alpha_t from it.
+// compute_dynamic_sgs first): the model reads the post-stream state and the
+// collision takes nu_t / alpha_t from it.  From the C-form the fused launches
+// above; from the M-form (after init or setters), or with the model off and
+// its last values kept, the model's three passes and the collision from the
+// M-form (MODE_FIRST_PRE).
 int dyn_step(cavlbm_solver* s, int write_aux) {
+  const StepCtx ctx = make_ctx(s, s->time_step);
+  if (s->dyn_on && s->c_valid) {
+    HIP_TRY(s, dyn_fused_launches(s, ctx, dyn_args(s, ctx), write_aux));
+    s->dyn_pre = true;
+    s->cur ^= 1;
+    s->m_valid = false;
+    s->time_step += 1;
+    return 0;
+  }
   int rc = materialize(s);
   if (rc) return rc;
-  const StepCtx ctx = make_ctx(s, s->time_step);
   if (s->dyn_on) {
-    HIP_TRY(s, launch_dynamic_sgs(s->geo, dyn_args(s, ctx), s->st));
+    HIP_TRY(s, launch_dynamic_sgs(s->geo, dyn_args(s, ctx), true, s->st));
     s->dyn_pre = true;
   }
   HIP_TRY(s, launch(s, MODE_FIRST_PRE, ctx, write_aux));
@@ -333,8 +368,8 @@ hipGraphExec_t capture(cavlbm_solver* s) {
   return ex;
 }
 
-// Captures kGraphSteps dynamic-SGS steps (materialize, the model's three
-// passes, the collision with per-node values) starting from parity `par`.
+// Captures kGraphSteps dynamic-SGS steps (dyn_fused_launches each) starting
+// from parity `par`.
 hipGraphExec_t capture_dyn(cavlbm_solver* s, int par) {
   const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);
   const DynSgsArgs args = dyn_args(s, ctx);
@@ -344,13 +379,14 @@ hipGraphExec_t capture_dyn(cavlbm_solver* s, int par) {
   if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   s->cur = par;
   bool ok = true;
+  const bool m_saved = s->m_valid;
   for (int i = 0; i < kGraphSteps && ok; ++i) {
-    ok = launch(s, MODE_MATERIALIZE, ctx, 0) == hipSuccess &&
-         launch_dynamic_sgs(s->geo, args, s->st) == hipSuccess &&
-         launch(s, MODE_FIRST_PRE, ctx, 0) == hipSuccess;
+    s->m_valid = false;  // every captured step materializes its f planes
+    ok = dyn_fused_launches(s, ctx, args, 0) == hipSuccess;
     s->cur ^= 1;
   }
   s->cur = saved;
+  s->m_valid = m_saved;
   if (hipStreamEndCapture(s->st, &g) != hipSuccess || !ok) {
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();

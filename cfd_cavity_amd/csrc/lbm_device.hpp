@@ -48,7 +48,9 @@ enum Mode {
   MODE_FIRST = 1,        // M-form pops+fields -> collide -> C-form
   MODE_MATERIALIZE = 2,  // C-form -> stream+BC+moments+walls -> M-form
   MODE_COLLIDE_M = 3,    // M-form pops+fields -> collide -> M-form pops (in place)
-  MODE_FIRST_PRE = 4     // MODE_FIRST with per-node nu_t / alpha_t given (dynamic SGS)
+  MODE_FIRST_PRE = 4,    // MODE_FIRST with per-node nu_t / alpha_t given (dynamic SGS)
+  MODE_MATERIALIZE_F = 5,  // MODE_MATERIALIZE writing the f planes and fields only
+  MODE_FUSED_DYN = 6     // MODE_FUSED with nu_t / alpha_t from the model's coefficient stage
 };
 
 struct Geom {
@@ -68,6 +70,88 @@ struct Walls {
   const double* theta_hot;  // ny values or nullptr (constant kTHot)
 };
 
+// ---------------------------------------------------------------------------
+// The last stage of the dynamic mixed-scale model as a per-node function: box
+// filter of the four Lilly contractions, the regularised coefficients and the
+// floored nu_t / alpha_t (solver.rs:787-815, box_filter :1204-1224,
+// lilly_coefficient :1251-1261).  Used by the model's coefficient pass and,
+// fused, by the collision of a dynamic step.  FP64 in the reference's order.
+constexpr double kCDynMax = 0.04;  // clamp of the Lilly coefficients
+
+// Inputs and outputs of the coefficient stage; planes of the solver's
+// pitched layout.
+struct DynCoef {
+  const double* terms;      // 4 planes: L:M, M:M, Lj Mj, Mj Mj
+  const double* nu_mix_g;   // grid-level mixed forms
+  const double* kap_mix_g;
+  double* cnu;              // out: C_nu, C_kappa, nu_t, alpha_t per node
+  double* ck;
+  double* nut_pre;
+  double* at_pre;
+  int width, half_left;     // box filter: width nodes, half_left = (width-1)/2
+  double nu_t_min, alpha_t_min;
+};
+
+// Window [i - half_left, i + width - half_left) clamped to [0, n).
+struct DynWin {
+  int x0, x1, y0, y1;
+  double cnt;
+};
+
+__device__ __forceinline__ DynWin dyn_window(const Geom& geo, int width, int half_left, int ix,
+                                             int iy) {
+  DynWin w;
+  w.x0 = ix >= half_left ? ix - half_left : 0;
+  w.x1 = min(ix + width - half_left, geo.nx);
+  w.y0 = iy >= half_left ? iy - half_left : 0;
+  w.y1 = min(iy + width - half_left, geo.ny);
+  w.cnt = (double)((w.y1 - w.y0) * (w.x1 - w.x0));
+  return w;
+}
+
+// Box mean in the reference's (y outer, x inner) order from +0.
+__device__ __forceinline__ double dyn_box(const Geom& geo, const DynWin& w, const double* f) {
+  double s = 0.0;
+  for (int yy = w.y0; yy < w.y1; ++yy)
+    for (int xx = w.x0; xx < w.x1; ++xx) s += f[yy * geo.pitch + xx];
+  return s / w.cnt;
+}
+
+// lilly_coefficient (solver.rs:1251-1261).
+__device__ __forceinline__ double dyn_lilly(double num, double den) {
+  if (!isfinite(den) || !isfinite(num) || den <= 0.0) return 0.0;
+  const double c = 0.5 * num / den;
+  if (isfinite(c)) return c < 0.0 ? 0.0 : (c > kCDynMax ? kCDynMax : c);
+  return 0.0;
+}
+
+// nu_t and alpha_t of node (ix, iy) (solver.rs:787-815); with `store` the
+// coefficients and the floored values are also written to their planes.
+__device__ __forceinline__ void dyn_coef_node(const Geom& geo, const DynCoef& d, int ix, int iy,
+                                              bool store, double& nut, double& at) {
+  const int k = iy * geo.pitch + ix;
+  const long long P = geo.plane;
+  const DynWin w = dyn_window(geo, d.width, d.half_left, ix, iy);
+  // regularised Lilly coefficients (:787-796), one sum per contraction
+  const double num_v = dyn_box(geo, w, d.terms + 0 * P);
+  const double den_v = dyn_box(geo, w, d.terms + 1 * P);
+  const double num_k = dyn_box(geo, w, d.terms + 2 * P);
+  const double den_k = dyn_box(geo, w, d.terms + 3 * P);
+  const double cnu = dyn_lilly(num_v, den_v);
+  const double ck = dyn_lilly(num_k, den_k);
+  // floored turbulent viscosity and diffusivity (:810-815)
+  const double nt = cnu * d.nu_mix_g[k];
+  nut = nt < d.nu_t_min ? d.nu_t_min : nt;
+  const double a = ck * d.kap_mix_g[k];
+  at = a < d.alpha_t_min ? d.alpha_t_min : a;
+  if (store) {
+    d.cnu[k] = cnu;
+    d.ck[k] = ck;
+    d.nut_pre[k] = nut;
+    d.at_pre[k] = at;
+  }
+}
+
 struct StepBufs {
   const void* src;  // C-form population set (storage type)
   void* dst;        // C-form population set (storage type)
@@ -80,6 +164,7 @@ struct StepBufs {
   double* temp;
   const double* nut_pre;  // MODE_FIRST_PRE: nu_t per node (FP64), else unused
   const double* at_pre;   // MODE_FIRST_PRE: alpha_t per node (FP64), else unused
+  DynCoef dc;             // MODE_FUSED_DYN: the model's coefficient stage
 };
 
 constexpr int kTileX = 64;
@@ -449,15 +534,16 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   R f[9], g[9];
   R rho, ux, uy, temp;
 
-  if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE) {
+  if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE || MODE == MODE_MATERIALIZE_F ||
+      MODE == MODE_FUSED_DYN) {
     const R fyp = static_cast<const R*>(b.fy)[k];
     pull_node<S, GEQ>(geo, walls, static_cast<const T*>(b.src), fyp, ix, iy, f, g, rho, ux, uy,
                       temp);
-    if (MODE == MODE_MATERIALIZE) {
+    if (MODE == MODE_MATERIALIZE || MODE == MODE_MATERIALIZE_F) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         b.mpops[i * P + k] = (double)f[i];
-        b.mpops[(9 + i) * P + k] = (double)g[i];
+        if (MODE == MODE_MATERIALIZE) b.mpops[(9 + i) * P + k] = (double)g[i];
       }
       b.rho[k] = (double)rho;
       b.ux[k] = (double)ux;
@@ -478,10 +564,18 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   }
 
   R fy_new, nut_new;
-  if (MODE == MODE_FIRST_PRE)
+  if (MODE == MODE_FIRST_PRE) {
     collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new,
                                R(b.nut_pre[k]), R(b.at_pre[k]));
-  else
+  } else if (MODE == MODE_FUSED_DYN) {
+    // the coefficient stage for this node; its planes are stored with the
+    // auxiliary outputs (the values the solver keeps for its getters and for
+    // later steps without the model)
+    double nt, at;
+    dyn_coef_node(geo, b.dc, ix, iy, write_aux != 0, nt, at);
+    collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new, R(nt),
+                               R(at));
+  } else
     collide_node<R, GEQ>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new);
   static_cast<R*>(b.fy)[k] = fy_new;
   if (write_aux) static_cast<R*>(b.nut)[k] = nut_new;

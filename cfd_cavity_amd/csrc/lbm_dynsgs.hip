@@ -16,7 +16,8 @@
 //                  contractions;
 //   k_dyn_coeffs : box of the contractions, C = clamp(num/(2 den), 0, 0.04),
 //                  nu_t = C_nu * nu_mix, alpha_t = C_kappa * kappa_mix, both
-//                  floored.
+//                  floored (dyn_coef_node; the solver's dynamic step runs it
+//                  inside the collision instead, MODE_FUSED_DYN).
 // The box sums run in the reference's (y outer, x inner) order from +0.
 //
 // The mixed forms a^alpha b^(1-alpha) use pow; the reference's powf is the C
@@ -36,27 +37,8 @@ __device__ __forceinline__ int node_of_thread(const Geom& geo, int& ix, int& iy)
   return ix < geo.nx && iy < geo.ny;
 }
 
-// Window [i - half_left, i + width - half_left) clamped to [0, n).
-struct Win {
-  int x0, x1, y0, y1;
-  double cnt;
-};
-
-__device__ __forceinline__ Win window(const Geom& geo, const DynSgsArgs& a, int ix, int iy) {
-  Win w;
-  w.x0 = ix >= a.half_left ? ix - a.half_left : 0;
-  w.x1 = min(ix + a.width - a.half_left, geo.nx);
-  w.y0 = iy >= a.half_left ? iy - a.half_left : 0;
-  w.y1 = min(iy + a.width - a.half_left, geo.ny);
-  w.cnt = (double)((w.y1 - w.y0) * (w.x1 - w.x0));
-  return w;
-}
-
-__device__ __forceinline__ double box(const Geom& geo, const Win& w, const double* f) {
-  double s = 0.0;
-  for (int yy = w.y0; yy < w.y1; ++yy)
-    for (int xx = w.x0; xx < w.x1; ++xx) s += f[yy * geo.pitch + xx];
-  return s / w.cnt;
+__device__ __forceinline__ DynWin window(const Geom& geo, const DynSgsArgs& a, int ix, int iy) {
+  return dyn_window(geo, a.width, a.half_left, ix, iy);
 }
 
 // Centred differences with one-sided clamping at the walls (solver.rs:1229-1242).
@@ -78,19 +60,11 @@ __device__ __forceinline__ double mixpow(double x, double e) {
   return pow(x, e);
 }
 
-// lilly_coefficient (solver.rs:1251-1261).
-__device__ __forceinline__ double lilly(double num, double den) {
-  if (!isfinite(den) || !isfinite(num) || den <= 0.0) return 0.0;
-  const double c = 0.5 * num / den;
-  if (isfinite(c)) return c < 0.0 ? 0.0 : (c > kCDynMax ? kCDynMax : c);
-  return 0.0;
-}
-
 __global__ __launch_bounds__(kBlock) void k_dyn_filter(Geom geo, DynSgsArgs a) {
   int ix, iy;
   if (!node_of_thread(geo, ix, iy)) return;
   const int k = iy * geo.pitch + ix;
-  const Win w = window(geo, a, ix, iy);
+  const DynWin w = window(geo, a, ix, iy);
   double s_u = 0.0, s_v = 0.0, s_t = 0.0, s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0,
          s_vt = 0.0;
   for (int yy = w.y0; yy < w.y1; ++yy)
@@ -168,9 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
   const double tmag_bar = sqrt(2.0 * (tb_x * tb_x + tb_y * tb_y));
 
   // test-filtered similarity scales (:713-714)
-  const Win w = window(geo, a, ix, iy);
-  const double qbar_h = box(geo, w, a.qsm);
-  const double phibar_h = box(geo, w, a.phi);
+  const DynWin w = window(geo, a, ix, iy);
+  const double qbar_h = dyn_box(geo, w, a.qsm);
+  const double phibar_h = dyn_box(geo, w, a.phi);
 
   // interpolated mixed forms at both levels (:733-740)
   const double dt = a.dtest, dx = 1.0, del = kDeltaT;
@@ -210,32 +184,17 @@ __global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
 __global__ __launch_bounds__(kBlock) void k_dyn_coeffs(Geom geo, DynSgsArgs a) {
   int ix, iy;
   if (!node_of_thread(geo, ix, iy)) return;
-  const int k = iy * geo.pitch + ix;
-  const long long P = geo.plane;
-  const Win w = window(geo, a, ix, iy);
-  // regularised Lilly coefficients (:787-796), one sum per contraction
-  const double num_v = box(geo, w, a.terms + 0 * P);
-  const double den_v = box(geo, w, a.terms + 1 * P);
-  const double num_k = box(geo, w, a.terms + 2 * P);
-  const double den_k = box(geo, w, a.terms + 3 * P);
-  const double cnu = lilly(num_v, den_v);
-  const double ck = lilly(num_k, den_k);
-  a.cnu[k] = cnu;
-  a.ck[k] = ck;
-  // floored turbulent viscosity and diffusivity (:810-815)
-  const double nt = cnu * a.nu_mix_g[k];
-  a.nut_pre[k] = nt < a.nu_t_min ? a.nu_t_min : nt;
-  const double at = ck * a.kap_mix_g[k];
-  a.at_pre[k] = at < a.alpha_t_min ? a.alpha_t_min : at;
+  double nut, at;
+  dyn_coef_node(geo, dyn_coef_of(a), ix, iy, true, nut, at);
 }
 
 }  // namespace
 
-hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, hipStream_t st) {
+hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs, hipStream_t st) {
   const dim3 grid(nblocks_of(geo)), block(kBlock);
   hipLaunchKernelGGL(k_dyn_filter, grid, block, 0, st, geo, a);
   hipLaunchKernelGGL(k_dyn_terms, grid, block, 0, st, geo, a);
-  hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, geo, a);
+  if (coeffs) hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, geo, a);
   return hipGetLastError();
 }
 

@@ -77,7 +77,6 @@ inline int nblocks_of(const Geom& g) { return g.nbx * g.nby; }
 // ---------------------------------------------------------------------------
 // Dynamic mixed-scale SGS (lbm_dynsgs.hip).  All arrays are FP64 pitched
 // planes of the solver's geometry; the inputs are the M-form state.
-constexpr double kCDynMax = 0.04;          // clamp of the Lilly coefficients
 constexpr double kTauTrtFloor = 0.505;     // TRT stability floor of tau+
 constexpr double kDeltaT = kTHot - kTCold; // global temperature difference
 struct DynSgsArgs {
@@ -106,7 +105,25 @@ struct DynSgsArgs {
   double* at_pre;   // out: alpha_t per node
 };
 constexpr int kDynPlanes = 20;  // planes of scratch + outputs above
-hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, hipStream_t st);
+// The model's passes: the filter and the terms, then (coeffs) the
+// coefficient pass; without it the collision runs that stage itself
+// (MODE_FUSED_DYN with the DynCoef of dyn_coef_of).
+hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs, hipStream_t st);
+__host__ __device__ inline DynCoef dyn_coef_of(const DynSgsArgs& a) {
+  DynCoef d;
+  d.terms = a.terms;
+  d.nu_mix_g = a.nu_mix_g;
+  d.kap_mix_g = a.kap_mix_g;
+  d.cnu = a.cnu;
+  d.ck = a.ck;
+  d.nut_pre = a.nut_pre;
+  d.at_pre = a.at_pre;
+  d.width = a.width;
+  d.half_left = a.half_left;
+  d.nu_t_min = a.nu_t_min;
+  d.alpha_t_min = a.alpha_t_min;
+  return d;
+}
 
 // ---------------------------------------------------------------------------
 // Batched step: many cavities advanced by one launch.  One descriptor per

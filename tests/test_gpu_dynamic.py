@@ -190,7 +190,7 @@ def test_unsupported_combinations():
 
 
 def test_graph_replayed_dynamic_steps_equal_single_steps():
-    """After the spin-up, runs of dynamic steps replay captured graphs (64
+    """After the spin-up, runs of dynamic steps replay captured graphs (48
     steps each); they must equal the same steps issued one by one."""
     runs = []
     for single in (False, True):
@@ -206,4 +206,29 @@ def test_graph_replayed_dynamic_steps_equal_single_steps():
     a, b = runs
     for w in ("f", "g"):
         assert bits_equal(a.get_pops(w), b.get_pops(w))
+    assert bits_equal(a.nu_t_pre, b.nu_t_pre) and a.dynamic_stats() == b.dynamic_stats()
+
+
+def test_reads_between_dynamic_steps_do_not_perturb():
+    """A dynamic step from the C-form materializes only the f planes and
+    fields, and the collision runs the coefficient stage itself; a read in
+    between (which materializes the whole M-form, so the next step skips its
+    own materialization) must not change the trajectory, the per-node values
+    or the statistics."""
+    runs = []
+    for reads in (False, True):
+        s = CavitySolver(*args_of(dict(SMALL, nx=32, ny=128, ra=1e6)))
+        s.run(120)
+        s.set_dynamic_sgs(True)
+        for chunk in (1, 2, 5, 60, 3):
+            s.run(chunk)
+            if reads:
+                s.get_field("temperature")
+                s.get_pops("g")
+        runs.append(s)
+    a, b = runs
+    for name in FIELDS:
+        assert bits_equal(a.get_field(name), b.get_field(name)), name
+    for w in ("f", "g"):
+        assert bits_equal(a.get_pops(w), b.get_pops(w)), w
     assert bits_equal(a.nu_t_pre, b.nu_t_pre) and a.dynamic_stats() == b.dynamic_stats()
