@@ -137,7 +137,14 @@ struct cavlbm_octree {
   uint64_t spin_up_steps = 0, time_step = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t graph = nullptr;
+  // The two M-form buffers of each level (the tiled sub-step kernel reads one
+  // and writes the other); bit gi of a key: level gi's m is mbuf[gi][1].
+  double* mbuf[5][2] = {};
+  int tile_ty[5][3] = {};  // output rows of the tiled kernel, by level and sub-steps (0: none)
+  bool tiled = true;       // use the tiled kernel where a level fits it
+  // coarse-step graphs by the key at their start, and the key they end on
+  hipGraphExec_t graph[32] = {};
+  int graph_next[32] = {};
   bool graph_failed = false;
   std::string err;
 };
@@ -160,7 +167,8 @@ int ofail(cavlbm_octree* o, int code, const std::string& m) {
   } while (0)
 
 void free_octree(cavlbm_octree* o) {
-  if (o->graph) (void)hipGraphExecDestroy(o->graph);
+  for (auto& g : o->graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* p : o->dev)
     if (p) (void)hipFree(p);
   if (o->ev0) (void)hipEventDestroy(o->ev0);
@@ -226,9 +234,43 @@ OctCollide collide_params(double beta, double nu, double alpha, const Relax& r, 
   return p;
 }
 
+// Which buffer holds each level's M-form, as a key (bit gi set: mbuf[gi][1]).
+int mkey(const cavlbm_octree* o) {
+  int key = 0;
+  for (int gi = 0; gi < o->ngrids; ++gi)
+    if (o->lv[gi].m == o->mbuf[gi][1]) key |= 1 << gi;
+  return key;
+}
+
+void set_mkey(cavlbm_octree* o, int key) {
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const int b = (key >> gi) & 1;
+    o->lv[gi].m = o->mbuf[gi][b];
+    o->lv[gi].m2 = o->mbuf[gi][b ^ 1];
+  }
+}
+
+// ns whole sub-steps of `count` levels in one tiled launch when every level
+// fits the tiling (then their M-forms move to the other buffer); false:
+// nothing enqueued.
+bool tiled_substeps(cavlbm_octree* o, int g0, int count, int ns, const OctCollide& c,
+                    hipError_t& e) {
+  const int g1 = g0 + (count > 1 ? 1 : 0);
+  const int ty = o->tile_ty[g0][ns];
+  if (!o->tiled || ty == 0 || o->tile_ty[g1][ns] != ty) return false;
+  OctLevel L[2] = {o->lv[g0], o->lv[g1]};
+  OctStream S[2] = {stream_rules(o, g0), stream_rules(o, g1)};
+  e = launch_oct_tile(L, S, count, ty, ns, c, o->geq, o->st);
+  if (e == hipSuccess)
+    for (int i = 0; i < count; ++i) std::swap(o->lv[g0 + i].m, o->lv[g0 + i].m2);
+  return true;
+}
+
 // One sub-step of `count` levels of the same kind (the two blocks, or the two
 // strips): collide_cavity, then stream + BCs + moments + walls.
 hipError_t substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
+  hipError_t te;
+  if (tiled_substeps(o, g0, count, 1, c, te)) return te;
   OctLevel L[2] = {o->lv[g0], o->lv[g0 + (count > 1 ? 1 : 0)]};
   OctStream S[2] = {stream_rules(o, g0), stream_rules(o, g0 + (count > 1 ? 1 : 0))};
   hipError_t e = launch_oct_collide(L, count, c, o->geq, o->st);
@@ -240,6 +282,8 @@ hipError_t substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
 // the first and collision of the second (C-form -> second C-form), and the
 // stream of the second from that second C-form.
 hipError_t double_substep(cavlbm_octree* o, int g0, int count, const OctCollide& c) {
+  hipError_t te;
+  if (tiled_substeps(o, g0, count, 2, c, te)) return te;
   OctLevel L[2] = {o->lv[g0], o->lv[g0 + (count > 1 ? 1 : 0)]};
   OctStream S[2] = {stream_rules(o, g0), stream_rules(o, g0 + (count > 1 ? 1 : 0))};
   hipError_t e = launch_oct_collide(L, count, c, o->geq, o->st);
@@ -268,8 +312,14 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   const OctCollide c0 = collide_params(beta_c, nu_eff, alpha_eff, tc, false);
   const OctCollide c1 = collide_params(beta_f, 2.0 * nu_eff, 2.0 * alpha_eff, tf, true);
   const OctCollide c2 = collide_params(beta_s, 4.0 * nu_eff, 4.0 * alpha_eff, ts, true);
-  const OctLevel coarse2[2] = {o->lv[0], o->lv[0]};
-  const OctLevel blocks[2] = {o->lv[1], o->lv[2]};
+  // level descriptors are read right before each transfer: a tiled sub-step
+  // moves its levels' M-forms to the other buffer
+  auto coarse2 = [o](OctLevel* L) { L[0] = L[1] = o->lv[0]; };
+  auto pair_of = [o](OctLevel* L, int g0) {
+    L[0] = o->lv[g0];
+    L[1] = o->lv[g0 + 1];
+  };
+  OctLevel lc[2], lf[2];
   hipError_t e;
 #define OCT_Q(x)                   \
   do {                             \
@@ -277,27 +327,39 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
     if (e != hipSuccess) return e; \
   } while (0)
   OCT_Q(substep(o, 0, 1, c0));
-  OCT_Q(launch_oct_coarse_to_fine(coarse2, blocks, &o->gt[1], 2, o->xi_a, o->geq, o->st));
+  coarse2(lc);
+  pair_of(lf, 1);
+  OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[1], 2, o->xi_a, o->geq, o->st));
   if (o->levels == 3) {
-    const OctLevel strips[2] = {o->lv[3], o->lv[4]};
     OCT_Q(substep(o, 1, 2, c1));
-    OCT_Q(launch_oct_coarse_to_fine(blocks, strips, &o->gt[3], 2, o->xi_b, o->geq, o->st));
+    pair_of(lc, 1);
+    pair_of(lf, 3);
+    OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[3], 2, o->xi_b, o->geq, o->st));
     OCT_Q(double_substep(o, 3, 2, c2));
-    OCT_Q(launch_oct_fine_to_coarse(strips, blocks, &o->ct[3], 2, o->xi_b, o->geq, o->st));
+    pair_of(lf, 3);
+    pair_of(lc, 1);
+    OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[3], 2, o->xi_b, o->geq, o->st));
     OCT_Q(substep(o, 1, 2, c1));
   } else {
     OCT_Q(double_substep(o, 1, 2, c1));
   }
-  OCT_Q(launch_oct_fine_to_coarse(blocks, coarse2, &o->ct[1], 2, o->xi_a, o->geq, o->st));
+  pair_of(lf, 1);
+  coarse2(lc);
+  OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[1], 2, o->xi_a, o->geq, o->st));
 #undef OCT_Q
   return hipSuccess;
 }
 
-hipGraphExec_t capture_step(cavlbm_octree* o) {
+// Captures the coarse step from the current buffers; *next: the key its
+// replay ends on (the buffers are left as they were).
+hipGraphExec_t capture_step(cavlbm_octree* o, int* next) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
+  const int key = mkey(o);
   if (hipStreamBeginCapture(o->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   const bool ok = enqueue_step(o, o->time_step) == hipSuccess;
+  *next = mkey(o);
+  set_mkey(o, key);
   if (hipStreamEndCapture(o->st, &g) != hipSuccess || !ok) {
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
@@ -313,11 +375,13 @@ int run_steps(cavlbm_octree* o, uint64_t n) {
   for (uint64_t s = 0; s < n; ++s) {
     // constant per-step scalars without the viscosity ramp: replay one graph
     if (o->nu_start_mult == 1.0 && !o->graph_failed) {
-      if (!o->graph) o->graph = capture_step(o);
-      if (!o->graph) {
+      const int key = mkey(o);
+      if (!o->graph[key]) o->graph[key] = capture_step(o, &o->graph_next[key]);
+      if (!o->graph[key]) {
         o->graph_failed = true;
       } else {
-        OHIP_TRY(o, hipGraphLaunch(o->graph, o->st));
+        OHIP_TRY(o, hipGraphLaunch(o->graph[key], o->st));
+        set_mkey(o, o->graph_next[key]);
         o->time_step += 1;
         continue;
       }
@@ -473,11 +537,23 @@ int cavlbm_octree_create(const cavlbm_octree_params* pin, cavlbm_octree** out) {
     L.ny = (int)dims[gi][1];
     L.plane = (long long)L.nx * L.ny;
     L.m = dalloc<double>(o, (size_t)kOctM * L.plane);
+    L.m2 = dalloc<double>(o, (size_t)kOctM * L.plane);
     L.c = dalloc<double>(o, (size_t)18 * L.plane);
     L.c2 = dalloc<double>(o, (size_t)18 * L.plane);
-    if (!L.m || !L.c || !L.c2) return fail_free("device allocation failed");
-    if (hipMemsetAsync(L.m, 0, (size_t)kOctM * L.plane * sizeof(double), o->st) != hipSuccess)
+    if (!L.m || !L.m2 || !L.c || !L.c2) return fail_free("device allocation failed");
+    if (hipMemsetAsync(L.m, 0, (size_t)kOctM * L.plane * sizeof(double), o->st) != hipSuccess ||
+        hipMemsetAsync(L.m2, 0, (size_t)kOctM * L.plane * sizeof(double), o->st) != hipSuccess)
       return fail_free("device memset failed");
+    o->mbuf[gi][0] = L.m;
+    o->mbuf[gi][1] = L.m2;
+    // tiled sub-steps: the tallest tile of at most 16 rows that fits (more
+    // workgroups for these latency-bound levels, bounded recomputation)
+    for (int ns = 1; ns <= 2; ++ns)
+      for (int ty = 16; ty >= 3; --ty)
+        if (oct_tile_fits(L.nx, L.ny, ty, ns)) {
+          o->tile_ty[gi][ns] = ty;
+          break;
+        }
   }
   for (int gi = 1; gi < o->ngrids; ++gi) {
     const HostGrid& h = o->hg[gi];

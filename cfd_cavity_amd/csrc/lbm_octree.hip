@@ -145,13 +145,24 @@ __device__ __forceinline__ bool bounce(const OctLevel& L, const OctStream& s, in
          (sy == L.ny - 1 && s.wt);
 }
 
-// stream + BCs + moments + walls of one node, from the post-collision set C.
-template <int GEQ>
-__device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& s, const double* C,
-                                            int k, int ix, int iy, double (&f)[9],
+// Post-collision values of a level in HBM, C[i*plane + y*nx + x].
+struct CGlobal {
+  const double* C;
+  long long P;
+  int nx;
+  __device__ __forceinline__ double operator()(int i, int x, int y) const {
+    return C[i * P + y * nx + x];
+  }
+};
+
+// stream + BCs + moments + walls of one node from post-collision values
+// src(i, x, y) (HBM or a tile in LDS); fyp is the node's fy of the same
+// sub-step's collision.
+template <int GEQ, class Src>
+__device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& s, const Src& src,
+                                            double fyp, int ix, int iy, double (&f)[9],
                                             double (&g)[9], double& rho, double& ux,
                                             double& uy, double& temp) {
-  const long long P = L.plane;
   const int nx = L.nx, ny = L.ny;
 
   // f: pull; the block form leaves its ghost layers as they are
@@ -160,11 +171,11 @@ __device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& 
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       const int sx = ix - kCx[i], sy = iy - kCy[i];
-      f[i] = bounce(L, s, sx, sy) ? C[kOpp[i] * P + k] : C[i * P + sy * nx + sx];
+      f[i] = bounce(L, s, sx, sy) ? src(kOpp[i], ix, iy) : src(i, sx, sy);
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = C[i * P + k];
+    for (int i = 0; i < 9; ++i) f[i] = src(i, ix, iy);
   }
 
   // g: the thermal boundary conditions resolved per node.  Every node ends
@@ -192,11 +203,10 @@ __device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& 
 #pragma unroll
     for (int i = 0; i < 9; ++i) g[i] = eq_g<double, GEQ>(i, tw, e0);
   } else {
-    const int kg = gy * nx + gx;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       const int sx = gx - kCx[i], sy = gy - kCy[i];
-      g[i] = bounce(L, s, sx, sy) ? C[(9 + i) * P + kg] : C[(9 + i) * P + sy * nx + sx];
+      g[i] = bounce(L, s, sx, sy) ? src(9 + i, gx, gy) : src(9 + i, sx, sy);
     }
   }
 
@@ -206,7 +216,7 @@ __device__ __forceinline__ void stream_node(const OctLevel& L, const OctStream& 
   uy = 0.0;
   if (mo.r > kF64Eps) {
     ux = mo.mx / mo.r;
-    uy = mo.my / mo.r + 0.5 * L.m[kOctFy * P + k] / mo.r;
+    uy = mo.my / mo.r + 0.5 * fyp / mo.r;
   }
   if (is_wall_node(L, s, ix, iy)) {
     ux = 0.0;
@@ -225,7 +235,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream(OctGroup grp) {
   if (k < 0) return;
   const long long P = L.plane;
   double f[9], g[9], rho, ux, uy, temp;
-  stream_node<GEQ>(L, grp.S[item], L.c, k, k % L.nx, k / L.nx, f, g, rho, ux, uy, temp);
+  stream_node<GEQ>(L, grp.S[item], CGlobal{L.c, P, L.nx}, L.m[kOctFy * P + k], k % L.nx,
+                   k / L.nx, f, g, rho, ux, uy, temp);
   double* M = L.m;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -253,7 +264,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream_collide(OctGroup grp, 
   const long long P = L.plane;
   const int ix = k % L.nx, iy = k / L.nx;
   double f[9], g[9], rho, ux, uy, temp;
-  stream_node<GEQ>(L, grp.S[item], L.c, k, ix, iy, f, g, rho, ux, uy, temp);
+  stream_node<GEQ>(L, grp.S[item], CGlobal{L.c, P, L.nx}, L.m[kOctFy * P + k], ix, iy, f, g, rho,
+                   ux, uy, temp);
   const bool in = !p.interior_only ||
                   (ix >= 1 && ix <= L.nx - 2 && iy >= 1 && iy <= L.ny - 2);
   if (in) {
@@ -265,6 +277,129 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream_collide(OctGroup grp, 
   for (int i = 0; i < 9; ++i) {
     L.c2[i * P + k] = f[i];
     L.c2[(9 + i) * P + k] = g[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Whole sub-steps of a level in one launch.  A workgroup owns output rows
+// [y0, y0 + ty) of a level, all columns.  Sub-step 1 collides the rows grown
+// by NS (read from the M-form m); each sub-step's stream runs on rows grown by
+// NS - s, pulling from the previous collision's values in LDS, and, before the
+// last, collides again; the last stream writes the new M-form to m2.  Every
+// node is computed with the separate launches' functions (collide_node,
+// stream_node), so the result is bitwise theirs.  The rows outside a tile
+// are recomputed by its neighbours; the level's m is only read, so no tile
+// sees another's output.
+struct OctTileGroup {
+  OctLevel L[2];
+  OctStream S[2];
+  int ty;      // output rows per tile
+  int ntile0;  // tiles of item 0
+};
+
+constexpr int kOctTileMaxThreads = 1024;
+constexpr int kOctTileLdsBytes = 65536;  // dynamic LDS of one tile, at most
+
+// Post-collision values of a tile's region in LDS: planes 0..8 f, 9 fy,
+// 10..18 g, rows from r0, all nx columns.
+struct CTile {
+  const double* sh;
+  int q, nx, r0;
+  __device__ __forceinline__ double operator()(int i, int x, int y) const {
+    return sh[(i < 9 ? i : i + 1) * q + (y - r0) * nx + x];
+  }
+};
+
+template <int GEQ, int NS>
+__global__ __launch_bounds__(kOctTileMaxThreads) void k_oct_tile(OctTileGroup grp,
+                                                                  OctCollide p) {
+  extern __shared__ double sh[];
+  int lt;
+  const int item = group_item(grp.ntile0, lt);
+  const OctLevel& L = grp.L[item];
+  const OctStream& S = grp.S[item];
+  const int nx = L.nx, ny = L.ny;
+  const long long P = L.plane;
+  const int y0 = lt * grp.ty, y1 = min(y0 + grp.ty, ny);
+  const int q = nx * (grp.ty + 2 * NS);  // LDS plane stride
+  const int t = (int)threadIdx.x;
+
+  // sub-step 1's collision on rows [y0 - NS, y1 + NS), from the M-form
+  int pr0 = max(y0 - NS, 0);
+  const int pr1 = min(y1 + NS, ny);
+  double f[9], g[9], fyc = 0.0;
+  bool have = t < nx * (pr1 - pr0);
+  if (have) {
+    const int ix = t % nx, iy = pr0 + t / nx;
+    const int k = iy * nx + ix;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      f[i] = L.m[i * P + k];
+      g[i] = L.m[(9 + i) * P + k];
+    }
+    fyc = L.m[kOctFy * P + k];
+    const bool in = !p.interior_only || (ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2);
+    if (in) {
+      double nut;
+      collide_node<double, GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
+                                L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx, 0.0, fyc, nut);
+    }
+  }
+
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) {
+    if (have) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        sh[i * q + t] = f[i];
+        sh[(10 + i) * q + t] = g[i];
+      }
+      sh[9 * q + t] = fyc;
+    }
+    __syncthreads();
+    // this sub-step's stream on rows [y0 - (NS - s), y1 + (NS - s))
+    const int r0 = max(y0 - (NS - s), 0), r1 = min(y1 + (NS - s), ny);
+    const bool act = t < nx * (r1 - r0);
+    const int ix = t % nx, iy = r0 + t / nx;
+    double f2[9], g2[9], rho = 0.0, ux = 0.0, uy = 0.0, temp = 0.0, fyp = 0.0;
+    if (act) {
+      fyp = sh[9 * q + (iy - pr0) * nx + ix];
+      stream_node<GEQ>(L, S, CTile{sh, q, nx, pr0}, fyp, ix, iy, f2, g2, rho, ux, uy, temp);
+    }
+    if (s == NS) {
+      if (act) {
+        const int k = iy * nx + ix;
+        double* M = L.m2;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+          M[i * P + k] = f2[i];
+          M[(9 + i) * P + k] = g2[i];
+        }
+        M[kOctRho * P + k] = rho;
+        M[kOctUx * P + k] = ux;
+        M[kOctUy * P + k] = uy;
+        M[kOctT * P + k] = temp;
+        M[kOctFy * P + k] = fyp;
+      }
+    } else {
+      __syncthreads();  // the next sub-step rewrites LDS
+      fyc = fyp;
+      if (act) {
+        const bool in =
+            !p.interior_only || (ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2);
+        if (in) {
+          double nut;
+          collide_node<double, GEQ>(f2, g2, rho, ux, uy, temp, p.ctx, 0.0, fyc, nut);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+          f[i] = f2[i];
+          g[i] = g2[i];
+        }
+      }
+      have = act;
+      pr0 = r0;
+    }
   }
 }
 
@@ -523,6 +658,46 @@ hipError_t launch_oct_stream_collide(const OctLevel* L, const OctStream* s, int 
     hipLaunchKernelGGL(k_oct_stream_collide<1>, dim3(total), dim3(kOctBlock), 0, st, g, p);
   else
     hipLaunchKernelGGL(k_oct_stream_collide<2>, dim3(total), dim3(kOctBlock), 0, st, g, p);
+  return hipGetLastError();
+}
+
+bool oct_tile_fits(int nx, int ny, int ty, int ns) {
+  // levels are at least 6 x 6 (the block rules reach 3 rows and columns in)
+  if (ns < 1 || ns > 2 || ty < 3 || nx < 6 || ny < 6) return false;
+  const long long rows = (long long)ty + 2 * ns;
+  if (nx * rows > kOctTileMaxThreads) return false;
+  if (19 * nx * rows * (long long)sizeof(double) > kOctTileLdsBytes) return false;
+  const int last = ny % ty;
+  return last == 0 || last >= 3;
+}
+
+hipError_t launch_oct_tile(const OctLevel* L, const OctStream* s, int count, int ty, int ns,
+                           const OctCollide& p, int geq, hipStream_t st) {
+  OctTileGroup g{};
+  int total = 0, threads = 0;
+  for (int i = 0; i < count; ++i) {
+    if (!oct_tile_fits(L[i].nx, L[i].ny, ty, ns)) return hipErrorInvalidValue;
+    g.L[i] = L[i];
+    g.S[i] = s[i];
+    const int nt = (L[i].ny + ty - 1) / ty;
+    if (i == 0) g.ntile0 = nt;
+    total += nt;
+    threads = std::max(threads, L[i].nx * (ty + 2 * ns));
+  }
+  if (count == 1) g.ntile0 = total;
+  g.ty = ty;
+  threads = (threads + 63) / 64 * 64;
+  int lds_nx = 0;
+  for (int i = 0; i < count; ++i) lds_nx = std::max(lds_nx, L[i].nx);
+  const size_t lds = (size_t)19 * lds_nx * (ty + 2 * ns) * sizeof(double);
+#define CAVLBM_OCT_TILE(G, N)                                                                \
+  if (geq == G && ns == N)                                                                   \
+    hipLaunchKernelGGL((k_oct_tile<G, N>), dim3(total), dim3(threads), lds, st, g, p);
+  CAVLBM_OCT_TILE(1, 1)
+  CAVLBM_OCT_TILE(1, 2)
+  CAVLBM_OCT_TILE(2, 1)
+  CAVLBM_OCT_TILE(2, 2)
+#undef CAVLBM_OCT_TILE
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ struct OctLevel {
   double* m;
   double* c;
   double* c2;  // second C-form, for two sub-steps fused across the stream
+  double* m2;  // second M-form: the tiled sub-step kernel reads m, writes m2
 };
 
 // Relaxation and force of one collision pass (RelaxTimes of the level,
@@ -68,6 +69,16 @@ hipError_t launch_oct_stream(const OctLevel* L, const OctStream* s, int count, i
                              hipStream_t st);
 hipError_t launch_oct_stream_collide(const OctLevel* L, const OctStream* s, int count,
                                      const OctCollide& p, int geq, hipStream_t st);
+// NS (1 or 2) whole sub-steps of `count` levels in one launch on full-width
+// row tiles of `ty` output rows: collision from the M-form (m), the sub-steps'
+// streams and collisions through LDS, the new M-form written to m2 (the
+// caller then exchanges m and m2).  oct_tile_fits: whether a level of nx x ny
+// nodes takes tiles of ty rows (threads, LDS, and a last tile of at least 3
+// rows so that the boundary rows' copies reach inside it).
+bool oct_tile_fits(int nx, int ny, int ty, int ns);
+hipError_t launch_oct_tile(const OctLevel* L, const OctStream* s, int count, int ty, int ns,
+                           const OctCollide& p, int geq, hipStream_t st);
+
 hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
                                      const OctGhostTab* t, int count, const OctXi& xi, int geq,
                                      hipStream_t st);

@@ -79,6 +79,9 @@ def test_initial_state_and_constants_bitwise(oracle, levels):
     ((64, 256, 8, 1.0e6), dict(levels=2), 200),
     ((64, 256, 8, 1.0e7), dict(levels=3, seed=7), 60),
     ((32, 64, 4, 1.0e6), dict(levels=2, geq_order=1), 15),
+    # a base grid too wide for the tiled sub-step kernel: it takes the
+    # separate launches while the blocks take tiles
+    ((128, 128, 8, 1.0e6), dict(levels=2), 20),
 ])
 def test_steps_bitwise(oracle, args, kw, nsteps):
     g, o = pair(oracle, *args, **kw)
