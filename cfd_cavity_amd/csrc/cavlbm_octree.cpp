@@ -230,6 +230,9 @@ OctCollide collide_params(double beta, double nu, double alpha, const Relax& r, 
   p.ctx.cs = 0.0;
   p.ctx.pr_t = 1.0;
   p.ctx.inv_pr_t = 1.0;
+  // the level's RelaxTimes as relaxation factors, in collide_node's order
+  // with its nu_t = alpha_t = +0 (Cs = 0)
+  p.rf = relax_factors<double>(0.5 + 3.0 * (nu + 0.0), 0.5 + 3.0 * (alpha + 0.0));
   p.interior_only = interior ? 1 : 0;
   return p;
 }

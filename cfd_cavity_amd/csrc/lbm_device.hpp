@@ -195,8 +195,8 @@ template <> struct Arith<double> {
     r = __builtin_fma(-b, q, a);
     return __builtin_fma(r, y, q);
   }
-  __device__ static __forceinline__ double div(double a, double b) { return a / b; }
-  __device__ static __forceinline__ double rcp(double b) { return 1.0 / b; }
+  __host__ __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+  __host__ __device__ static __forceinline__ double rcp(double b) { return 1.0 / b; }
 };
 template <> struct Arith<float> {
   __device__ static __forceinline__ float divr(float a, float, float y) { return a * y; }
@@ -312,19 +312,17 @@ __device__ __forceinline__ R geq(int i, R t, R ux, R uy) {
 }
 
 // ---------------------------------------------------------------------------
-// collide_node (solver.rs:1054-1185): TRT Lambda = 3/16 with Smagorinsky nu_t
-// from pi_neq and the Guo odd/even force split for f; BGK for g.  With PRE the
-// turbulent viscosity and diffusivity are the given per-node values of the
-// dynamic model (solver.rs:1118-1139) instead of (Cs dx)^2 |S| and nu_t/Pr_t.
-template <class R, int GEQ, bool PRE = false>
-__device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, R uy,
-                                             R temp, const StepCtx& c, R cs, R& fy_out,
-                                             R& nut_out, R nut_pre = R(0.0),
-                                             R at_pre = R(0.0)) {
-  using A = Arith<R>;
-  const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
-  fy_out = fy;
-
+// The collision in two parts.  collide_neq: the equilibria (the crate's
+// feq/geq) and the non-equilibrium parts f - feq, g - geq.  collide_relax:
+// the TRT update of f with the Guo odd/even force split and the BGK update of
+// g, given the relaxation factors
+//   a_f = (1/tau+ + 1/tau-)/2, b_f = (1/tau+ - 1/tau-)/2,
+//   a_g = (1/tau_t + 1/tau_t)/2, b_g = (1/tau_t - 1/tau_t)/2,
+//   k_minus = 1 - 1/(2 tau-), k_plus = 1 - 1/(2 tau+)
+// (solver.rs:1140-1185).
+template <class R, int GEQ>
+__device__ __forceinline__ void collide_neq(const R (&f)[9], const R (&g)[9], R rho, R ux, R uy,
+                                            R temp, R (&d_f)[9], R (&d_g)[9]) {
   EqTerms<R> e;
   eq_terms<R>(ux, uy, e);
   R feq_[9], geq_[9];
@@ -340,38 +338,45 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
     feq_[0] = rho - (feq_[1] + feq_[2] + feq_[3] + feq_[4] + feq_[5] + feq_[6] + feq_[7] + feq_[8]);
     geq_[0] = temp - (geq_[1] + geq_[2] + geq_[3] + geq_[4] + geq_[5] + geq_[6] + geq_[7] + geq_[8]);
   }
-  R d_f[9], d_g[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     d_f[i] = f[i] - feq_[i];
     d_g[i] = g[i] - geq_[i];
   }
-  // pi_neq = sum c c (f - feq) (solver.rs:1096-1106).  With c in {-1,0,1} the
-  // products are exact; zero products are dropped because the accumulators
-  // start at +0 and never hold -0, so adding a signed zero leaves them as is.
-  const R pi_xx = R(0.0) + d_f[1] + d_f[2] + d_f[4] + d_f[5] + d_f[6] + d_f[8];
-  const R pi_xy = R(0.0) + d_f[2] - d_f[4] + d_f[6] - d_f[8];
-  const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
-  const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
-  const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
-  const R nu_t = PRE ? nut_pre : cs * cs * s_mag;
-  nut_out = nu_t;
+}
 
-  const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
+// Relaxation factors of one collision (the per-node part of collide_node, or
+// constants of a level with fixed relaxation times).
+template <class R>
+struct RelaxF {
+  R a_f, b_f, a_g, b_g, k_minus, k_plus;
+};
+
+// From tau+ and tau_t+ (tau- from the magic parameter Lambda = 3/16), in the
+// reference's order: 1/tau (IEEE); 0.5/tau = 0.5*(1/tau) exactly
+// (power-of-two scaling).  Host and device evaluate it alike.
+template <class R>
+__host__ __device__ __forceinline__ RelaxF<R> relax_factors(R tau_plus, R tau_t_plus) {
+  using A = Arith<R>;
   const R tau_minus = R(0.5) + A::div(R(3.0 / 16.0), tau_plus - R(0.5));
-  const R alpha_t = PRE ? at_pre : A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
-  const R tau_t_plus = R(0.5) + R(3.0) * (R(c.alpha) + alpha_t);
-  // 1/tau (IEEE); 0.5/tau = 0.5*(1/tau) exactly (power-of-two scaling).
   const R itp = A::rcp(tau_plus);
   const R itm = A::rcp(tau_minus);
   const R itt = A::rcp(tau_t_plus);  // tau_t_minus = tau_t_plus (BGK for g)
-  const R a_f = R(0.5) * (itp + itm);
-  const R b_f = R(0.5) * (itp - itm);
-  const R a_g = R(0.5) * (itt + itt);
-  const R b_g = R(0.5) * (itt - itt);
-  const R k_minus = R(1.0) - R(0.5) * itm;
-  const R k_plus = R(1.0) - R(0.5) * itp;
+  RelaxF<R> r;
+  r.a_f = R(0.5) * (itp + itm);
+  r.b_f = R(0.5) * (itp - itm);
+  r.a_g = R(0.5) * (itt + itt);
+  r.b_g = R(0.5) * (itt - itt);
+  r.k_minus = R(1.0) - R(0.5) * itm;
+  r.k_plus = R(1.0) - R(0.5) * itp;
+  return r;
+}
 
+template <class R>
+__device__ __forceinline__ void collide_relax(R (&f)[9], R (&g)[9], R ux, R uy, R fy,
+                                              const R (&d_f)[9], const R (&d_g)[9],
+                                              const RelaxF<R>& r) {
+  using A = Arith<R>;
   // Guo source S_i = w_i [(cy - uy)/cs2 + (c.u) cy / cs2^2] fy; its odd part
   // s_minus = w_i (cy/cs2) fy carries the momentum (solver.rs:1168-1179).
   // (cy - uy)/cs2 takes three values; (c.u) cy is +-eu of the directions
@@ -395,14 +400,60 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
     const R cy = R(kCy[i]);
     const R s_minus = R(kW[i]) * (cy / R(kCs2)) * fy;
     const R s_plus = s_i[i] - s_minus;
-    fo[i] = f[i] + (-a_f * d_f[i] - b_f * d_f[kOpp[i]] + k_minus * s_minus +
-                    k_plus * s_plus);
+    fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + r.k_minus * s_minus +
+                    r.k_plus * s_plus);
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     f[i] = fo[i];
-    g[i] = g[i] + (-a_g * d_g[i] - b_g * d_g[kOpp[i]]);
+    g[i] = g[i] + (-r.a_g * d_g[i] - r.b_g * d_g[kOpp[i]]);
   }
+}
+
+// collide_node (solver.rs:1054-1185): TRT Lambda = 3/16 with Smagorinsky nu_t
+// from pi_neq and the Guo odd/even force split for f; BGK for g.  With PRE the
+// turbulent viscosity and diffusivity are the given per-node values of the
+// dynamic model (solver.rs:1118-1139) instead of (Cs dx)^2 |S| and nu_t/Pr_t.
+template <class R, int GEQ, bool PRE = false>
+__device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, R uy,
+                                             R temp, const StepCtx& c, R cs, R& fy_out,
+                                             R& nut_out, R nut_pre = R(0.0),
+                                             R at_pre = R(0.0)) {
+  using A = Arith<R>;
+  const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
+  fy_out = fy;
+  R d_f[9], d_g[9];
+  collide_neq<R, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g);
+  // pi_neq = sum c c (f - feq) (solver.rs:1096-1106).  With c in {-1,0,1} the
+  // products are exact; zero products are dropped because the accumulators
+  // start at +0 and never hold -0, so adding a signed zero leaves them as is.
+  const R pi_xx = R(0.0) + d_f[1] + d_f[2] + d_f[4] + d_f[5] + d_f[6] + d_f[8];
+  const R pi_xy = R(0.0) + d_f[2] - d_f[4] + d_f[6] - d_f[8];
+  const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
+  const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
+  const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
+  const R nu_t = PRE ? nut_pre : cs * cs * s_mag;
+  nut_out = nu_t;
+
+  const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
+  const R alpha_t = PRE ? at_pre : A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
+  const R tau_t_plus = R(0.5) + R(3.0) * (R(c.alpha) + alpha_t);
+  collide_relax<R>(f, g, ux, uy, fy, d_f, d_g, relax_factors<R>(tau_plus, tau_t_plus));
+}
+
+// A collision with relaxation factors fixed in advance (the octree levels'
+// RelaxTimes, solver.rs:578-596 of the octree; collide_node_cavity,
+// cavity.rs:167-243): the force, the non-equilibrium parts and the update.
+template <int GEQ>
+__device__ __forceinline__ void collide_node_fixed(double (&f)[9], double (&g)[9], double rho,
+                                                   double ux, double uy, double temp,
+                                                   double beta, double t_ref,
+                                                   const RelaxF<double>& r, double& fy_out) {
+  const double fy = rho * beta * (temp - t_ref);
+  fy_out = fy;
+  double d_f[9], d_g[9];
+  collide_neq<double, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g);
+  collide_relax<double>(f, g, ux, uy, fy, d_f, d_g, r);
 }
 
 // ---------------------------------------------------------------------------

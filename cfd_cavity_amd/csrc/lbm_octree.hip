@@ -100,10 +100,11 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_walls(OctLevel L, OctStream s
 }
 
 // collide_cavity (cavity.rs:167-243): collide_node_cavity is the uniform
-// collide_node with nu_t = 0 (Cs = 0): tau+ = 0.5 + 3 nu, tau- = 0.5 +
-// (3/16)/(tau+ - 0.5), tau_t = 0.5 + 3 alpha, a_g = 1/tau_t, b_g = 0, all
-// bitwise the level's RelaxTimes.  Nodes outside the window keep their
-// populations (ghost layers carry the injected values).
+// collide_node's force, equilibria and update with the level's fixed
+// relaxation factors (tau+ = 0.5 + 3 nu, tau- = 0.5 + (3/16)/(tau+ - 0.5),
+// tau_t = 0.5 + 3 alpha, a_g = 1/tau_t, b_g = 0: the level's RelaxTimes).
+// Nodes outside the window keep their populations (ghost layers carry the
+// injected values).
 template <int GEQ>
 __global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctGroup grp, OctCollide p) {
   int lb;
@@ -125,9 +126,9 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctGroup grp, OctColl
     f[i] = L.m[i * P + k];
     g[i] = L.m[(9 + i) * P + k];
   }
-  double fy, nut;
-  collide_node<double, GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
-                            L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx, 0.0, fy, nut);
+  double fy;
+  collide_node_fixed<GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k], L.m[kOctUy * P + k],
+                          L.m[kOctT * P + k], p.ctx.beta, p.ctx.t_ref, p.rf, fy);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     L.c[i * P + k] = f[i];
@@ -269,8 +270,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream_collide(OctGroup grp, 
   const bool in = !p.interior_only ||
                   (ix >= 1 && ix <= L.nx - 2 && iy >= 1 && iy <= L.ny - 2);
   if (in) {
-    double fy, nut;
-    collide_node<double, GEQ>(f, g, rho, ux, uy, temp, p.ctx, 0.0, fy, nut);
+    double fy;
+    collide_node_fixed<GEQ>(f, g, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref, p.rf, fy);
     L.m[kOctFy * P + k] = fy;
   }
 #pragma unroll
@@ -339,11 +340,10 @@ __global__ __launch_bounds__(kOctTileMaxThreads) void k_oct_tile(OctTileGroup gr
     }
     fyc = L.m[kOctFy * P + k];
     const bool in = !p.interior_only || (ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2);
-    if (in) {
-      double nut;
-      collide_node<double, GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
-                                L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx, 0.0, fyc, nut);
-    }
+    if (in)
+      collide_node_fixed<GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
+                              L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx.beta, p.ctx.t_ref,
+                              p.rf, fyc);
   }
 
 #pragma unroll
@@ -387,10 +387,8 @@ __global__ __launch_bounds__(kOctTileMaxThreads) void k_oct_tile(OctTileGroup gr
       if (act) {
         const bool in =
             !p.interior_only || (ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2);
-        if (in) {
-          double nut;
-          collide_node<double, GEQ>(f2, g2, rho, ux, uy, temp, p.ctx, 0.0, fyc, nut);
-        }
+        if (in)
+          collide_node_fixed<GEQ>(f2, g2, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref, p.rf, fyc);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
           f[i] = f2[i];

@@ -25,10 +25,13 @@ struct OctLevel {
   double* m2;  // second M-form: the tiled sub-step kernel reads m, writes m2
 };
 
-// Relaxation and force of one collision pass (RelaxTimes of the level,
-// beta of the level), handed to the shared collide_node with Smagorinsky off.
+// Relaxation and force of one collision pass: beta of the level and the
+// relaxation factors of its RelaxTimes (computed on the host with
+// relax_factors from tau+ = 0.5 + 3 nu, tau_t = 0.5 + 3 alpha), handed to
+// collide_node_fixed.
 struct OctCollide {
-  StepCtx ctx;
+  StepCtx ctx;        // beta, t_ref (= 0) of the level
+  RelaxF<double> rf;  // the level's relaxation factors
   int interior_only;  // window [1, n-2]^2 (fine levels) or the whole level
 };
 
