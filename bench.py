@@ -6,7 +6,7 @@ Pr=0.71, Pr_t=0.9, Cs=0.10, utarget=0.005, seed 42, the reference's init
 (linear T + xorshift64* noise of +-1e-3).  A step is one fused
 stream+BC+moments+collide pass over the whole lattice; inputs stay resident
 in HBM.  The run loop advances several steps per kernel launch (temporal
-blocking, bitwise equal to single steps: 4 in FP64, 3 in FP32); the
+blocking, bitwise equal to single steps: 4 in FP64, 3 in FP32 and FP16-offset); the
 "fp64_2step" and "fp64_1step" forms time two steps per launch and the
 single-step kernel beside it.  Prints ONE JSON line.
 

@@ -609,8 +609,9 @@ int cavlbm_create(const cavlbm_params* pin, cavlbm_solver** out) {
     s->geon[n] = g;
     s->fitsn[n] = stepn_grid((int)p.storage, n, s->geon[n]);
   }
-  s->tblock = p.storage == ST_FP64 ? kDefaultBlockFp64
-                                   : (p.storage == ST_FP32 ? kDefaultBlockFp32 : 1);
+  s->tblock = p.storage == ST_FP64
+                  ? kDefaultBlockFp64
+                  : (p.storage == ST_FP32 ? kDefaultBlockFp32 : kDefaultBlockFp16);
   if (18 * g.plane >= (1LL << 31)) {
     delete s;
     return fail(nullptr, CAVLBM_E_PARAM, "lattice too large: 18*pitch*ny must be < 2^31");
@@ -1371,8 +1372,9 @@ int cavlbm_batch_create(const cavlbm_params* cavities, uint64_t n, cavlbm_batch*
   }
   // steps per launch: the storage's default, or the largest that every
   // cavity's tiling fits
-  const int want = b->storage == ST_FP64 ? kDefaultBlockFp64
-                                         : (b->storage == ST_FP32 ? kDefaultBlockFp32 : 1);
+  const int want = b->storage == ST_FP64
+                       ? kDefaultBlockFp64
+                       : (b->storage == ST_FP32 ? kDefaultBlockFp32 : kDefaultBlockFp16);
   b->nblk = 1;
   for (int n = want; n >= 2; --n)
     if (batch_fits(b, n)) {
@@ -1532,7 +1534,7 @@ int cavlbm_batch_autotune(cavlbm_batch* b, uint64_t trial_steps, cavlbm_tune_res
   for (int shape = 0; shape < kNumBatchShapes; ++shape)
     for (int xcd = 1; xcd >= 0; --xcd) cands.push_back({shape, 1, xcd});
   for (int n = 2; n <= kMaxBlockSteps; ++n)
-    if ((b->storage == ST_FP64 || b->storage == ST_FP32) && batch_fits(b, n))
+    if (batch_fits(b, n))
       for (int xcd = 1; xcd >= 0; --xcd) cands.push_back({b->shape, n, xcd});
   double best = 1e300, first = 0.0;
   Cand best_c = cands[0];

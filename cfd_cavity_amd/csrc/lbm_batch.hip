@@ -147,9 +147,13 @@ hipError_t launch_step_batch_lowp(int storage, int geq, int shape, const CavDesc
 hipError_t launch_stepn_batch_lowp(int storage, int geq, int n, const CavDesc* d, const int* tc,
                                    int total, int par, int fpar, int step, int wa, int xcd,
                                    hipStream_t st) {
-  if (storage != ST_FP32) return hipErrorInvalidValue;
-  return geq == 1 ? by_n<ST_FP32, 1>(n, d, tc, total, par, fpar, step, wa, xcd, st)
-                  : by_n<ST_FP32, 2>(n, d, tc, total, par, fpar, step, wa, xcd, st);
+  if (storage == ST_FP32)
+    return geq == 1 ? by_n<ST_FP32, 1>(n, d, tc, total, par, fpar, step, wa, xcd, st)
+                    : by_n<ST_FP32, 2>(n, d, tc, total, par, fpar, step, wa, xcd, st);
+  if (storage == ST_FP16_OFFSET)
+    return geq == 1 ? by_n<ST_FP16_OFFSET, 1>(n, d, tc, total, par, fpar, step, wa, xcd, st)
+                    : by_n<ST_FP16_OFFSET, 2>(n, d, tc, total, par, fpar, step, wa, xcd, st);
+  return hipErrorInvalidValue;
 }
 #endif
 

@@ -225,6 +225,16 @@ template <> struct StorageT<ST_FP32> {
   __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
   __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R) { p[o] = v; }
 };
+// The FP32 value x as an opaque register value: keeps the compiler from
+// fusing the FP32 operation that produced x with a following FP32 -> FP16
+// conversion (v_fma_mixlo_f16 rounds the exact result to FP16 once, the
+// separate conversion rounds the FP32 result again), so every kernel rounds
+// a stored deviation the same way: to FP32, then to FP16.
+__device__ __forceinline__ float opaque_f32(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // FP16 storage of the deviations from each population's local rest state:
 // f_i - w_i, and g_i - w_i*T_j with T_j the temperature of the node j that
 // wrote the value (kept in one FP32 plane after the 18 FP16 planes).  The
@@ -241,10 +251,10 @@ template <> struct StorageT<ST_FP16_OFFSET> {
     return __half2float(p[o]) + (float)kW[i] * t_src;
   }
   __device__ static __forceinline__ void store_f(T* p, unsigned o, int i, R v) {
-    p[o] = __float2half_rn(v - (float)kW[i]);
+    p[o] = __float2half_rn(opaque_f32(v - (float)kW[i]));
   }
   __device__ static __forceinline__ void store_g(T* p, unsigned o, int i, R v, R t_node) {
-    p[o] = __float2half_rn(v - (float)kW[i] * t_node);
+    p[o] = __float2half_rn(opaque_f32(v - (float)kW[i] * t_node));
   }
 };
 

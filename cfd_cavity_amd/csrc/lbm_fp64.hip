@@ -49,6 +49,9 @@ bool stepn_tile_dims(int storage, int n, int* tx, int* ty) {
   CAVLBM_TILE(ST_FP32, 2)
   CAVLBM_TILE(ST_FP32, 3)
   CAVLBM_TILE(ST_FP32, 4)
+  CAVLBM_TILE(ST_FP16_OFFSET, 2)
+  CAVLBM_TILE(ST_FP16_OFFSET, 3)
+  CAVLBM_TILE(ST_FP16_OFFSET, 4)
 #undef CAVLBM_TILE
   *tx = *ty = 0;
   return false;

@@ -48,27 +48,27 @@ hipError_t launch_step_lowp(int storage, int geq, int mode, const Geom& geo,
   return hipErrorInvalidValue;
 }
 
-template <int GEQ, int N>
+template <int S, int GEQ, int N>
 static hipError_t gon(const Geom& geo_n, const StepCtxN& cn, const Walls& walls, const void* src,
                       void* dst, const void* fy_in, void* fy_out, void* nut, int write_aux,
                       hipStream_t st) {
-  if (!stepn_fits<ST_FP32, N>(geo_n.nx, geo_n.ny)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_stepn<ST_FP32, GEQ, N>), dim3(geo_n.nbx * geo_n.nby),
-                     dim3(TileN<ST_FP32, N>::kNT), 0, st, geo_n, cn, walls,
-                     static_cast<const float*>(src), static_cast<float*>(dst),
-                     static_cast<const float*>(fy_in), static_cast<float*>(fy_out),
-                     static_cast<float*>(nut), write_aux);
+  using T = typename StorageT<S>::T;
+  using R = typename StorageT<S>::R;
+  if (!stepn_fits<S, N>(geo_n.nx, geo_n.ny)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_stepn<S, GEQ, N>), dim3(geo_n.nbx * geo_n.nby), dim3(TileN<S, N>::kNT), 0,
+                     st, geo_n, cn, walls, static_cast<const T*>(src), static_cast<T*>(dst),
+                     static_cast<const R*>(fy_in), static_cast<R*>(fy_out), static_cast<R*>(nut),
+                     write_aux);
   return hipGetLastError();
 }
 
-// Several fused steps per launch for FP32 storage (FP16-offset storage rounds
-// the populations between steps and keeps the single-step kernel).
-hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+template <int S>
+static hipError_t dispatch_n(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st) {
-  if (storage != ST_FP32) return hipErrorInvalidValue;
-#define CAVLBM_CASE(G, N) \
-  if (geq == G && n == N) return gon<G, N>(geo_n, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
+#define CAVLBM_CASE(G, N)                                                                  \
+  if (geq == G && n == N)                                                                  \
+    return gon<S, G, N>(geo_n, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, st);
   CAVLBM_CASE(2, 2)
   CAVLBM_CASE(2, 3)
   CAVLBM_CASE(2, 4)
@@ -76,6 +76,21 @@ hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, con
   CAVLBM_CASE(1, 3)
   CAVLBM_CASE(1, 4)
 #undef CAVLBM_CASE
+  return hipErrorInvalidValue;
+}
+
+// Several fused steps per launch for FP32 and FP16-offset storage (the
+// latter rounds the populations through its store and load between the
+// sub-steps, as single steps do).
+hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+                             const Walls& walls, const void* src, void* dst, const void* fy_in,
+                             void* fy_out, void* nut, int write_aux, hipStream_t st) {
+  if (storage == ST_FP32)
+    return dispatch_n<ST_FP32>(geq, n, geo_n, cn, walls, src, dst, fy_in, fy_out, nut, write_aux,
+                               st);
+  if (storage == ST_FP16_OFFSET)
+    return dispatch_n<ST_FP16_OFFSET>(geq, n, geo_n, cn, walls, src, dst, fy_in, fy_out, nut,
+                                      write_aux, st);
   return hipErrorInvalidValue;
 }
 

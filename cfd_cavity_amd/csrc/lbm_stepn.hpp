@@ -12,9 +12,10 @@
 //
 // Every node's arithmetic is the single-step kernel's (pull_node /
 // node_moments / collide_node), and the values a sub-step pulls are the ones
-// the previous step would have stored, so the result equals N launches of the
-// single-step kernel bit for bit for storage types equal to the arithmetic
-// type (FP64, FP32).
+// the previous step would have stored and the next one loaded (for
+// FP16-offset storage the populations go through the storage's own store and
+// load), so the result equals N launches of the single-step kernel bit for
+// bit in every storage form.
 //
 // fy is read at the halo nodes, which other tiles write in the same launch,
 // so it is double-buffered: the launch reads fy_in and writes fy_out.
@@ -49,10 +50,32 @@ template <> struct TileN<ST_FP64, 4> { static constexpr int kTX = 16, kTY = 16, 
 template <> struct TileN<ST_FP32, 2> { static constexpr int kTX = 64, kTY = 8, kNT = 704; };
 template <> struct TileN<ST_FP32, 3> { static constexpr int kTX = 32, kTY = 16, kNT = 768; };
 template <> struct TileN<ST_FP32, 4> { static constexpr int kTX = 16, kTY = 16, kNT = 512; };
+// FP16-offset storage: FP32 arithmetic, the FP32 tiles.
+template <int N> struct TileN<ST_FP16_OFFSET, N> : TileN<ST_FP32, N> {};
 
 // Steps per launch the run loop uses by default, per storage type.
 constexpr int kDefaultBlockFp64 = 4;
 constexpr int kDefaultBlockFp32 = 3;
+constexpr int kDefaultBlockFp16 = 3;
+
+// The value a population takes through a store to and a load from the
+// storage (identity for FP64 and FP32; FP16-offset rounds the deviation from
+// the rest state, the node's own temperature t for g).
+template <int S>
+__device__ __forceinline__ void storage_round(typename StorageT<S>::R (&f)[9],
+                                              typename StorageT<S>::R (&g)[9],
+                                              typename StorageT<S>::R t) {
+  using ST = StorageT<S>;
+  if (!ST::kTPlane) return;
+  typename ST::T q;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    ST::store_f(&q, 0u, i, f[i]);
+    f[i] = ST::load_f(&q, 0u, i);
+    ST::store_g(&q, 0u, i, g[i], t);
+    g[i] = ST::load_g(&q, 0u, i, t);
+  }
+}
 
 // The adiabatic top row copies the pulls of row ny-2, which reach row ny-3.
 // For sub-steps s < N the grown tile's first row lies below the top row; for
@@ -96,6 +119,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       pull_node<S, GEQ, INTERIOR>(geo, walls, src, fy_in[iy * geo.pitch + ix], ix, iy, f, g,
                                   rho, ux, uy, tc);
       collide_node<R, GEQ>(f, g, rho, ux, uy, tc, cn.c[0], R(cn.c[0].cs), fyc, nutc);
+      if (N > 1) storage_round<S>(f, g, tc);
     }
   }
 
@@ -141,6 +165,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       R rho, ux, uy;
       node_moments<R>(f2, g2, fyp, wall, rho, ux, uy, tc);
       collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, cn.c[s - 1], R(cn.c[s - 1].cs), fyc, nutc);
+      if (s < N) storage_round<S>(f2, g2, tc);
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         f[i] = f2[i];
@@ -163,6 +188,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
     ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
     ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], tc);
   }
+  if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)tc;
 }
 
 template <int S, int GEQ, int N>

@@ -411,7 +411,7 @@ class CavitySolver:
     @property
     def time_blocking(self):
         """Most steps per launch requested for the fused run loop (1..4), or
-        None for the library's default (FP64 4, FP32 3, FP16-offset 1)."""
+        None for the library's default (FP64 4, FP32 and FP16-offset 3)."""
         return self._tblock
 
     @time_blocking.setter

@@ -189,10 +189,11 @@ int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms
 /* Temporal blocking of the fused run loop (no reference counterpart; a
  * performance setting of this implementation): the run loop advances up to
  * `steps` (1..4) steps per kernel launch.  The defaults are 4 for FP64
- * storage, 3 for FP32 and 1 for FP16-offset, whose populations are rounded
- * between steps.  A block of n >= 2 steps needs a lattice that fits that
- * block's tiling: (ny - 1) % 12 != 0 for FP64 n = 2, % 16 for FP64 n = 3, 4,
- * % 8 for FP32 n = 2, % 16 for FP32 n = 3, 4.  It also needs no dynamic
+ * storage and 3 for FP32 and FP16-offset (whose populations pass through the
+ * storage's rounding between the steps of a launch as between launches).  A
+ * block of n >= 2 steps needs a lattice that fits that block's tiling:
+ * (ny - 1) % 12 != 0 for FP64 n = 2, % 16 for FP64 n = 3, 4, % 8 for FP32 /
+ * FP16-offset n = 2, % 16 for n = 3, 4.  It also needs no dynamic
  * model and a handle outside a batch.  Otherwise the loop takes the largest
  * block that fits, down to single steps.  Every choice gives the same
  * populations bit for bit.  cavlbm_steps_per_launch reports the steps per
@@ -286,12 +287,12 @@ int cavlbm_batch_synchronize(cavlbm_batch *b);
 /* Borrowed handle of cavity i for reads, diagnostics and setters (valid
  * until cavlbm_batch_destroy; do not destroy it; do not step it alone). */
 int cavlbm_batch_solver(cavlbm_batch *b, uint64_t i, cavlbm_solver **out);
-/* Times every single-step tile shape and every N-step block (N = 2..4, FP64
- * and FP32 storage, when every cavity fits its tiling), each with and
- * without the XCD remap, for trial_steps steps and keeps the fastest; the
- * cavities' state is restored afterwards.  A new batch advances the
- * storage's default steps per launch (4 FP64, 3 FP32) when every cavity
- * fits, single steps otherwise.  The batch
+/* Times every single-step tile shape and every N-step block (N = 2..4, when
+ * every cavity fits its tiling), each with and without the XCD remap, for
+ * trial_steps steps and keeps the fastest; the cavities' state is restored
+ * afterwards.  A new batch advances the storage's default steps per launch
+ * (4 FP64, 3 FP32 and FP16-offset) when every cavity fits, single steps
+ * otherwise.  The batch
  * must have been run at least one step (CAVLBM_E_PARAM otherwise); tune after
  * the spin-up so that the trials time the graph-replayed steady steps. */
 int cavlbm_batch_autotune(cavlbm_batch *b, uint64_t trial_steps,

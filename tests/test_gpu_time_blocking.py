@@ -45,12 +45,13 @@ SIZES = [(16, 40), (33, 50), (40, 14), (35, 34), (3, 3), (70, 82)]
 
 def fits(storage, n, ny):
     ty = {("fp64", 2): 12, ("fp64", 3): 16, ("fp64", 4): 16,
-          ("fp32", 2): 8, ("fp32", 3): 16, ("fp32", 4): 16}[(storage, n)]
+          ("fp32", 2): 8, ("fp32", 3): 16, ("fp32", 4): 16}[
+        ("fp32" if storage == "fp16_offset" else storage, n)]
     return (ny - 1) % ty != 0
 
 
 @pytest.mark.parametrize("n", [2, 3, 4])
-@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16_offset"])
 @pytest.mark.parametrize("nx,ny", SIZES)
 def test_block_launches_equal_single_steps(storage, n, nx, ny):
     cfg = dict(SMALL, nx=nx, ny=ny)
@@ -73,6 +74,8 @@ def test_defaults():
     assert a.steps_per_launch == 4  # FP64 default
     a = CavitySolver(*args_of(SMALL), storage="fp32")
     assert a.steps_per_launch == 3
+    a = CavitySolver(*args_of(SMALL), storage="fp16_offset")
+    assert a.steps_per_launch == 3
     a = CavitySolver(*args_of(dict(SMALL, ny=17)))  # 16 % 16 == 0: N=3, 4 do not fit
     assert a.steps_per_launch == 2
     with pytest.raises(Exception):
@@ -81,7 +84,8 @@ def test_defaults():
 
 @pytest.mark.parametrize("n,kw", [(4, dict(geq_order=1)), (4, dict(nu_start_mult=4.0)),
                                   (3, dict(nu_start_mult=4.0)), (2, dict(nu_start_mult=4.0)),
-                                  (3, dict(storage="fp32", nu_start_mult=4.0))])
+                                  (3, dict(storage="fp32", nu_start_mult=4.0)),
+                                  (3, dict(storage="fp16_offset", nu_start_mult=4.0))])
 def test_block_graphs_and_spin_up(n, kw):
     """Long runs: direct blocks through the spin-up, then graph replays of
     48 steps, then the direct tail that writes nu_t."""
@@ -117,7 +121,7 @@ def test_block_switching_mid_run():
         assert_same(a, b, f"switch t={a.time_step}")
 
 
-@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16_offset"])
 def test_heights_outside_the_tiling_use_single_steps(storage):
     ny = 49  # 48 = 0 mod 8, 12 and 16: no block fits
     cfg = dict(SMALL, nx=20, ny=ny)
@@ -129,7 +133,16 @@ def test_heights_outside_the_tiling_use_single_steps(storage):
     assert_same(a, b, f"{storage} 20x{ny}")
 
 
-def test_fp16_offset_keeps_single_steps():
-    a = make(SMALL, 4, storage="fp16_offset")
-    assert a.steps_per_launch == 1
-    assert CavitySolver(*args_of(SMALL), storage="fp16_offset").steps_per_launch == 1
+def test_fp16_offset_hot_wall_blocks():
+    """The FP16-offset g deviations refer to the writing node's temperature;
+    with a hot-wall profile the rounding between sub-steps must still follow
+    the stored temperature plane exactly."""
+    rng = np.random.default_rng(9)
+    theta = 0.5 + 0.01 * rng.uniform(-1, 1, SMALL["ny"])
+    a = make(SMALL, 3, storage="fp16_offset")
+    b = make(SMALL, 1, storage="fp16_offset")
+    a.set_hot_wall(theta)
+    b.set_hot_wall(theta)
+    a.run(131)
+    b.run(131)
+    assert_same(a, b, "fp16 hot wall")
