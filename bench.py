@@ -247,13 +247,11 @@ def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8)):
     return out
 
 
-def cpu_baseline(budget_s: float):
-    """The oracle (row-band OpenMP restatement of the rayon path) on a bounded
-    sample of the same workload: steps from the reference init until the time
-    budget is spent, with all host threads (max 16)."""
+def cpu_rate(threads: int, budget_s: float):
+    """Steps of the oracle on the 256x1024 workload from the reference init
+    until the time budget is spent: (MLUPS, steps, seconds)."""
     from oracle import pyoracle
 
-    threads = min(os.cpu_count() or 1, 16)
     w = WORKLOAD
     o = pyoracle.OracleSolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"],
                               w["utarget"], w["seed"], threads)
@@ -262,12 +260,86 @@ def cpu_baseline(budget_s: float):
     steps = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
-        o.run(5)
-        steps += 5
+        o.run(5 if threads > 1 else 1)
+        steps += 5 if threads > 1 else 1
     dt = time.perf_counter() - t0
-    return {"value": nodes * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+    return nodes * steps / dt / 1e6, steps, dt
+
+
+def cpu_baseline(budget_s: float):
+    """The oracle (row-band OpenMP restatement of the rayon path) on a bounded
+    sample of the same workload, with all host threads (max 16), and on a
+    shorter sample with one thread."""
+    threads = min(os.cpu_count() or 1, 16)
+    v, steps, dt = cpu_rate(threads, budget_s)
+    v1, steps1, dt1 = cpu_rate(1, max(budget_s / 4, 1.0))
+    return {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of 256x1024 Ra=1e10 from the reference init "
-                      f"({dt:.1f} s, oracle/cavity_oracle.c, {threads} OpenMP row-band threads)"}
+                      f"({dt:.1f} s, oracle/cavity_oracle.c, {threads} OpenMP row-band threads)",
+            "value_1thread": v1,
+            "sample_1thread": f"{steps1} steps, {dt1:.1f} s, 1 thread",
+            "reference_published": "14.1 (1 thread) -> 32.0 (8 threads) MLUPS, M1 Max"}
+
+
+def copy_bandwidth():
+    """Device-to-device copies of 80 MB (the FP64 state's size) and 1 GiB,
+    timed with HIP events: the practical HBM ceiling on this box (GB/s of
+    read + write bytes)."""
+    import torch
+
+    out = {}
+    for name, nbytes in (("copy_80MB", 79691776), ("copy_1GiB", 1 << 30)):
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = 50 if nbytes < (1 << 28) else 10
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_gbs"] = 2 * nbytes / (ms * 1e-3) / 1e9
+        del a, b
+    return out
+
+
+def precision_run(steps: int):
+    """The precision comparison at 256x1024, Ra=1e10: FP64, FP32 and
+    FP16-offset storage from the same init for `steps` steps.  Reports per
+    form Nu (hot, cold) at the end, the mean Nu_hot over the second half
+    (sampled every 1000 steps), w_max on the mid-height row ny/2
+    (src/main.rs:325-330), and the relative mass drift; the reduced forms
+    also their relative errors against FP64."""
+    nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
+    out = {"steps": steps}
+    for storage in ("fp64", "fp32", "fp16_offset"):
+        s = make_solver(storage)
+        m0 = s.diagnostics()["total_mass"]
+        s.run(steps // 2)
+        acc, n = 0.0, 0
+        for _ in range(steps // 2 // 1000):
+            s.run(1000)
+            acc += s.diagnostics()["nusselt_hot"]
+            n += 1
+        s.run(steps - steps // 2 - 1000 * n)
+        d = s.diagnostics()
+        w, _ = s.max_uy_on_row(WORKLOAD["ny"] // 2)
+        out[storage] = {"nusselt_hot": d["nusselt_hot"], "nusselt_cold": d["nusselt_cold"],
+                        "mean_nusselt_hot_2nd_half": acc / max(n, 1), "w_max_mid": w,
+                        "mass_drift_rel": abs(d["total_mass"] - m0) / m0,
+                        "all_finite": d["all_finite"], "time_step": d["time_step"]}
+        s.close()
+    ref = out["fp64"]
+    for storage in ("fp32", "fp16_offset"):
+        v = out[storage]
+        for key in ("nusselt_hot", "mean_nusselt_hot_2nd_half", "w_max_mid"):
+            v["rel_err_" + key] = abs(v[key] - ref[key]) / abs(ref[key])
+    out["nodes"] = nodes
+    return out
 
 
 def main():
@@ -284,6 +356,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the dynamic-SGS and octree measurements")
     ap.add_argument("--octree-levels", default="2,3", help="octree configurations to measure")
+    ap.add_argument("--precision-steps", type=int, default=200000,
+                    help="steps of the FP64/FP32/FP16-offset comparison (0: skip)")
     ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree",
                     help="paths to measure (a subset prints a partial line, for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -331,6 +405,12 @@ def main():
                 print(f"[bench] octree, {lv} levels", file=sys.stderr, flush=True)
                 extra[f"octree_levels{lv}"] = octree_form(lv, 1000, cs)
 
+    precision = None
+    if "uniform" in paths and a.precision_steps > 0:
+        print(f"[bench] precision forms, {a.precision_steps} steps", file=sys.stderr, flush=True)
+        precision = precision_run(a.precision_steps)
+    copy = copy_bandwidth() if main_form is not None else None
+
     cpu = None
     if not a.no_cpu and main_form is not None:
         print(f"[bench] cpu baseline ({a.cpu_seconds} s)", file=sys.stderr, flush=True)
@@ -375,6 +455,8 @@ def main():
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
         "batched": batched,
+        "precision_forms": precision,
+        "copy_bandwidth": copy,
         "other_paths": ({k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
                              for kk, vv in v.items()} for k, v in extra.items()}
                         if extra else None),
