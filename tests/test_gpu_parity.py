@@ -80,13 +80,17 @@ def test_64x256_ra1e6_bitwise_1000_steps(oracle):
     assert_diag_close(g, o)
 
 
-def test_128x512_ra1e8_bitwise_1000_steps(oracle):
+def test_128x512_ra1e8_bitwise_1k_and_10k_steps(oracle):
+    """The GPU drop-in configuration: 128x512, Ra=1e8, compared at 1k and 10k
+    steps (the 100k-step comparison is tests/long_parity.py)."""
     cfg = dict(SMALL, nx=128, ny=512, ra=1.0e8)
     g, o = pair(oracle, cfg, oracle_jobs=8)
-    g.run(1000)
-    o.run(1000)
-    assert_state_bitwise(g, o, "128x512 Ra1e8 @1000")
-    assert_diag_close(g, o)
+    for upto in (1000, 10000):
+        g.run(upto - g.time_step)
+        o.run(upto - o.time_step)
+        print(f"[parity] 128x512 Ra1e8 at {upto}", flush=True)
+        assert_state_bitwise(g, o, f"128x512 Ra1e8 @{upto}")
+        assert_diag_close(g, o)
 
 
 def test_256x1024_ra1e10_bitwise_short(oracle):
