@@ -187,8 +187,9 @@ def copy_probe(mib: int, reps: int = 5):
 
 def dynamic_form(steps: int, warmup: int, cpu_s: float):
     """The dynamic mixed-scale SGS path (solver.rs:562-828) on the same
-    256x1024 workload, FP64: the model's three passes plus the collision per
-    step, against the oracle's dynamic step on a bounded CPU sample."""
+    256x1024 workload, FP64: one launch per step (streaming, the model and
+    the collision per tile), against the oracle's dynamic step on a bounded
+    CPU sample."""
     from oracle import pyoracle
 
     s = make_solver("fp64")
@@ -197,7 +198,7 @@ def dynamic_form(steps: int, warmup: int, cpu_s: float):
     s.run(warmup)
     ms = s.time_run(steps)
     out = {"steps": steps, "ms_per_step": ms / steps, "mlups": nodes * steps / ms / 1e3,
-           "cs_dyn_mean": s.dynamic_stats()[0]}
+           "launches_per_step": s.dynamic_launches, "cs_dyn_mean": s.dynamic_stats()[0]}
     if cpu_s > 0:
         threads = min(os.cpu_count() or 1, 16)
         w = WORKLOAD

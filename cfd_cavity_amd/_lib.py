@@ -110,6 +110,8 @@ SIGNATURES = {
     "cavlbm_time_step_kernel": (C.c_int, [_vp, _u64, _dp]),
     "cavlbm_set_time_blocking": (C.c_int, [_vp, _i32]),
     "cavlbm_steps_per_launch": (C.c_int, [_vp, C.POINTER(_i32)]),
+    "cavlbm_set_dynamic_fusion": (C.c_int, [_vp, _i32]),
+    "cavlbm_dynamic_launches": (C.c_int, [_vp, C.POINTER(_i32)]),
     "cavlbm_last_error": (C.c_char_p, [_vp]),
     "cavlbm_average_reset": (C.c_int, [_vp]),
     "cavlbm_average_accumulate": (C.c_int, [_vp]),

@@ -59,7 +59,8 @@ struct cavlbm_solver {
   // by starting buffer parity cur | (fy parity << 1)
   hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   int fy_par = 0;  // swaps of fy / fy_alt so far, mod 2
-  hipGraphExec_t dgraph[2] = {nullptr, nullptr};  // dynamic-SGS steps, by parity
+  // dynamic-SGS steps, by starting buffer parity cur | (fy parity << 1)
+  hipGraphExec_t dgraph[4] = {nullptr, nullptr, nullptr, nullptr};
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
@@ -75,6 +76,7 @@ struct cavlbm_solver {
   bool dyn_on = false, dyn_pre = false;
   double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
   double* dyn = nullptr;  // kDynPlanes FP64 planes, allocated on first enable
+  bool dyn_fuse = true;   // one launch per dynamic step where it applies
   std::string err;
 };
 
@@ -211,6 +213,11 @@ void drop_graphs(cavlbm_solver* s) {
     }
 }
 
+void swap_fy(cavlbm_solver* s) {
+  std::swap(s->fy, s->fy_alt);
+  s->fy_par ^= 1;
+}
+
 // Scratch layout of the dynamic model, planes of s->dyn: ubx, uby, tb, 5 box
 // products, qsm, phi, 4 Lilly terms, nu_mix, kappa_mix, C_nu, C_kappa,
 // nu_t_pre, alpha_t_pre (indices 0..19).
@@ -255,13 +262,22 @@ DynSgsArgs dyn_args(const cavlbm_solver* s, const StepCtx& ctx) {
   return a;
 }
 
-// The launches of one dynamic step from the C-form: the post-stream f planes
-// and fields the model reads (MODE_MATERIALIZE_F, skipped when the M-form is
-// current), the model's filter and terms, and the collision, which pulls the
-// C-form again, runs the coefficient stage per node and stores its planes
-// with the auxiliary outputs (MODE_FUSED_DYN).
+// The launches of one dynamic step from the C-form.  FP64 storage with a
+// test filter of 1..3 nodes: one launch (k_dyn_step), which reads fy and
+// writes fy_alt; the caller swaps them (fy_swap).  Otherwise: the post-stream
+// f planes and fields the model reads (MODE_MATERIALIZE_F, skipped when the
+// M-form is current), the model's filter and terms, and the collision, which
+// pulls the C-form again, runs the coefficient stage per node and stores its
+// planes with the auxiliary outputs (MODE_FUSED_DYN).
 hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgsArgs& args,
-                              int write_aux) {
+                              int write_aux, bool& fy_swap) {
+  fy_swap = false;
+  if (s->storage == ST_FP64 && s->dyn_fuse && dyn_step_fits(args.width)) {
+    fy_swap = true;
+    return launch_dyn_step_fp64(s->geq, s->geo, ctx, Walls{s->theta}, s->cpop[s->cur],
+                                s->cpop[s->cur ^ 1], s->fy, s->fy_alt, s->nut, args, write_aux,
+                                s->st);
+  }
   hipError_t e = hipSuccess;
   if (!s->m_valid) e = launch(s, MODE_MATERIALIZE_F, ctx, 0);
   if (e == hipSuccess) e = launch_dynamic_sgs(s->geo, args, false, s->st);
@@ -280,7 +296,9 @@ hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgs
 int dyn_step(cavlbm_solver* s, int write_aux) {
   const StepCtx ctx = make_ctx(s, s->time_step);
   if (s->dyn_on && s->c_valid) {
-    HIP_TRY(s, dyn_fused_launches(s, ctx, dyn_args(s, ctx), write_aux));
+    bool fy_swap;
+    HIP_TRY(s, dyn_fused_launches(s, ctx, dyn_args(s, ctx), write_aux, fy_swap));
+    if (fy_swap) swap_fy(s);
     s->dyn_pre = true;
     s->cur ^= 1;
     s->m_valid = false;
@@ -312,11 +330,6 @@ int one_step(cavlbm_solver* s, int write_aux) {
   s->m_valid = false;
   s->time_step += 1;
   return 0;
-}
-
-void swap_fy(cavlbm_solver* s) {
-  std::swap(s->fy, s->fy_alt);
-  s->fy_par ^= 1;
 }
 
 // n steps in one launch: collide(time_step + j), stream, ... (n >= 2).
@@ -368,24 +381,31 @@ hipGraphExec_t capture(cavlbm_solver* s) {
   return ex;
 }
 
-// Captures kGraphSteps dynamic-SGS steps (dyn_fused_launches each) starting
-// from parity `par`.
-hipGraphExec_t capture_dyn(cavlbm_solver* s, int par) {
+// Captures kGraphSteps dynamic-SGS steps (dyn_fused_launches each) from the
+// current buffers (cur, fy); kGraphSteps is even, so the replay ends on them.
+hipGraphExec_t capture_dyn(cavlbm_solver* s) {
   const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);
   const DynSgsArgs args = dyn_args(s, ctx);
   const int saved = s->cur;
+  void* const fy_saved = s->fy;
+  void* const fy_alt_saved = s->fy_alt;
+  const int fy_par_saved = s->fy_par;
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-  s->cur = par;
   bool ok = true;
   const bool m_saved = s->m_valid;
   for (int i = 0; i < kGraphSteps && ok; ++i) {
     s->m_valid = false;  // every captured step materializes its f planes
-    ok = dyn_fused_launches(s, ctx, args, 0) == hipSuccess;
+    bool fy_swap;
+    ok = dyn_fused_launches(s, ctx, args, 0, fy_swap) == hipSuccess;
+    if (fy_swap) swap_fy(s);
     s->cur ^= 1;
   }
   s->cur = saved;
+  s->fy = fy_saved;
+  s->fy_alt = fy_alt_saved;
+  s->fy_par = fy_par_saved;
   s->m_valid = m_saved;
   if (hipStreamEndCapture(s->st, &g) != hipSuccess || !ok) {
     if (g) (void)hipGraphDestroy(g);
@@ -404,8 +424,8 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
     // dynamic-SGS steps after the spin-up: graphs of kGraphSteps steps
     if (s->dyn_on && s->c_valid && !s->graph_failed && s->time_step >= s->d.spin_up_steps &&
         left > (uint64_t)kGraphSteps) {
-      hipGraphExec_t& g = s->dgraph[s->cur];
-      if (!g) g = capture_dyn(s, s->cur);
+      hipGraphExec_t& g = s->dgraph[s->cur | (s->fy_par << 1)];
+      if (!g) g = capture_dyn(s);
       if (!g) {
         s->graph_failed = true;
         continue;
@@ -1036,6 +1056,26 @@ int cavlbm_set_time_blocking(cavlbm_solver* s, int32_t steps) {
 int cavlbm_steps_per_launch(const cavlbm_solver* s, int32_t* out) {
   if (!s || !out) return CAVLBM_E_PARAM;
   *out = block_steps(s, kGraphSteps, true);
+  return 0;
+}
+
+int cavlbm_set_dynamic_fusion(cavlbm_solver* s, int32_t one_launch) {
+  if (!s) return CAVLBM_E_PARAM;
+  if (s->dyn_fuse != (one_launch != 0)) {
+    HIP_TRY(s, hipSetDevice(s->p.device));
+    drop_graphs(s);
+    s->graph_failed = false;
+    s->dyn_fuse = one_launch != 0;
+  }
+  return 0;
+}
+
+int cavlbm_dynamic_launches(const cavlbm_solver* s, int32_t* out) {
+  if (!s || !out) return CAVLBM_E_PARAM;
+  const StepCtx ctx = make_ctx(s, s->time_step);
+  const bool one =
+      s->storage == ST_FP64 && s->dyn_fuse && dyn_step_fits(dyn_args(s, ctx).width);
+  *out = one ? 1 : 4;
   return 0;
 }
 

@@ -125,25 +125,36 @@ __device__ __forceinline__ double dyn_lilly(double num, double den) {
   return 0.0;
 }
 
-// nu_t and alpha_t of node (ix, iy) (solver.rs:787-815); with `store` the
-// coefficients and the floored values are also written to their planes.
+// The coefficient stage from the boxed contractions of one node: regularised
+// Lilly coefficients (solver.rs:787-796), floored turbulent viscosity and
+// diffusivity (:810-815).
+__device__ __forceinline__ void dyn_coef_vals(double num_v, double den_v, double num_k,
+                                              double den_k, double nu_mix_g, double kap_mix_g,
+                                              double nu_t_min, double alpha_t_min, double& cnu,
+                                              double& ck, double& nut, double& at) {
+  cnu = dyn_lilly(num_v, den_v);
+  ck = dyn_lilly(num_k, den_k);
+  const double nt = cnu * nu_mix_g;
+  nut = nt < nu_t_min ? nu_t_min : nt;
+  const double a = ck * kap_mix_g;
+  at = a < alpha_t_min ? alpha_t_min : a;
+}
+
+// nu_t and alpha_t of node (ix, iy) from the contraction planes; with `store`
+// the coefficients and the floored values are also written to their planes.
 __device__ __forceinline__ void dyn_coef_node(const Geom& geo, const DynCoef& d, int ix, int iy,
                                               bool store, double& nut, double& at) {
   const int k = iy * geo.pitch + ix;
   const long long P = geo.plane;
   const DynWin w = dyn_window(geo, d.width, d.half_left, ix, iy);
-  // regularised Lilly coefficients (:787-796), one sum per contraction
+  // one box sum per contraction
   const double num_v = dyn_box(geo, w, d.terms + 0 * P);
   const double den_v = dyn_box(geo, w, d.terms + 1 * P);
   const double num_k = dyn_box(geo, w, d.terms + 2 * P);
   const double den_k = dyn_box(geo, w, d.terms + 3 * P);
-  const double cnu = dyn_lilly(num_v, den_v);
-  const double ck = dyn_lilly(num_k, den_k);
-  // floored turbulent viscosity and diffusivity (:810-815)
-  const double nt = cnu * d.nu_mix_g[k];
-  nut = nt < d.nu_t_min ? d.nu_t_min : nt;
-  const double a = ck * d.kap_mix_g[k];
-  at = a < d.alpha_t_min ? d.alpha_t_min : a;
+  double cnu, ck;
+  dyn_coef_vals(num_v, den_v, num_k, den_k, d.nu_mix_g[k], d.kap_mix_g[k], d.nu_t_min,
+                d.alpha_t_min, cnu, ck, nut, at);
   if (store) {
     d.cnu[k] = cnu;
     d.ck[k] = ck;

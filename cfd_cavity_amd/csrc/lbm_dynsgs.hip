@@ -2,10 +2,13 @@
 // (reference: src/uniform/solver.rs:562-828 compute_dynamic_sgs, helpers
 // box_filter :1204-1224, ddx/ddy :1229-1242, lilly_coefficient :1251-1261).
 //
+// FP64, built with -ffp-contract=off so that every sum, product, division and
+// square root is the IEEE result in the reference's operand order.  The
+// model's per-node arithmetic is written once (dyn_similarity,
+// dyn_grid_strain, dyn_terms_node, dyn_coef_vals) and reached from two forms:
+//
 // Three passes over the M-form state (post-stream populations and moments),
-// each one thread per node, FP64, built with -ffp-contract=off so that every
-// sum, product, division and square root is the IEEE result in the
-// reference's operand order:
+// one thread per node, planes in HBM:
 //   k_dyn_filter : test-filter (box of `width` nodes, clamped one-sided at the
 //                  walls) of u, v, T and of the products uu, uv, vv, uT, vT;
 //                  the node-local similarity scales qsm^1/2 and Phi_sm^1/2;
@@ -16,10 +19,19 @@
 //                  contractions;
 //   k_dyn_coeffs : box of the contractions, C = clamp(num/(2 den), 0, 0.04),
 //                  nu_t = C_nu * nu_mix, alpha_t = C_kappa * kappa_mix, both
-//                  floored (dyn_coef_node; the solver's dynamic step runs it
+//                  floored (dyn_coef_node; a step from the C-form runs it
 //                  inside the collision instead, MODE_FUSED_DYN).
-// The box sums run in the reference's (y outer, x inner) order from +0.
 //
+// One launch per dynamic step from the C-form (k_dyn_step, FP64 storage,
+// test-filter widths 1..3): each workgroup pulls the populations of its tile
+// grown by the model's stencil (the filter windows, the centred differences
+// and the box of the contractions), runs the filter, the similarity scales,
+// the contractions and the coefficient box through LDS, then pulls its own
+// nodes again (from L2) and collides and stores them.  Tiles read the force
+// plane fy of their halo nodes, so the step reads one fy plane and writes the
+// other.
+//
+// The box sums run in the reference's (y outer, x inner) order from +0.
 // The mixed forms a^alpha b^(1-alpha) use pow; the reference's powf is the C
 // library's pow, which is not correctly rounded, so this path agrees with a
 // CPU evaluation to a few ulp per node rather than bitwise.  For alpha = 0.5
@@ -30,29 +42,8 @@ namespace cavlbm {
 
 namespace {
 
-__device__ __forceinline__ int node_of_thread(const Geom& geo, int& ix, int& iy) {
-  const int bx = (int)blockIdx.x % geo.nbx, by = (int)blockIdx.x / geo.nbx;
-  ix = bx * kTileX + (int)(threadIdx.x % kTileX);
-  iy = by * kTileY + (int)(threadIdx.x / kTileX);
-  return ix < geo.nx && iy < geo.ny;
-}
-
-__device__ __forceinline__ DynWin window(const Geom& geo, const DynSgsArgs& a, int ix, int iy) {
-  return dyn_window(geo, a.width, a.half_left, ix, iy);
-}
-
-// Centred differences with one-sided clamping at the walls (solver.rs:1229-1242).
-__device__ __forceinline__ double ddx(const Geom& geo, const double* f, int ix, int iy) {
-  const int xm = ix == 0 ? 0 : ix - 1;
-  const int xp = ix + 1 >= geo.nx ? geo.nx - 1 : ix + 1;
-  return 0.5 * (f[iy * geo.pitch + xp] - f[iy * geo.pitch + xm]);
-}
-
-__device__ __forceinline__ double ddy(const Geom& geo, const double* f, int ix, int iy) {
-  const int ym = iy == 0 ? 0 : iy - 1;
-  const int yp = iy + 1 >= geo.ny ? geo.ny - 1 : iy + 1;
-  return 0.5 * (f[yp * geo.pitch + ix] - f[ym * geo.pitch + ix]);
-}
+// ---------------------------------------------------------------------------
+// Per-node pieces of the model.
 
 // x^e for the mixed forms; every base here is >= +0.
 __device__ __forceinline__ double mixpow(double x, double e) {
@@ -60,11 +51,187 @@ __device__ __forceinline__ double mixpow(double x, double e) {
   return pow(x, e);
 }
 
+// Box mean of an accessor f(x, y) over a window, (y outer, x inner) from +0.
+template <class Acc>
+__device__ __forceinline__ double box_mean(const DynWin& w, const Acc& f) {
+  double s = 0.0;
+  for (int yy = w.y0; yy < w.y1; ++yy)
+    for (int xx = w.x0; xx < w.x1; ++xx) s += f(xx, yy);
+  return s / w.cnt;
+}
+
+// Centred differences with one-sided clamping at the walls (solver.rs:1229-1242).
+template <class Acc>
+__device__ __forceinline__ double ddx(const Acc& f, int nx, int ix, int iy) {
+  const int xm = ix == 0 ? 0 : ix - 1;
+  const int xp = ix + 1 >= nx ? nx - 1 : ix + 1;
+  return 0.5 * (f(xp, iy) - f(xm, iy));
+}
+
+template <class Acc>
+__device__ __forceinline__ double ddy(const Acc& f, int ny, int ix, int iy) {
+  const int ym = iy == 0 ? 0 : iy - 1;
+  const int yp = iy + 1 >= ny ? ny - 1 : iy + 1;
+  return 0.5 * (f(ix, yp) - f(ix, ym));
+}
+
+// Similarity scales of one node from its resolved and test-filtered values
+// (solver.rs:688-698).
+__device__ __forceinline__ void dyn_similarity(double ux, double uy, double t, double ubx,
+                                               double uby, double tb, double& qsm, double& phi) {
+  const double dux = ux - ubx, duy = uy - uby;
+  const double du2 = dux * dux + duy * duy;
+  qsm = sqrt(0.5 * du2) / 2.0;
+  const double dt_local = t - tb;
+  phi = 0.5 * fabs(dt_local) * sqrt(0.5 * du2);
+}
+
+// S~ and |S~| from pi_neq of the post-stream populations with collide_node's
+// formulas (solver.rs:638-657).  With c in {-1,0,1} the products fneq*c*c
+// are exact and the zero ones leave the +0-started accumulators unchanged.
+struct DynStrain {
+  double xx, xy, yy, mag;
+};
+
+__device__ __forceinline__ DynStrain dyn_grid_strain(const double (&fp)[9], double rho_k,
+                                                     double ux_k, double uy_k, double tau) {
+  EqTerms<double> e;
+  eq_terms<double>(ux_k, uy_k, e);
+  double d[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = fp[i] - eq_quad<double>(i, rho_k, e);
+  const double pi_xx = 0.0 + d[1] + d[2] + d[4] + d[5] + d[6] + d[8];
+  const double pi_xy = 0.0 + d[2] - d[4] + d[6] - d[8];
+  const double pi_yy = 0.0 + d[2] + d[3] + d[4] + d[6] + d[7] + d[8];
+  const double rho_m = fmax(rho_k, 1e-12);
+  const double denom = 2.0 * kCs2 * tau * rho_m;
+  const double pi2 = pi_xx * pi_xx + 2.0 * pi_xy * pi_xy + pi_yy * pi_yy;
+  DynStrain s;
+  s.xx = -pi_xx / denom;
+  s.xy = -pi_xy / denom;
+  s.yy = -pi_yy / denom;
+  s.mag = sqrt(pi2) / denom;
+  return s;
+}
+
+// Inputs of the contractions at one node besides the grid-level strain.
+struct DynLocal {
+  double sbxx, sbyy, sbxy;      // S- (centred differences of the filtered u, v)
+  double gtx, gty, gbx, gby;    // resolved and filtered temperature gradients
+  double qsm_h, phi_h;          // similarity scales of the node
+  double qbar_h, phibar_h;      // and their test-filtered values
+  double ubx, uby, tb;          // test-filtered u, v, T
+  double lb[5];                 // box of uu, uv, vv, uT, vT
+};
+
+// The four Lilly contractions and the grid-level mixed forms of one node
+// (solver.rs:662-775).
+struct DynTerms {
+  double t[4];  // L:M, M:M, Lj Mj, Mj Mj
+  double nu_mix_g, kap_mix_g;
+};
+
+__device__ __forceinline__ DynTerms dyn_terms_node(const DynStrain& sh, const DynLocal& q,
+                                                   const DynSgsArgs& a) {
+  const double shxx = sh.xx, shxy = sh.xy, shyy = sh.yy, smag_hat = sh.mag;
+  const double sbxx = q.sbxx, sbyy = q.sbyy, sbxy = q.sbxy;
+  const double smag_bar = sqrt(sbxx * sbxx + 2.0 * sbxy * sbxy + sbyy * sbyy);
+  const double gtx = q.gtx, gty = q.gty, gbx = q.gbx, gby = q.gby;
+
+  // |T~| and |T-| (:702-707)
+  const double th_x = gtx * shxx + gty * shxy;
+  const double th_y = gtx * shxy + gty * shyy;
+  const double tmag_hat = sqrt(2.0 * (th_x * th_x + th_y * th_y));
+  const double tb_x = gbx * sbxx + gby * sbxy;
+  const double tb_y = gbx * sbxy + gby * sbyy;
+  const double tmag_bar = sqrt(2.0 * (tb_x * tb_x + tb_y * tb_y));
+
+  // interpolated mixed forms at both levels (:733-740)
+  const double dt = a.dtest, dx = 1.0, del = kDeltaT;
+  const double ae = a.alpha_exp, be = a.beta_exp;
+  const double nu_mix_t = mixpow(dt * dt * smag_bar, ae) * mixpow(dt * q.qbar_h, be);
+  const double nu_mix_g = mixpow(dx * dx * smag_hat, ae) * mixpow(dx * q.qsm_h, be);
+  const double kap_mix_t =
+      mixpow(dt * dt * dt / del * tmag_bar, ae) * mixpow(dt / del * q.phibar_h, be);
+  const double kap_mix_g =
+      mixpow(dx * dx * dx / del * tmag_hat, ae) * mixpow(dx / del * q.phi_h, be);
+
+  // Germano M (:748-755), Leonard L at the test filter (:760-769)
+  const double mxx = nu_mix_t * sbxx - nu_mix_g * shxx;
+  const double mxy = nu_mix_t * sbxy - nu_mix_g * shxy;
+  const double myy = nu_mix_t * sbyy - nu_mix_g * shyy;
+  const double mx = kap_mix_t * gbx - kap_mix_g * gtx;
+  const double my = kap_mix_t * gby - kap_mix_g * gty;
+  const double ubx = q.ubx, uby = q.uby, tb = q.tb;
+  const double lxx = q.lb[0] - ubx * ubx;
+  const double lxy = q.lb[1] - ubx * uby;
+  const double lyy = q.lb[2] - uby * uby;
+  const double lkk3 = (lxx + lyy) / 3.0;
+  const double ldxx = lxx - lkk3;
+  const double ldyy = lyy - lkk3;
+  const double ljx = q.lb[3] - ubx * tb;
+  const double ljy = q.lb[4] - uby * tb;
+
+  // Lilly contractions (:772-775)
+  DynTerms r;
+  r.t[0] = ldxx * mxx + 2.0 * lxy * mxy + ldyy * myy;
+  r.t[1] = mxx * mxx + 2.0 * mxy * mxy + myy * myy;
+  r.t[2] = ljx * mx + ljy * my;
+  r.t[3] = mx * mx + my * my;
+  r.nu_mix_g = nu_mix_g;
+  r.kap_mix_g = kap_mix_g;
+  return r;
+}
+
+// The S-, gradient and box inputs of node (ix, iy) from accessors of the
+// resolved fields (u, v, T), the filtered fields (ub, vb, tb) and the
+// similarity scales (qs, ph); lb is filled by the caller.
+template <class AR, class AF, class AQ>
+__device__ __forceinline__ void dyn_local(const Geom& geo, const DynWin& w, int ix, int iy,
+                                          const AR& rt, const AF& fu, const AF& fv,
+                                          const AF& ft, const AQ& qs, const AQ& ph,
+                                          DynLocal& q) {
+  const int nx = geo.nx, ny = geo.ny;
+  // S- from centred differences of the test-filtered velocity (:662-667)
+  q.sbxx = ddx(fu, nx, ix, iy);
+  q.sbyy = ddy(fv, ny, ix, iy);
+  q.sbxy = 0.5 * (ddx(fv, nx, ix, iy) + ddy(fu, ny, ix, iy));
+  // temperature gradients, resolved and test-filtered (:671-674)
+  q.gtx = ddx(rt, nx, ix, iy);
+  q.gty = ddy(rt, ny, ix, iy);
+  q.gbx = ddx(ft, nx, ix, iy);
+  q.gby = ddy(ft, ny, ix, iy);
+  q.qsm_h = qs(ix, iy);
+  q.phi_h = ph(ix, iy);
+  // test-filtered similarity scales (:713-714)
+  q.qbar_h = box_mean(w, qs);
+  q.phibar_h = box_mean(w, ph);
+  q.ubx = fu(ix, iy);
+  q.uby = fv(ix, iy);
+  q.tb = ft(ix, iy);
+}
+
+// ---------------------------------------------------------------------------
+// Three-pass form.
+
+struct Plane {
+  const double* p;
+  int pitch;
+  __device__ __forceinline__ double operator()(int x, int y) const { return p[y * pitch + x]; }
+};
+
+__device__ __forceinline__ int node_of_thread(const Geom& geo, int& ix, int& iy) {
+  const int bx = (int)blockIdx.x % geo.nbx, by = (int)blockIdx.x / geo.nbx;
+  ix = bx * kTileX + (int)(threadIdx.x % kTileX);
+  iy = by * kTileY + (int)(threadIdx.x / kTileX);
+  return ix < geo.nx && iy < geo.ny;
+}
+
 __global__ __launch_bounds__(kBlock) void k_dyn_filter(Geom geo, DynSgsArgs a) {
   int ix, iy;
   if (!node_of_thread(geo, ix, iy)) return;
   const int k = iy * geo.pitch + ix;
-  const DynWin w = window(geo, a, ix, iy);
+  const DynWin w = dyn_window(geo, a.width, a.half_left, ix, iy);
   double s_u = 0.0, s_v = 0.0, s_t = 0.0, s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0,
          s_vt = 0.0;
   for (int yy = w.y0; yy < w.y1; ++yy)
@@ -89,12 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_dyn_filter(Geom geo, DynSgsArgs a) {
   a.lb[2 * geo.plane + k] = s_vv / w.cnt;
   a.lb[3 * geo.plane + k] = s_ut / w.cnt;
   a.lb[4 * geo.plane + k] = s_vt / w.cnt;
-  // similarity scales (solver.rs:688-698)
-  const double dux = a.ux[k] - ubx, duy = a.uy[k] - uby;
-  const double du2 = dux * dux + duy * duy;
-  a.qsm[k] = sqrt(0.5 * du2) / 2.0;
-  const double dt_local = a.temp[k] - tb;
-  a.phi[k] = 0.5 * fabs(dt_local) * sqrt(0.5 * du2);
+  dyn_similarity(a.ux[k], a.uy[k], a.temp[k], ubx, uby, tb, a.qsm[k], a.phi[k]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
@@ -102,83 +264,22 @@ __global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
   if (!node_of_thread(geo, ix, iy)) return;
   const int k = iy * geo.pitch + ix;
   const long long P = geo.plane;
-  const double rho_k = a.rho[k], ux_k = a.ux[k], uy_k = a.uy[k];
-
-  // S~ and |S~| from pi_neq with collide_node's formulas (solver.rs:638-657).
-  // With c in {-1,0,1} the products fneq*c*c are exact and the zero ones
-  // leave the +0-started accumulators unchanged.
-  EqTerms<double> e;
-  eq_terms<double>(ux_k, uy_k, e);
-  double d[9];
+  double fp[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) d[i] = a.mpops[i * P + k] - eq_quad<double>(i, rho_k, e);
-  const double pi_xx = 0.0 + d[1] + d[2] + d[4] + d[5] + d[6] + d[8];
-  const double pi_xy = 0.0 + d[2] - d[4] + d[6] - d[8];
-  const double pi_yy = 0.0 + d[2] + d[3] + d[4] + d[6] + d[7] + d[8];
-  const double rho_m = fmax(rho_k, 1e-12);
-  const double denom = 2.0 * kCs2 * a.tau * rho_m;
-  const double pi2 = pi_xx * pi_xx + 2.0 * pi_xy * pi_xy + pi_yy * pi_yy;
-  const double shxx = -pi_xx / denom, shxy = -pi_xy / denom, shyy = -pi_yy / denom;
-  const double smag_hat = sqrt(pi2) / denom;
-
-  // S- from centred differences of the test-filtered velocity (:662-667)
-  const double sbxx = ddx(geo, a.ubx, ix, iy);
-  const double sbyy = ddy(geo, a.uby, ix, iy);
-  const double sbxy = 0.5 * (ddx(geo, a.uby, ix, iy) + ddy(geo, a.ubx, ix, iy));
-  const double smag_bar = sqrt(sbxx * sbxx + 2.0 * sbxy * sbxy + sbyy * sbyy);
-
-  // temperature gradients, resolved and test-filtered (:671-674)
-  const double gtx = ddx(geo, a.temp, ix, iy), gty = ddy(geo, a.temp, ix, iy);
-  const double gbx = ddx(geo, a.tb, ix, iy), gby = ddy(geo, a.tb, ix, iy);
-
-  const double qsm_h = a.qsm[k], phi_h = a.phi[k];
-
-  // |T~| and |T-| (:702-707)
-  const double th_x = gtx * shxx + gty * shxy;
-  const double th_y = gtx * shxy + gty * shyy;
-  const double tmag_hat = sqrt(2.0 * (th_x * th_x + th_y * th_y));
-  const double tb_x = gbx * sbxx + gby * sbxy;
-  const double tb_y = gbx * sbxy + gby * sbyy;
-  const double tmag_bar = sqrt(2.0 * (tb_x * tb_x + tb_y * tb_y));
-
-  // test-filtered similarity scales (:713-714)
-  const DynWin w = window(geo, a, ix, iy);
-  const double qbar_h = dyn_box(geo, w, a.qsm);
-  const double phibar_h = dyn_box(geo, w, a.phi);
-
-  // interpolated mixed forms at both levels (:733-740)
-  const double dt = a.dtest, dx = 1.0, del = kDeltaT;
-  const double ae = a.alpha_exp, be = a.beta_exp;
-  const double nu_mix_t = mixpow(dt * dt * smag_bar, ae) * mixpow(dt * qbar_h, be);
-  const double nu_mix_g = mixpow(dx * dx * smag_hat, ae) * mixpow(dx * qsm_h, be);
-  const double kap_mix_t =
-      mixpow(dt * dt * dt / del * tmag_bar, ae) * mixpow(dt / del * phibar_h, be);
-  const double kap_mix_g =
-      mixpow(dx * dx * dx / del * tmag_hat, ae) * mixpow(dx / del * phi_h, be);
-
-  // Germano M (:748-755), Leonard L at the test filter (:760-769)
-  const double mxx = nu_mix_t * sbxx - nu_mix_g * shxx;
-  const double mxy = nu_mix_t * sbxy - nu_mix_g * shxy;
-  const double myy = nu_mix_t * sbyy - nu_mix_g * shyy;
-  const double mx = kap_mix_t * gbx - kap_mix_g * gtx;
-  const double my = kap_mix_t * gby - kap_mix_g * gty;
-  const double ubx = a.ubx[k], uby = a.uby[k], tb = a.tb[k];
-  const double lxx = a.lb[0 * P + k] - ubx * ubx;
-  const double lxy = a.lb[1 * P + k] - ubx * uby;
-  const double lyy = a.lb[2 * P + k] - uby * uby;
-  const double lkk3 = (lxx + lyy) / 3.0;
-  const double ldxx = lxx - lkk3;
-  const double ldyy = lyy - lkk3;
-  const double ljx = a.lb[3 * P + k] - ubx * tb;
-  const double ljy = a.lb[4 * P + k] - uby * tb;
-
-  // Lilly contractions (:772-775)
-  a.terms[0 * P + k] = ldxx * mxx + 2.0 * lxy * mxy + ldyy * myy;
-  a.terms[1 * P + k] = mxx * mxx + 2.0 * mxy * mxy + myy * myy;
-  a.terms[2 * P + k] = ljx * mx + ljy * my;
-  a.terms[3 * P + k] = mx * mx + my * my;
-  a.nu_mix_g[k] = nu_mix_g;
-  a.kap_mix_g[k] = kap_mix_g;
+  for (int i = 0; i < 9; ++i) fp[i] = a.mpops[i * P + k];
+  const DynStrain sh = dyn_grid_strain(fp, a.rho[k], a.ux[k], a.uy[k], a.tau);
+  const int pitch = geo.pitch;
+  const DynWin w = dyn_window(geo, a.width, a.half_left, ix, iy);
+  DynLocal q;
+  dyn_local(geo, w, ix, iy, Plane{a.temp, pitch}, Plane{a.ubx, pitch}, Plane{a.uby, pitch},
+            Plane{a.tb, pitch}, Plane{a.qsm, pitch}, Plane{a.phi, pitch}, q);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) q.lb[j] = a.lb[j * P + k];
+  const DynTerms r = dyn_terms_node(sh, q, a);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) a.terms[j * P + k] = r.t[j];
+  a.nu_mix_g[k] = r.nu_mix_g;
+  a.kap_mix_g[k] = r.kap_mix_g;
 }
 
 __global__ __launch_bounds__(kBlock) void k_dyn_coeffs(Geom geo, DynSgsArgs a) {
@@ -186,6 +287,257 @@ __global__ __launch_bounds__(kBlock) void k_dyn_coeffs(Geom geo, DynSgsArgs a) {
   if (!node_of_thread(geo, ix, iy)) return;
   double nut, at;
   dyn_coef_node(geo, dyn_coef_of(a), ix, iy, true, nut, at);
+}
+
+// ---------------------------------------------------------------------------
+// One-launch form.
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Node sets of a tile of TX x TY output nodes for the test-filter width W,
+// as extensions below / above the tile (the same in x and y; windows are
+// [i - kHL, i + kHR]):
+//   T  the contractions, boxed by the outputs' windows;
+//   Q  the similarity scales, boxed by T's windows;
+//   F  the filtered fields, differenced at T +- 1 and used by Q;
+//   R  the pulled nodes, boxed by F's windows.
+// LDS holds u, v, T on R, ub, vb, tb on F, qsm, Phi on Q and the grid-level
+// strain on T; after the contractions are computed, they and the mixed forms
+// (6 planes on T) overwrite the R and F planes.
+template <int W, int TX, int TY>
+struct DynStepTile {
+  static constexpr int kNT = TX * TY;
+  static constexpr int kHL = (W - 1) / 2, kHR = W - 1 - kHL;
+  static constexpr int kTlo = kHL, kThi = kHR;
+  static constexpr int kQlo = 2 * kHL, kQhi = 2 * kHR;
+  static constexpr int kFlo = cmax(kHL + 1, 2 * kHL), kFhi = cmax(kHR + 1, 2 * kHR);
+  static constexpr int kRlo = kFlo + kHL, kRhi = kFhi + kHR;
+  static constexpr int kRW = TX + kRlo + kRhi, kRH = TY + kRlo + kRhi, kR = kRW * kRH;
+  static constexpr int kFW = TX + kFlo + kFhi, kFH = TY + kFlo + kFhi, kF = kFW * kFH;
+  static constexpr int kQW = TX + kQlo + kQhi, kQH = TY + kQlo + kQhi, kQ = kQW * kQH;
+  static constexpr int kTW = TX + kTlo + kThi, kTH = TY + kTlo + kThi, kT = kTW * kTH;
+  static constexpr int kTPer = (kT + kNT - 1) / kNT;  // contraction nodes per thread
+  // LDS planes (doubles): R at 0, F at kOffF, Q at kOffQ, strain at kOffS;
+  // contractions and mixed forms at 0 once R and F are dead.
+  static constexpr int kOffF = 3 * kR, kOffQ = kOffF + 3 * kF, kOffS = kOffQ + 2 * kQ;
+  static constexpr int kLds = kOffS + 4 * kT;
+  static_assert(6 * kT <= kOffQ, "contractions must fit over the R and F planes");
+};
+
+// LDS plane accessor in global node coordinates: plane of width `w` whose
+// local (0, 0) is the global node (x0, y0).
+struct LdsPlane {
+  const double* p;
+  int x0, y0, w;
+  __device__ __forceinline__ double operator()(int x, int y) const {
+    return p[(y - y0) * w + (x - x0)];
+  }
+};
+
+template <int GEQ, int W, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, StepCtx ctx,
+                                                      Walls walls, const double* __restrict__ src,
+                                                      double* __restrict__ dst,
+                                                      const double* __restrict__ fy_in,
+                                                      double* __restrict__ fy_out,
+                                                      double* __restrict__ nut, DynSgsArgs a,
+                                                      DynCoef dc, int write_aux) {
+  using DT = DynStepTile<W, TX, TY>;
+  __shared__ double sh[DT::kLds];
+  int bx, by;
+  tile_of_block(tiles, bx, by);
+  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
+  const unsigned P32 = (unsigned)geo.plane;
+  const int ox = bx * TX, oy = by * TY;  // first output node of the tile
+  const int t = (int)threadIdx.x;
+  // every pulled node at least one node inside the walls: no boundary handling
+  const bool interior = ox - DT::kRlo >= 1 && ox + TX - 1 + DT::kRhi <= nx - 2 &&
+                        oy - DT::kRlo >= 1 && oy + TY - 1 + DT::kRhi <= ny - 2;
+
+  const int rx0 = ox - DT::kRlo, ry0 = oy - DT::kRlo;
+  const int fx0 = ox - DT::kFlo, fy0 = oy - DT::kFlo;
+  const int qx0 = ox - DT::kQlo, qy0 = oy - DT::kQlo;
+  const int tx0 = ox - DT::kTlo, ty0 = oy - DT::kTlo;
+  const LdsPlane ru{sh + 0 * DT::kR, rx0, ry0, DT::kRW};
+  const LdsPlane rv{sh + 1 * DT::kR, rx0, ry0, DT::kRW};
+  const LdsPlane rt{sh + 2 * DT::kR, rx0, ry0, DT::kRW};
+  const LdsPlane fu{sh + DT::kOffF + 0 * DT::kF, fx0, fy0, DT::kFW};
+  const LdsPlane fv{sh + DT::kOffF + 1 * DT::kF, fx0, fy0, DT::kFW};
+  const LdsPlane ft{sh + DT::kOffF + 2 * DT::kF, fx0, fy0, DT::kFW};
+  const LdsPlane qs{sh + DT::kOffQ + 0 * DT::kQ, qx0, qy0, DT::kQW};
+  const LdsPlane ph{sh + DT::kOffQ + 1 * DT::kQ, qx0, qy0, DT::kQW};
+
+  // Pulled node (x, y): its fields on R and, inside T, its grid-level strain.
+  auto put_node = [&](int x, int y, const double (&fp)[9], double r, double u, double v,
+                      double tt) {
+    const int ir = (y - ry0) * DT::kRW + (x - rx0);
+    sh[0 * DT::kR + ir] = u;
+    sh[1 * DT::kR + ir] = v;
+    sh[2 * DT::kR + ir] = tt;
+    const int lx = x - tx0, ly = y - ty0;
+    if (lx >= 0 && lx < DT::kTW && ly >= 0 && ly < DT::kTH) {
+      const DynStrain s = dyn_grid_strain(fp, r, u, v, a.tau);
+      const int it = ly * DT::kTW + lx;
+      sh[DT::kOffS + 0 * DT::kT + it] = s.xx;
+      sh[DT::kOffS + 1 * DT::kT + it] = s.xy;
+      sh[DT::kOffS + 2 * DT::kT + it] = s.yy;
+      sh[DT::kOffS + 3 * DT::kT + it] = s.mag;
+    }
+  };
+
+  // 1. streaming of every node of R that lies in the lattice
+  auto pull = [&](int x, int y, double (&pf)[9], double (&pg)[9], double& r, double& u,
+                  double& v, double& tt) {
+    const double fyp = fy_in[y * pitch + x];
+    if (interior)
+      pull_node<ST_FP64, GEQ, true>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+    else
+      pull_node<ST_FP64, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+  };
+  for (int idx = t; idx < DT::kR; idx += DT::kNT) {
+    const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    double pf[9], pg[9], r, u, v, tt;
+    pull(x, y, pf, pg, r, u, v, tt);
+    put_node(x, y, pf, r, u, v, tt);
+  }
+  __syncthreads();
+
+  // 2. test filter on F (box_filter, :1204-1224) and similarity scales on Q
+  for (int idx = t; idx < DT::kF; idx += DT::kNT) {
+    const int x = fx0 + idx % DT::kFW, y = fy0 + idx / DT::kFW;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
+    double s_u = 0.0, s_v = 0.0, s_t = 0.0;
+    for (int yy = w.y0; yy < w.y1; ++yy)
+      for (int xx = w.x0; xx < w.x1; ++xx) {
+        s_u += ru(xx, yy);
+        s_v += rv(xx, yy);
+        s_t += rt(xx, yy);
+      }
+    const double ubx = s_u / w.cnt, uby = s_v / w.cnt, tb = s_t / w.cnt;
+    sh[DT::kOffF + 0 * DT::kF + idx] = ubx;
+    sh[DT::kOffF + 1 * DT::kF + idx] = uby;
+    sh[DT::kOffF + 2 * DT::kF + idx] = tb;
+    const int lx = x - qx0, ly = y - qy0;
+    if (lx >= 0 && lx < DT::kQW && ly >= 0 && ly < DT::kQH) {
+      double q, p;
+      dyn_similarity(ru(x, y), rv(x, y), rt(x, y), ubx, uby, tb, q, p);
+      sh[DT::kOffQ + 0 * DT::kQ + ly * DT::kQW + lx] = q;
+      sh[DT::kOffQ + 1 * DT::kQ + ly * DT::kQW + lx] = p;
+    }
+  }
+  __syncthreads();
+
+  // 3. contractions and mixed forms on T, held in registers until R and F
+  //    are no longer read
+  DynTerms keep[DT::kTPer];
+  bool have[DT::kTPer];
+#pragma unroll
+  for (int j = 0; j < DT::kTPer; ++j) {
+    const int idx = t + j * DT::kNT;
+    const int x = tx0 + idx % DT::kTW, y = ty0 + idx / DT::kTW;
+    have[j] = idx < DT::kT && x >= 0 && x < nx && y >= 0 && y < ny;
+    if (!have[j]) continue;
+    const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
+    DynLocal q;
+    dyn_local(geo, w, x, y, rt, fu, fv, ft, qs, ph, q);
+    double s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0, s_vt = 0.0;
+    for (int yy = w.y0; yy < w.y1; ++yy)
+      for (int xx = w.x0; xx < w.x1; ++xx) {
+        const double u = ru(xx, yy), v = rv(xx, yy), tt = rt(xx, yy);
+        s_uu += u * u;
+        s_uv += u * v;
+        s_vv += v * v;
+        s_ut += u * tt;
+        s_vt += v * tt;
+      }
+    q.lb[0] = s_uu / w.cnt;
+    q.lb[1] = s_uv / w.cnt;
+    q.lb[2] = s_vv / w.cnt;
+    q.lb[3] = s_ut / w.cnt;
+    q.lb[4] = s_vt / w.cnt;
+    DynStrain s;
+    s.xx = sh[DT::kOffS + 0 * DT::kT + idx];
+    s.xy = sh[DT::kOffS + 1 * DT::kT + idx];
+    s.yy = sh[DT::kOffS + 2 * DT::kT + idx];
+    s.mag = sh[DT::kOffS + 3 * DT::kT + idx];
+    keep[j] = dyn_terms_node(s, q, a);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < DT::kTPer; ++j) {
+    if (!have[j]) continue;
+    const int idx = t + j * DT::kNT;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sh[m * DT::kT + idx] = keep[j].t[m];
+    sh[4 * DT::kT + idx] = keep[j].nu_mix_g;
+    sh[5 * DT::kT + idx] = keep[j].kap_mix_g;
+  }
+  __syncthreads();
+
+  // 4. coefficient stage (box of the contractions) and the collision with
+  //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185); the
+  //    node's populations are pulled again (the tile's reads of step 1 are
+  //    still in L2) rather than held in registers through steps 2 and 3
+  const int ix = ox + t % TX, iy = oy + t / TX;
+  if (ix >= nx || iy >= ny) return;
+  const int k = iy * pitch + ix;
+  double f[9], g[9], rho, ux, uy, temp;
+  pull(ix, iy, f, g, rho, ux, uy, temp);
+  const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
+  double num[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    num[m] = box_mean(w, LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW});
+  const int it = (iy - ty0) * DT::kTW + (ix - tx0);
+  double cnu, ck, nt, at;
+  dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
+                dc.nu_t_min, dc.alpha_t_min, cnu, ck, nt, at);
+  double fy_new, nut_new;
+  collide_node<double, GEQ, true>(f, g, rho, ux, uy, temp, ctx, ctx.cs, fy_new, nut_new, nt, at);
+  fy_out[k] = fy_new;
+  if (write_aux) {
+    nut[k] = nut_new;
+    dc.cnu[k] = cnu;
+    dc.ck[k] = ck;
+    dc.nut_pre[k] = nt;
+    dc.at_pre[k] = at;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    dst[(unsigned)i * P32 + (unsigned)k] = f[i];
+    dst[(unsigned)(9 + i) * P32 + (unsigned)k] = g[i];
+  }
+}
+
+constexpr int kDynTX = 16, kDynTY = 16;
+
+template <int GEQ, int W>
+hipError_t go_dyn_step(const Geom& geo, const StepCtx& ctx, const Walls& walls, const void* src,
+                       void* dst, const void* fy_in, void* fy_out, void* nut,
+                       const DynSgsArgs& a, int write_aux, hipStream_t st) {
+  Geom tiles = geo;
+  tiles.nbx = (geo.nx + kDynTX - 1) / kDynTX;
+  tiles.nby = (geo.ny + kDynTY - 1) / kDynTY;
+  hipLaunchKernelGGL((k_dyn_step<GEQ, W, kDynTX, kDynTY>), dim3(tiles.nbx * tiles.nby),
+                     dim3(kDynTX * kDynTY), 0, st, geo, tiles, ctx, walls,
+                     static_cast<const double*>(src), static_cast<double*>(dst),
+                     static_cast<const double*>(fy_in), static_cast<double*>(fy_out),
+                     static_cast<double*>(nut), a, dyn_coef_of(a), write_aux);
+  return hipGetLastError();
+}
+
+template <int GEQ>
+hipError_t by_width(int width, const Geom& geo, const StepCtx& ctx, const Walls& walls,
+                    const void* src, void* dst, const void* fy_in, void* fy_out, void* nut,
+                    const DynSgsArgs& a, int write_aux, hipStream_t st) {
+  switch (width) {
+    case 1: return go_dyn_step<GEQ, 1>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    case 2: return go_dyn_step<GEQ, 2>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    case 3: return go_dyn_step<GEQ, 3>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
@@ -196,6 +548,16 @@ hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs,
   hipLaunchKernelGGL(k_dyn_terms, grid, block, 0, st, geo, a);
   if (coeffs) hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, geo, a);
   return hipGetLastError();
+}
+
+hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,
+                                const void* src, void* dst, const void* fy_in, void* fy_out,
+                                void* nut, const DynSgsArgs& a, int write_aux, hipStream_t st) {
+  if (!dyn_step_fits(a.width)) return hipErrorInvalidValue;
+  return geq == 1 ? by_width<1>(a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                                write_aux, st)
+                  : by_width<2>(a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                                write_aux, st);
 }
 
 }  // namespace cavlbm

@@ -109,6 +109,15 @@ constexpr int kDynPlanes = 20;  // planes of scratch + outputs above
 // coefficient pass; without it the collision runs that stage itself
 // (MODE_FUSED_DYN with the DynCoef of dyn_coef_of).
 hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs, hipStream_t st);
+// One launch per dynamic step from the FP64 C-form (src -> dst): streaming,
+// the model's passes, the coefficient stage and the collision, reading the
+// force plane fy_in and writing fy_out (tiles read their halo's fy).  With
+// write_aux the collision's nu_t and the coefficient planes are stored.
+// Test-filter widths 1..3.
+inline bool dyn_step_fits(int width) { return width >= 1 && width <= 3; }
+hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,
+                                const void* src, void* dst, const void* fy_in, void* fy_out,
+                                void* nut, const DynSgsArgs& a, int write_aux, hipStream_t st);
 __host__ __device__ inline DynCoef dyn_coef_of(const DynSgsArgs& a) {
   DynCoef d;
   d.terms = a.terms;

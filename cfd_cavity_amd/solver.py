@@ -425,3 +425,16 @@ class CavitySolver:
         out = C.c_int32()
         self._check(self._L.cavlbm_steps_per_launch(self._h, C.byref(out)))
         return out.value
+
+    def set_dynamic_fusion(self, one_launch):
+        """One launch per dynamic-SGS step (True, the default, where it applies:
+        FP64 storage, test filter of 1..3 nodes) or four (False); both give
+        the same state bit for bit."""
+        self._check(self._L.cavlbm_set_dynamic_fusion(self._h, 1 if one_launch else 0))
+
+    @property
+    def dynamic_launches(self):
+        """Kernel launches per dynamic-SGS step for the current settings."""
+        out = C.c_int32()
+        self._check(self._L.cavlbm_dynamic_launches(self._h, C.byref(out)))
+        return out.value

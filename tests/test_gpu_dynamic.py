@@ -232,3 +232,70 @@ def test_reads_between_dynamic_steps_do_not_perturb():
     for w in ("f", "g"):
         assert bits_equal(a.get_pops(w), b.get_pops(w)), w
     assert bits_equal(a.nu_t_pre, b.nu_t_pre) and a.dynamic_stats() == b.dynamic_stats()
+
+
+def _dyn_pair(cfg, model, spin):
+    runs = []
+    for one in (True, False):
+        s = CavitySolver(*args_of(cfg))
+        s.run(spin)
+        s.set_dynamic_fusion(one)
+        s.set_dynamic_sgs(True, *model)
+        runs.append(s)
+    return runs
+
+
+def assert_same_dynamic_state(a, b, what):
+    for name in FIELDS:
+        assert bits_equal(a.get_field(name), b.get_field(name)), (what, name)
+    for w in ("f", "g"):
+        ga, gb = a.get_pops(w), b.get_pops(w)
+        assert bits_equal(ga, gb), (what, w, first_mismatch(ga, gb))
+    assert bits_equal(a.nu_t_pre, b.nu_t_pre), (what, "nu_t_pre")
+    assert bits_equal(a.alpha_t_pre, b.alpha_t_pre), (what, "alpha_t_pre")
+    assert a.dynamic_stats() == b.dynamic_stats(), what
+
+
+@pytest.mark.parametrize("model", [(0.5, -0.1, 2.0), (0.5, -0.1, 1.0), (0.3, -0.1, 3.0),
+                                   (0.7, -0.5, 2.4)])
+@pytest.mark.parametrize("cfg", [SMALL, dict(SMALL, nx=50, ny=70, ra=1e6),
+                                 dict(SMALL, nx=64, ny=256, ra=1e6)])
+def test_one_launch_step_equals_four_launches(cfg, model):
+    """The one-launch dynamic step (tiles of the model's stencil through LDS)
+    gives the four-launch step's state bit for bit: test-filter widths 1, 2, 3,
+    lattices narrower than a tile and not multiples of it, direct steps, reads
+    in between and graph replays (48 steps)."""
+    a, b = _dyn_pair(cfg, model, 190)   # steps 190..199 in the buoyancy ramp
+    assert a.dynamic_launches == 1 and b.dynamic_launches == 4
+    for chunk in (1, 2, 7, 60, 3):
+        a.run(chunk)
+        b.run(chunk)
+        assert_same_dynamic_state(a, b, f"{cfg['nx']}x{cfg['ny']} {model} t={a.time_step}")
+
+
+def test_one_launch_step_matches_oracle(oracle):
+    g, o = pair(oracle)
+    g.run(200)
+    o.run(200)
+    g.set_dynamic_sgs(True, 0.5, -0.1, 2.0)
+    o.set_dynamic_sgs(True, 0.5, -0.1, 2.0)
+    assert g.dynamic_launches == 1
+    g.step()
+    o.step()
+    assert relmax(g.nu_t_pre, o.sgs_pre(0)) < 1e-13
+    assert_state_close(g, o, 1e-13, "1 step")
+    g.run(59)
+    o.run(59)
+    assert_state_close(g, o, 1e-10, "60 steps")
+    assert_stats_close(g, o, 1e-9)
+
+
+def test_wide_filter_keeps_four_launches():
+    s = CavitySolver(*args_of(SMALL))
+    s.set_dynamic_sgs(True, 0.5, -0.1, 5.0)      # width 5: beyond the one-launch tiles
+    assert s.dynamic_launches == 4
+    s.run(3)
+    assert s.diagnostics()["all_finite"]
+    f32 = CavitySolver(*args_of(SMALL), 1, storage="fp32")
+    f32.set_dynamic_sgs(True)
+    assert f32.dynamic_launches == 4
