@@ -152,6 +152,9 @@ SIGNATURES = {
     "cavlbm_octree_set_from_snapshot": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _u64]),
     "cavlbm_octree_diagnostics": (C.c_int, [_vp, C.POINTER(OctreeDiag)]),
     "cavlbm_octree_set_dynamic_sgs": (C.c_int, [_vp, _i32, _d, _d, _d]),
+    "cavlbm_octree_dynamic_stats": (C.c_int, [_vp, _dp]),
+    "cavlbm_octree_get_sgs_pre": (C.c_int, [_vp, _i32, _i32, _dp, _u64]),
+    "cavlbm_octree_interface_factors": (C.c_int, [_vp, _dp]),
     "cavlbm_octree_last_error": (C.c_char_p, [_vp]),
     "cavlbm_octsnap_write": (C.c_int, [C.c_char_p, C.c_uint32, _u64, C.c_uint32,
                                        C.POINTER(_u64), _dp, _dp, _dp, _dp, _i32]),

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/cavlbm.h"
+#include "lbm_launch.hpp"
 #include "lbm_octree.hpp"
 
 using namespace cavlbm;
@@ -146,6 +147,16 @@ struct cavlbm_octree {
   hipGraphExec_t graph[32] = {};
   int graph_next[32] = {};
   bool graph_failed = false;
+  // Dynamic mixed-scale SGS (solver.rs:229-248): per grid kDynPlanes FP64
+  // planes (the model's scratch, C_nu 16, C_kappa 17, nu_t_pre 18, alpha_t_pre
+  // 19), allocated and zeroed on the first enable; the interface factors on
+  // the device ([0] A, [1] / [2] B of the hot / cold block), which the
+  // transfers use from then on (the reference keeps the last recomputed
+  // factors when the model is switched off).
+  bool dyn_on = false;
+  double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
+  double* dyn[5] = {};
+  OctXi* xi_dev = nullptr;
   std::string err;
 };
 
@@ -297,8 +308,102 @@ hipError_t double_substep(cavlbm_octree* o, int g0, int count, const OctCollide&
   return launch_oct_stream(L2, S, count, o->geq, o->st);
 }
 
-// step_cavity (solver.rs:647-947) with the dynamic SGS off: enqueues the
-// launches of the coarse step `step`.  The reference walks the hot block and
+// compute_dynamic_sgs_level (smag.rs:144-416) on grid gi's current M-form:
+// the uniform model's filter, contractions and coefficient passes with the
+// level's molecular tau, nu and alpha; nu_t / alpha_t land in planes 18 / 19.
+hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, double alpha) {
+  const OctLevel& L = o->lv[gi];
+  Geom geo{};
+  geo.nx = L.nx;
+  geo.ny = L.ny;
+  geo.pitch = L.nx;
+  geo.nbx = (L.nx + kTileX - 1) / kTileX;
+  geo.nby = (L.ny + kTileY - 1) / kTileY;
+  geo.plane = L.plane;
+  const size_t P = (size_t)L.plane;
+  DynSgsArgs a{};
+  a.mpops = L.m;
+  a.rho = L.m + kOctRho * P;
+  a.ux = L.m + kOctUx * P;
+  a.uy = L.m + kOctUy * P;
+  a.temp = L.m + kOctT * P;
+  a.tau = tau_mol;
+  // smag.rs:170-175: eta = max(eta_test, 1), width = round(eta) nodes (from
+  // 2 max(nx, ny) + 1 on, every width averages the whole level)
+  const double eta = std::fmax(o->dyn_eta, 1.0);
+  const double cap = 2.0 * (double)std::max(L.nx, L.ny) + 1.0;
+  const double wr = std::round(eta);
+  a.width = (int)(wr < cap ? wr : cap);
+  if (a.width < 1) a.width = 1;
+  a.half_left = (a.width - 1) / 2;
+  a.dtest = eta * 1.0;
+  a.alpha_exp = o->dyn_alpha;
+  a.beta_exp = 1.0 - o->dyn_alpha;
+  const double margin = (kTauTrtFloor - 0.5) / 3.0;
+  a.nu_t_min = std::fmax(o->dyn_min * nu, margin - nu);
+  a.alpha_t_min = std::fmax(o->dyn_min * alpha, margin - alpha);
+  double* q = o->dyn[gi];
+  a.ubx = q + 0 * P;
+  a.uby = q + 1 * P;
+  a.tb = q + 2 * P;
+  a.lb = q + 3 * P;
+  a.qsm = q + 8 * P;
+  a.phi = q + 9 * P;
+  a.terms = q + 10 * P;
+  a.nu_mix_g = q + 14 * P;
+  a.kap_mix_g = q + 15 * P;
+  a.cnu = q + 16 * P;
+  a.ck = q + 17 * P;
+  a.nut_pre = q + 18 * P;
+  a.at_pre = q + 19 * P;
+  return launch_dynamic_sgs(geo, a, true, o->st);
+}
+
+const double* nut_of(const cavlbm_octree* o, int gi) {
+  return o->dyn[gi] + 18 * (size_t)o->lv[gi].plane;
+}
+const double* at_of(const cavlbm_octree* o, int gi) {
+  return o->dyn[gi] + 19 * (size_t)o->lv[gi].plane;
+}
+
+// The per-node relaxation of levels g0 (and g0 + 1) in a collision pass.
+void with_model(const cavlbm_octree* o, OctCollide& c, int g0, int count, const Relax& r) {
+  for (int i = 0; i < count; ++i) {
+    c.nut[i] = nut_of(o, g0 + i);
+    c.at[i] = at_of(o, g0 + i);
+  }
+  c.tp = r.tp;
+  c.tt = r.tt;
+}
+
+// The factors of one interface from the mean nu_t / alpha_t of its coarse
+// grid gc and of the fine grids gf[0..nf) (solver.rs:690-722, :807-830).
+OctXiUpdate xi_update(const cavlbm_octree* o, int gc, const int* gf, int nf, double nu_c,
+                      double alpha_c, double nu_f, double alpha_f, OctXi* out) {
+  OctXiUpdate u{};
+  u.nut_c = nut_of(o, gc);
+  u.at_c = at_of(o, gc);
+  u.n_c = o->lv[gc].plane;
+  for (int i = 0; i < nf; ++i) {
+    u.nut_f[i] = nut_of(o, gf[i]);
+    u.at_f[i] = at_of(o, gf[i]);
+    u.n_f[i] = o->lv[gf[i]].plane;
+  }
+  u.nf = nf;
+  u.nu_c = nu_c;
+  u.alpha_c = alpha_c;
+  u.nu_f = nu_f;
+  u.alpha_f = alpha_f;
+  u.odd_exact = o->p.odd_exact != 0;
+  u.out = out;
+  return u;
+}
+
+// step_cavity (solver.rs:647-947): enqueues the launches of the coarse step
+// `step`.  With the dynamic model the model runs on each level before each
+// of its collisions, which use the per-node nu_t / alpha_t, and the interface
+// factors are recomputed on the device (the tiled kernel then takes one
+// sub-step per launch).  The reference walks the hot block and
 // then the cold one; the two touch disjoint fine levels and disjoint coarse
 // nodes after the coarse-to-fine injection, so both advance in the same
 // launches.
@@ -312,11 +417,21 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   const double beta_c = o->beta_eff_coarse * br;
   const double beta_f = o->beta_eff_fine * br;
   const double beta_s = 0.25 * o->beta_eff_coarse * br;
-  const OctCollide c0 = collide_params(beta_c, nu_eff, alpha_eff, tc, false);
-  const OctCollide c1 = collide_params(beta_f, 2.0 * nu_eff, 2.0 * alpha_eff, tf, true);
-  const OctCollide c2 = collide_params(beta_s, 4.0 * nu_eff, 4.0 * alpha_eff, ts, true);
-  // level descriptors are read right before each transfer: a tiled sub-step
-  // moves its levels' M-forms to the other buffer
+  OctCollide c0 = collide_params(beta_c, nu_eff, alpha_eff, tc, false);
+  OctCollide c1 = collide_params(beta_f, 2.0 * nu_eff, 2.0 * alpha_eff, tf, true);
+  OctCollide c2 = collide_params(beta_s, 4.0 * nu_eff, 4.0 * alpha_eff, ts, true);
+  const bool dyn = o->dyn_on;
+  // the molecular tau of each level's model (solver.rs:665, :761-763)
+  const double tau_l0 = 0.5 + 3.0 * nu_eff;
+  const double tau_l1 = 0.5 + 3.0 * 2.0 * nu_eff;
+  const double tau_l2 = 0.5 + 3.0 * 4.0 * nu_eff;
+  // interface factors: the device's once the model has been enabled
+  const OctXi* xa[2] = {o->xi_dev, o->xi_dev};
+  const OctXi* xb[2] = {o->xi_dev ? o->xi_dev + 1 : nullptr, o->xi_dev ? o->xi_dev + 2 : nullptr};
+  const OctXi* const* xi_a = o->xi_dev ? xa : nullptr;
+  const OctXi* const* xi_b = o->xi_dev ? xb : nullptr;
+  // level descriptors are read right before each transfer and model pass: a
+  // tiled sub-step moves its levels' M-forms to the other buffer
   auto coarse2 = [o](OctLevel* L) { L[0] = L[1] = o->lv[0]; };
   auto pair_of = [o](OctLevel* L, int g0) {
     L[0] = o->lv[g0];
@@ -329,26 +444,68 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
     e = (x);                       \
     if (e != hipSuccess) return e; \
   } while (0)
+  if (dyn) {
+    // the model on L0, then interface A from L0 (this step) and the blocks
+    // (their last sub-step) (solver.rs:663-723)
+    OCT_Q(level_model(o, 0, tau_l0, nu_eff, alpha_eff));
+    const int blocks[2] = {1, 2};
+    const OctXiUpdate u = xi_update(o, 0, blocks, 2, nu_eff, alpha_eff, 2.0 * nu_eff,
+                                    2.0 * alpha_eff, o->xi_dev);
+    OCT_Q(launch_oct_xi_update(&u, 1, o->st));
+    with_model(o, c0, 0, 1, tc);
+    with_model(o, c1, 1, 2, tf);
+    with_model(o, c2, 3, o->levels == 3 ? 2 : 0, ts);
+  }
   OCT_Q(substep(o, 0, 1, c0));
   coarse2(lc);
   pair_of(lf, 1);
-  OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[1], 2, o->xi_a, o->geq, o->st));
+  OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[1], 2, o->xi_a, xi_a, o->geq, o->st));
+  if (dyn) {  // the blocks' model before sub-step 1 (solver.rs:767-786)
+    OCT_Q(level_model(o, 1, tau_l1, 2.0 * nu_eff, 2.0 * alpha_eff));
+    OCT_Q(level_model(o, 2, tau_l1, 2.0 * nu_eff, 2.0 * alpha_eff));
+  }
   if (o->levels == 3) {
     OCT_Q(substep(o, 1, 2, c1));
+    if (dyn) {  // interface B of each block from L1 (now) and L2 (last) (:807-830)
+      const int h[1] = {3}, c[1] = {4};
+      const OctXiUpdate u[2] = {
+          xi_update(o, 1, h, 1, 2.0 * nu_eff, 2.0 * alpha_eff, 4.0 * nu_eff, 4.0 * alpha_eff,
+                    o->xi_dev + 1),
+          xi_update(o, 2, c, 1, 2.0 * nu_eff, 2.0 * alpha_eff, 4.0 * nu_eff, 4.0 * alpha_eff,
+                    o->xi_dev + 2)};
+      OCT_Q(launch_oct_xi_update(u, 2, o->st));
+    }
     pair_of(lc, 1);
     pair_of(lf, 3);
-    OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[3], 2, o->xi_b, o->geq, o->st));
-    OCT_Q(double_substep(o, 3, 2, c2));
+    OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[3], 2, o->xi_b, xi_b, o->geq, o->st));
+    if (dyn) {
+      for (int sp = 0; sp < 2; ++sp) {  // the strips' model before each sub-step (:834-850)
+        OCT_Q(level_model(o, 3, tau_l2, 4.0 * nu_eff, 4.0 * alpha_eff));
+        OCT_Q(level_model(o, 4, tau_l2, 4.0 * nu_eff, 4.0 * alpha_eff));
+        OCT_Q(substep(o, 3, 2, c2));
+      }
+    } else {
+      OCT_Q(double_substep(o, 3, 2, c2));
+    }
     pair_of(lf, 3);
     pair_of(lc, 1);
-    OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[3], 2, o->xi_b, o->geq, o->st));
+    OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[3], 2, o->xi_b, xi_b, o->geq, o->st));
+    if (dyn) {  // before sub-step 2 the reference passes the unramped nu / alpha (:879-892)
+      OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
+      OCT_Q(level_model(o, 2, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
+    }
+    OCT_Q(substep(o, 1, 2, c1));
+  } else if (dyn) {
+    OCT_Q(substep(o, 1, 2, c1));
+    OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
+    OCT_Q(level_model(o, 2, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
     OCT_Q(substep(o, 1, 2, c1));
   } else {
     OCT_Q(double_substep(o, 1, 2, c1));
   }
   pair_of(lf, 1);
   coarse2(lc);
-  OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[1], 2, o->xi_a, o->geq, o->st));
+  OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[1], 2, o->xi_a, xi_a, o->geq, o->st));
 #undef OCT_Q
   return hipSuccess;
 }
@@ -874,10 +1031,104 @@ int cavlbm_octree_diagnostics(cavlbm_octree* o, cavlbm_octree_diag* d) {
   return 0;
 }
 
-int cavlbm_octree_set_dynamic_sgs(cavlbm_octree* o, int32_t enable, double, double, double) {
+int cavlbm_octree_set_dynamic_sgs(cavlbm_octree* o, int32_t enable, double smag_alpha,
+                                  double nu_sm_min, double eta_test) {
   if (!o) return CAVLBM_E_PARAM;
-  if (enable)
-    return ofail(o, CAVLBM_E_UNSUPPORTED, "the octree's dynamic SGS is not on the GPU path");
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  if (enable && !o->xi_dev) {
+    // the buffers of every level start at zero (solver.rs:414-462), the
+    // device factors at the molecular ones
+    for (int gi = 0; gi < o->ngrids; ++gi) {
+      const size_t n = (size_t)kDynPlanes * (size_t)o->lv[gi].plane;
+      o->dyn[gi] = dalloc<double>(o, n);
+      if (!o->dyn[gi]) return ofail(o, CAVLBM_E_HIP, "hipMalloc of the dynamic SGS planes");
+      OHIP_TRY(o, hipMemsetAsync(o->dyn[gi], 0, n * sizeof(double), o->st));
+    }
+    OctXi* x = dalloc<OctXi>(o, 3);
+    if (!x) return ofail(o, CAVLBM_E_HIP, "hipMalloc of the interface factors");
+    const OctXi init[3] = {o->xi_a, o->xi_b, o->xi_b};
+    OHIP_TRY(o, hipMemcpyAsync(x, init, sizeof(init), hipMemcpyHostToDevice, o->st));
+    OHIP_TRY(o, hipStreamSynchronize(o->st));
+    o->xi_dev = x;
+  }
+  // graphs captured with the previous settings
+  for (auto& g : o->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  o->graph_failed = false;
+  o->dyn_on = enable != 0;
+  o->dyn_alpha = smag_alpha;
+  o->dyn_min = nu_sm_min;
+  o->dyn_eta = eta_test;
+  return 0;
+}
+
+int cavlbm_octree_dynamic_stats(cavlbm_octree* o, double out[5]) {
+  if (!o || !out) return CAVLBM_E_PARAM;
+  for (int i = 0; i < 5; ++i) out[i] = 0.0;
+  if (!o->xi_dev) return 0;
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  // C_nu / C_kappa of L0's last model pass: sequential mean and max from +0
+  // (smag.rs:397-415)
+  const size_t n0 = (size_t)o->lv[0].plane;
+  std::vector<double> c(2 * n0);
+  int rc = copy_down(o, o->dyn[0] + 16 * n0, c.data(), 2 * n0);
+  if (rc) return rc;
+  double sn = 0.0, mn = 0.0, sk = 0.0, mk = 0.0;
+  for (size_t k = 0; k < n0; ++k) {
+    sn += c[k];
+    if (c[k] > mn) mn = c[k];
+    sk += c[n0 + k];
+    if (c[n0 + k] > mk) mk = c[n0 + k];
+  }
+  out[0] = sn / (double)n0;
+  out[1] = mn;
+  out[2] = sk / (double)n0;
+  out[3] = mk;
+  // nu_t max over every level (solver.rs:918-931)
+  double m = 0.0;
+  for (int gi = 0; gi < o->ngrids; ++gi) {
+    const size_t n = (size_t)o->lv[gi].plane;
+    std::vector<double> v(n);
+    rc = copy_down(o, o->dyn[gi] + 18 * n, v.data(), n);
+    if (rc) return rc;
+    for (double x : v) m = std::fmax(m, std::fabs(x));
+  }
+  out[4] = m;
+  return 0;
+}
+
+int cavlbm_octree_get_sgs_pre(cavlbm_octree* o, int32_t grid, int32_t which, double* out,
+                              uint64_t n) {
+  if (!o || !out) return CAVLBM_E_PARAM;
+  if (!grid_ok(o, grid) || (which != 0 && which != 1))
+    return ofail(o, CAVLBM_E_PARAM, "grid or which out of range");
+  const size_t P = (size_t)o->lv[grid].plane;
+  if (n != P) return ofail(o, CAVLBM_E_SIZE, "buffer size differs from the grid's node count");
+  if (!o->xi_dev) {
+    std::fill(out, out + P, 0.0);
+    return 0;
+  }
+  OHIP_TRY(o, hipSetDevice(o->p.device));
+  return copy_down(o, o->dyn[grid] + (18 + which) * P, out, P);
+}
+
+int cavlbm_octree_interface_factors(cavlbm_octree* o, double out[9]) {
+  if (!o || !out) return CAVLBM_E_PARAM;
+  OctXi x[3] = {o->xi_a, o->xi_b, o->xi_b};
+  if (o->xi_dev) {
+    OHIP_TRY(o, hipSetDevice(o->p.device));
+    OHIP_TRY(o, hipMemcpyAsync(x, o->xi_dev, sizeof(x), hipMemcpyDeviceToHost, o->st));
+    OHIP_TRY(o, hipStreamSynchronize(o->st));
+  }
+  for (int i = 0; i < 3; ++i) {
+    const bool none = i > 0 && o->levels == 2;
+    out[3 * i] = none ? 0.0 : x[i].even;
+    out[3 * i + 1] = none ? 0.0 : x[i].odd;
+    out[3 * i + 2] = none ? 0.0 : x[i].thermal;
+  }
   return 0;
 }
 

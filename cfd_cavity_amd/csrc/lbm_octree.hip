@@ -34,6 +34,7 @@ struct OctXferGroup {
   OctLevel Lc[2], Lf[2];
   OctGhostTab gt[2];
   OctCoincTab ct[2];
+  const OctXi* xd[2];  // per item: device factors, or null (the launch's xi)
   int nblk0;
 };
 
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_walls(OctLevel L, OctStream s
 template <int GEQ>
 __global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctGroup grp, OctCollide p) {
   int lb;
-  const OctLevel& L = grp.L[group_item(grp.nblk0, lb)];
+  const int item = group_item(grp.nblk0, lb);
+  const OctLevel& L = grp.L[item];
   const int k = local_id(lb, L.plane);
   if (k < 0) return;
   const long long P = L.plane;
@@ -128,7 +130,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_collide(OctGroup grp, OctColl
   }
   double fy;
   collide_node_fixed<GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k], L.m[kOctUy * P + k],
-                          L.m[kOctT * P + k], p.ctx.beta, p.ctx.t_ref, p.rf, fy);
+                          L.m[kOctT * P + k], p.ctx.beta, p.ctx.t_ref, oct_relax(p, item, k),
+                          fy);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     L.c[i * P + k] = f[i];
@@ -271,7 +274,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_stream_collide(OctGroup grp, 
                   (ix >= 1 && ix <= L.nx - 2 && iy >= 1 && iy <= L.ny - 2);
   if (in) {
     double fy;
-    collide_node_fixed<GEQ>(f, g, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref, p.rf, fy);
+    collide_node_fixed<GEQ>(f, g, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref,
+                            oct_relax(p, item, k), fy);
     L.m[kOctFy * P + k] = fy;
   }
 #pragma unroll
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(kOctTileMaxThreads) void k_oct_tile(OctTileGroup gr
     if (in)
       collide_node_fixed<GEQ>(f, g, L.m[kOctRho * P + k], L.m[kOctUx * P + k],
                               L.m[kOctUy * P + k], L.m[kOctT * P + k], p.ctx.beta, p.ctx.t_ref,
-                              p.rf, fyc);
+                              oct_relax(p, item, k), fyc);
   }
 
 #pragma unroll
@@ -388,7 +392,8 @@ __global__ __launch_bounds__(kOctTileMaxThreads) void k_oct_tile(OctTileGroup gr
         const bool in =
             !p.interior_only || (ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2);
         if (in)
-          collide_node_fixed<GEQ>(f2, g2, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref, p.rf, fyc);
+          collide_node_fixed<GEQ>(f2, g2, rho, ux, uy, temp, p.ctx.beta, p.ctx.t_ref,
+                                  oct_relax(p, item, iy * nx + ix), fyc);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
           f[i] = f2[i];
@@ -438,7 +443,7 @@ __device__ __forceinline__ void xfer_lane(const OctLevel& Ls, int ks, double w,
 // of the coarse level, rescaled by xi, injected on the ghost nodes, whose
 // moments are then recomputed from the injected populations.
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi xi) {
+__global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi xi_arg) {
   constexpr int NS = 4, EPB = XferShape<NS>::kEntries;
   __shared__ double sh[EPB * NS][kXferVals];
   int lb;
@@ -446,6 +451,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi x
   const OctLevel& Lc = grp.Lc[item];
   const OctLevel& Lf = grp.Lf[item];
   const OctGhostTab& t = grp.gt[item];
+  const OctXi xi = grp.xd[item] ? *grp.xd[item] : xi_arg;
   const int e = (int)threadIdx.x / NS, q = (int)threadIdx.x % NS;
   const int j = lb * EPB + e;
   const bool valid = e < EPB && j < t.n;
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_c2f(OctXferGroup grp, OctXi x
 // grid.rs:178-196), then refresh_moments_nodes (solver.rs:1188-1206) of the
 // restricted node.
 template <int GEQ>
-__global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi xi) {
+__global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi xi_arg) {
   constexpr int NS = 9, EPB = XferShape<NS>::kEntries;
   __shared__ double sh[EPB * NS][kXferVals];
   int lb;
@@ -519,6 +525,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
   const OctLevel& Lf = grp.Lf[item];
   const OctLevel& Lc = grp.Lc[item];
   const OctCoincTab& t = grp.ct[item];
+  const OctXi xi = grp.xd[item] ? *grp.xd[item] : xi_arg;
   const int e = (int)threadIdx.x / NS, q = (int)threadIdx.x % NS;
   const int j = lb * EPB + e;
   const bool valid = e < EPB && j < t.n;
@@ -587,7 +594,65 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
 
 inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kOctBlock - 1) / kOctBlock)); }
 
+// Interface factors from the levels' mean nu_t / alpha_t: one workgroup per
+// update.  A mean is a fixed-order tree sum: thread t adds elements t, t +
+// kXiThreads, ... from +0, then the partials are halved pairwise in LDS.
+constexpr int kXiThreads = 1024;
+
+__device__ double block_mean(const double* v, long long n, double* red) {
+  const int t = (int)threadIdx.x;
+  double s = 0.0;
+  for (long long k = t; k < n; k += kXiThreads) s += v[k];
+  red[t] = s;
+  __syncthreads();
+  for (int h = kXiThreads / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] = red[t] + red[t + h];
+    __syncthreads();
+  }
+  const double m = red[0] / (double)n;
+  __syncthreads();
+  return m;
+}
+
+__global__ __launch_bounds__(kXiThreads) void k_oct_xi_update(OctXiUpdate u0, OctXiUpdate u1) {
+  __shared__ double red[kXiThreads];
+  const OctXiUpdate& u = blockIdx.x == 0 ? u0 : u1;
+  const double nt_c = block_mean(u.nut_c, u.n_c, red);
+  const double at_c = block_mean(u.at_c, u.n_c, red);
+  double nt_f = 0.0, at_f = 0.0;
+  for (int i = 0; i < u.nf; ++i) {
+    nt_f += block_mean(u.nut_f[i], u.n_f[i], red);
+    at_f += block_mean(u.at_f[i], u.n_f[i], red);
+  }
+  if (threadIdx.x != 0) return;
+  const double nf = (double)u.nf;
+  // RelaxTimes::from_nu_alpha (lbm.rs:100-109) of both sides
+  const double tpc = 0.5 + 3.0 * (u.nu_c + nt_c);
+  const double tmc = 0.5 + (3.0 / 16.0) / (tpc - 0.5);
+  const double ttc = 0.5 + 3.0 * (u.alpha_c + at_c);
+  const double tpf = 0.5 + 3.0 * (u.nu_f + nt_f / nf);
+  const double tmf = 0.5 + (3.0 / 16.0) / (tpf - 0.5);
+  const double ttf = 0.5 + 3.0 * (u.alpha_f + at_f / nf);
+  // rescaling_factors_multilevel (interface.rs:141-158), two sub-steps
+  const double dt = 1.0 / 2.0;
+  OctXi x;
+  x.even = dt * tpf / tpc;
+  x.odd = u.odd_exact ? dt * tmf / tmc : dt;
+  x.thermal = dt * ttf / ttc;
+  x.inv_even = 1.0 / x.even;
+  x.inv_odd = 1.0 / x.odd;
+  x.inv_thermal = 1.0 / x.thermal;
+  *u.out = x;
+}
+
 }  // namespace
+
+hipError_t launch_oct_xi_update(const OctXiUpdate* u, int count, hipStream_t st) {
+  if (count < 1 || count > 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_oct_xi_update, dim3(count), dim3(kXiThreads), 0, st, u[0],
+                     u[count - 1]);
+  return hipGetLastError();
+}
 
 hipError_t launch_oct_init_eq(const OctLevel& L, int geq, hipStream_t st) {
   if (geq == 1)
@@ -700,14 +765,15 @@ hipError_t launch_oct_tile(const OctLevel* L, const OctStream* s, int count, int
 }
 
 hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
-                                     const OctGhostTab* t, int count, const OctXi& xi, int geq,
-                                     hipStream_t st) {
+                                     const OctGhostTab* t, int count, const OctXi& xi,
+                                     const OctXi* const* xi_dev, int geq, hipStream_t st) {
   OctXferGroup g{};
   int total = 0;
   for (int i = 0; i < count; ++i) {
     g.Lc[i] = coarse[i];
     g.Lf[i] = fine[i];
     g.gt[i] = t[i];
+    g.xd[i] = xi_dev ? xi_dev[i] : nullptr;
     const int nb = (t[i].n + XferShape<4>::kEntries - 1) / XferShape<4>::kEntries;
     if (i == 0) g.nblk0 = nb;
     total += nb;
@@ -721,14 +787,15 @@ hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fin
 }
 
 hipError_t launch_oct_fine_to_coarse(const OctLevel* fine, const OctLevel* coarse,
-                                     const OctCoincTab* t, int count, const OctXi& xi, int geq,
-                                     hipStream_t st) {
+                                     const OctCoincTab* t, int count, const OctXi& xi,
+                                     const OctXi* const* xi_dev, int geq, hipStream_t st) {
   OctXferGroup g{};
   int total = 0;
   for (int i = 0; i < count; ++i) {
     g.Lf[i] = fine[i];
     g.Lc[i] = coarse[i];
     g.ct[i] = t[i];
+    g.xd[i] = xi_dev ? xi_dev[i] : nullptr;
     const int nb = (t[i].n + XferShape<9>::kEntries - 1) / XferShape<9>::kEntries;
     if (i == 0) g.nblk0 = nb;
     total += nb;

@@ -33,7 +33,21 @@ struct OctCollide {
   StepCtx ctx;        // beta, t_ref (= 0) of the level
   RelaxF<double> rf;  // the level's relaxation factors
   int interior_only;  // window [1, n-2]^2 (fine levels) or the whole level
+  // Dynamic SGS: per-node nu_t / alpha_t of each level of the launch (null:
+  // the fixed factors rf), with the level's tau+ and tau_t; the node's
+  // relaxation times are 0.5 + 3 (tau - 0.5) / 3 + 3 nu_t (cavity.rs:104-113).
+  const double* nut[2];
+  const double* at[2];
+  double tp, tt;
 };
+
+// The relaxation factors of node k of launch item `item`.
+__device__ __forceinline__ RelaxF<double> oct_relax(const OctCollide& p, int item, int k) {
+  if (!p.nut[item]) return p.rf;
+  const double tp = 0.5 + 3.0 * (p.tp - 0.5) / 3.0 + 3.0 * p.nut[item][k];
+  const double tt = 0.5 + 3.0 * (p.tt - 0.5) / 3.0 + 3.0 * p.at[item][k];
+  return relax_factors<double>(tp, tt);
+}
 
 // Streaming / boundary rules of a level.
 struct OctStream {
@@ -61,6 +75,28 @@ struct OctXi {
   double inv_even, inv_odd, inv_thermal;  // RN(1/xi), for the restriction's quotients
 };
 
+// Device-side update of the interface factors from the levels' mean nu_t /
+// alpha_t (dynamic SGS, solver.rs:690-722 and :807-830):
+//   xi = rescaling(relax(m_c nu + <nu_t>_c, m_c alpha + <alpha_t>_c),
+//                  relax(m_f nu + <nu_t>_f, m_f alpha + <alpha_t>_f))
+// where <.>_c is the mean over the coarse level and <.>_f the average over
+// the fine levels given (the two blocks for interface A: their means added
+// in order, then / 2).  Means are deterministic tree sums (not the
+// reference's sequential order).
+struct OctXiUpdate {
+  const double* nut_c;
+  const double* at_c;
+  long long n_c;
+  const double* nut_f[2];
+  const double* at_f[2];
+  long long n_f[2];
+  int nf;                 // fine levels averaged (1 or 2)
+  double nu_c, alpha_c;   // molecular (ramped) nu / alpha of each side
+  double nu_f, alpha_f;
+  int odd_exact;
+  OctXi* out;
+};
+
 // Launches over `count` (1 or 2) independent levels or interfaces at once:
 // the hot and cold blocks (or strips) of a level take one launch, their
 // workgroups split between the two.
@@ -82,11 +118,14 @@ bool oct_tile_fits(int nx, int ny, int ty, int ns);
 hipError_t launch_oct_tile(const OctLevel* L, const OctStream* s, int count, int ty, int ns,
                            const OctCollide& p, int geq, hipStream_t st);
 
+// xi_dev: null (the factors xi) or per item the device factors to use.
 hipError_t launch_oct_coarse_to_fine(const OctLevel* coarse, const OctLevel* fine,
-                                     const OctGhostTab* t, int count, const OctXi& xi, int geq,
-                                     hipStream_t st);
+                                     const OctGhostTab* t, int count, const OctXi& xi,
+                                     const OctXi* const* xi_dev, int geq, hipStream_t st);
 hipError_t launch_oct_fine_to_coarse(const OctLevel* fine, const OctLevel* coarse,
-                                     const OctCoincTab* t, int count, const OctXi& xi, int geq,
-                                     hipStream_t st);
+                                     const OctCoincTab* t, int count, const OctXi& xi,
+                                     const OctXi* const* xi_dev, int geq, hipStream_t st);
+// One or two interface-factor updates (one workgroup each).
+hipError_t launch_oct_xi_update(const OctXiUpdate* u, int count, hipStream_t st);
 
 }  // namespace cavlbm

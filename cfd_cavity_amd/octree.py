@@ -97,9 +97,29 @@ class OctreeCavitySolver:
         return t.value
 
     def set_dynamic_sgs(self, on, smag_alpha=0.5, nu_sm_min=-0.1, eta_test=2.0):
-        """solver.rs:475-480; the octree's dynamic model is not on the GPU path."""
+        """solver.rs:475-480: the dynamic mixed-scale model on every level
+        (src/octree/smag.rs) before each of its collisions."""
         self._check(self._L.cavlbm_octree_set_dynamic_sgs(self._h, int(bool(on)), smag_alpha,
                                                           nu_sm_min, eta_test))
+
+    def dynamic_stats(self):
+        """(cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max, nu_t_max)."""
+        out = np.zeros(5)
+        self._check(self._L.cavlbm_octree_dynamic_stats(self._h, _dp(out)))
+        return tuple(float(v) for v in out)
+
+    def sgs_pre(self, grid, which=0):
+        """nu_t_pre (0) or alpha_t_pre (1) of a grid."""
+        nx, ny = self.dims[grid]
+        out = np.empty(nx * ny)
+        self._check(self._L.cavlbm_octree_get_sgs_pre(self._h, grid, which, _dp(out), out.size))
+        return out
+
+    def interface_factors(self):
+        """xi (even, odd, thermal) of interface A, then B of the hot and cold block."""
+        out = np.zeros(9)
+        self._check(self._L.cavlbm_octree_interface_factors(self._h, _dp(out)))
+        return out
 
     # --- state
     def derived(self):

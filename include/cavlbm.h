@@ -396,10 +396,26 @@ int cavlbm_octree_set_pops(cavlbm_octree *o, int32_t grid, int32_t which, const 
 int cavlbm_octree_set_from_snapshot(cavlbm_octree *o, const double *rho, const double *ux,
                                     const double *uy, const double *t, uint64_t total);
 int cavlbm_octree_diagnostics(cavlbm_octree *o, cavlbm_octree_diag *out);
-/* CavitySolver::set_dynamic_sgs (solver.rs:475-480): enable != 0 returns
- * CAVLBM_E_UNSUPPORTED. */
+/* CavitySolver::set_dynamic_sgs (solver.rs:475-480): the dynamic
+ * mixed-scale model (src/octree/smag.rs) on every level before each of its
+ * collisions, with per-node relaxation times, and the interface factors
+ * recomputed every step from the levels' mean nu_t / alpha_t
+ * (solver.rs:647-947).  The means are deterministic tree sums on the device
+ * (the reference sums sequentially), and the model's mixed forms use the
+ * device's pow, so the path agrees with a CPU evaluation to a tolerance.
+ * cavlbm_octree_dynamic_stats: cs_dyn_mean, cs_dyn_max, kappa_dyn_mean,
+ * kappa_dyn_max of L0's last model pass, nu_t_max over every level (zeros
+ * before the model first runs).  cavlbm_octree_get_sgs_pre: a grid's
+ * nu_t_pre (which 0) or alpha_t_pre (1), n = its node count.
+ * cavlbm_octree_interface_factors: xi even / odd / thermal of interface A
+ * (L0-L1), then interface B (L1-L2) of the hot and of the cold block (zeros
+ * with two levels); the factors in use, which the model recomputes. */
 int cavlbm_octree_set_dynamic_sgs(cavlbm_octree *o, int32_t enable, double smag_alpha,
                                   double nu_sm_min, double eta_test);
+int cavlbm_octree_dynamic_stats(cavlbm_octree *o, double out[5]);
+int cavlbm_octree_get_sgs_pre(cavlbm_octree *o, int32_t grid, int32_t which, double *out,
+                              uint64_t n);
+int cavlbm_octree_interface_factors(cavlbm_octree *o, double out[9]);
 const char *cavlbm_octree_last_error(const cavlbm_octree *o);
 
 /* OCTSNAP1 multi-level snapshots (src/octree/snapshot.rs:1-170), host-side

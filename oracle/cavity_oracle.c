@@ -570,12 +570,20 @@ static double ddy(const double *f, size_t nx, size_t ny, size_t ix, size_t iy) {
   return 0.5 * (f[yp * nx + ix] - f[ym * nx + ix]);
 }
 
-void orc_compute_dynamic_sgs(orc_solver *s) {
-  const size_t nx = s->nx, ny = s->ny, n = s->n;
-  collide_ctx c = make_ctx(s, s->time_step);
-  const double tau = c.tau;
-  const double alpha_exp = s->smag_alpha;
-  const double eta = fmax(s->eta_test, 1.0);
+/* The model on given arrays: f k-major [k][9] (post-stream), the moments,
+ * the molecular tau of the collision and the molecular nu / alpha of the
+ * floors.  compute_dynamic_sgs (solver.rs:562-828) on the uniform solver;
+ * compute_dynamic_sgs_level (src/octree/smag.rs:144-416, stated there as an
+ * exact port of the uniform method) on one octree level.  stats: mean and
+ * max of C_nu, mean and max of C_kappa (sequential sums). */
+void orc_dynamic_model(size_t nx, size_t ny, const double *f9, const double *rho,
+                       const double *ux, const double *uy, const double *temp, double tau,
+                       double smag_alpha, double eta_test, double nu_mol, double alpha_mol,
+                       double nu_sm_min, double *nu_t_out, double *alpha_t_out,
+                       double stats[4]) {
+  const size_t n = nx * ny;
+  const double alpha_exp = smag_alpha;
+  const double eta = fmax(eta_test, 1.0);
   /* (eta.round() as usize).max(1); every width from 2*max(nx, ny) + 1 on
    * averages the whole domain, and the cap keeps the cast defined */
   const double cap = 2.0 * (double)(nx > ny ? nx : ny) + 1.0;
@@ -583,8 +591,6 @@ void orc_compute_dynamic_sgs(orc_solver *s) {
   size_t width = (size_t)(wr < cap ? wr : cap);
   if (width < 1) width = 1;
   const double dx = 1.0, dtest = eta * dx, delta_t = DELTA_T;
-  const double nu_mol = s->nu, alpha_mol = s->alpha, nu_sm_min = s->nu_sm_min;
-  const double *rho = s->rho, *ux = s->ux, *uy = s->uy, *temp = s->temp;
   enum { UBX, UBY, TB, PXX, PXY, PYY, PTX, PTY, LXX, LXY, LYY, LTX, LTY,
          SHXX, SHXY, SHYY, SMH, SBXX, SBXY, SBYY, SMB, GTX, GTY, GBX, GBY,
          QSM, PHI, TMH, TMB, QBAR, PHIBAR, NUMV, DENV, NUMK, DENK, NUG, KAPG,
@@ -613,7 +619,7 @@ void orc_compute_dynamic_sgs(orc_solver *s) {
       double pi_xx = 0.0, pi_xy = 0.0, pi_yy = 0.0;
       for (int i = 0; i < 9; ++i) {
         double cx = (double)CX[i], cy = (double)CY[i];
-        double fneq = s->f[k].f[i] - orc_feq(i, rho_k, ux_k, uy_k);
+        double fneq = f9[9 * k + (size_t)i] - orc_feq(i, rho_k, ux_k, uy_k);
         pi_xx += fneq * cx * cx;
         pi_xy += fneq * cx * cy;
         pi_yy += fneq * cy * cy;
@@ -681,8 +687,6 @@ void orc_compute_dynamic_sgs(orc_solver *s) {
   box_filter(A(DENV), nx, ny, width, A(DENVB));
   box_filter(A(NUMK), nx, ny, width, A(NUMKB));
   box_filter(A(DENK), nx, ny, width, A(DENKB));
-  if (!s->nu_t_pre) s->nu_t_pre = (double *)xcalloc(n, sizeof(double));
-  if (!s->alpha_t_pre) s->alpha_t_pre = (double *)xcalloc(n, sizeof(double));
   double cnu_sum = 0.0, cnu_max = 0.0, ck_sum = 0.0, ck_max = 0.0;
   const double tau_floor_margin = (TAU_TRT_FLOOR - 0.5) / 3.0;
   for (size_t k = 0; k < n; ++k) {
@@ -694,17 +698,31 @@ void orc_compute_dynamic_sgs(orc_solver *s) {
     if (ck > ck_max) ck_max = ck;
     double nt = cnu * A(NUG)[k];
     double nu_t_min = fmax(nu_sm_min * nu_mol, tau_floor_margin - nu_mol);
-    s->nu_t_pre[k] = nt < nu_t_min ? nu_t_min : nt;
+    nu_t_out[k] = nt < nu_t_min ? nu_t_min : nt;
     double at = ck * A(KAPG)[k];
     double alpha_t_min = fmax(nu_sm_min * alpha_mol, tau_floor_margin - alpha_mol);
-    s->alpha_t_pre[k] = at < alpha_t_min ? alpha_t_min : at;
+    alpha_t_out[k] = at < alpha_t_min ? alpha_t_min : at;
   }
-  s->cs_dyn_mean = cnu_sum / (double)n;
-  s->cs_dyn_max = cnu_max;
-  s->kappa_dyn_mean = ck_sum / (double)n;
-  s->kappa_dyn_max = ck_max;
+  stats[0] = cnu_sum / (double)n;
+  stats[1] = cnu_max;
+  stats[2] = ck_sum / (double)n;
+  stats[3] = ck_max;
 #undef A
   free(a);
+}
+
+void orc_compute_dynamic_sgs(orc_solver *s) {
+  collide_ctx c = make_ctx(s, s->time_step);
+  if (!s->nu_t_pre) s->nu_t_pre = (double *)xcalloc(s->n, sizeof(double));
+  if (!s->alpha_t_pre) s->alpha_t_pre = (double *)xcalloc(s->n, sizeof(double));
+  double st[4];
+  orc_dynamic_model(s->nx, s->ny, &s->f[0].f[0], s->rho, s->ux, s->uy, s->temp, c.tau,
+                    s->smag_alpha, s->eta_test, s->nu, s->alpha, s->nu_sm_min, s->nu_t_pre,
+                    s->alpha_t_pre, st);
+  s->cs_dyn_mean = st[0];
+  s->cs_dyn_max = st[1];
+  s->kappa_dyn_mean = st[2];
+  s->kappa_dyn_max = st[3];
 }
 
 void orc_dynamic_stats(const orc_solver *s, double out[4]) {

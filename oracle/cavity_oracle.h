@@ -89,6 +89,13 @@ int orc_check_finite(const orc_solver *s);
 void orc_set_dynamic_sgs(orc_solver *s, int on, double smag_alpha, double nu_sm_min,
                          double eta_test);
 void orc_compute_dynamic_sgs(orc_solver *s);
+/* The model on given arrays (f k-major [k][9]); stats: C_nu mean, max,
+ * C_kappa mean, max.  Used by the uniform solver and by each octree level. */
+void orc_dynamic_model(size_t nx, size_t ny, const double *f9, const double *rho,
+                       const double *ux, const double *uy, const double *temp, double tau,
+                       double smag_alpha, double eta_test, double nu_mol, double alpha_mol,
+                       double nu_sm_min, double *nu_t_out, double *alpha_t_out,
+                       double stats[4]);
 double orc_lilly_coefficient(double num, double den);
 double orc_mixed_factor(double a, double b, double alpha);
 /* cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max */

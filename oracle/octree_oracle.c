@@ -6,8 +6,7 @@
  * into oracle/libcavity_oracle.so with -ffp-contract=off, so that expressions
  * evaluated in the reference's operand order are bitwise reproducible.
  *
- * Each function cites the reference file:line it restates.  The dynamic
- * mixed-scale SGS of the octree (src/octree/smag.rs) is not restated here.
+ * Each function cites the reference file:line it restates.
  *
  * Parity status (tests in tests/test_oracle_octree.py):
  *   - per-level phases (collide_cavity, stream_cavity(_block), the thermal
@@ -19,7 +18,15 @@
  *     and an independent bilinear stencil;
  *   - the solver (init, step order, ramps, diagnostics): the reference's V2 and
  *     V6 tests (src/octree/tests.rs:396-575) and invariants; the order of the
- *     phases inside step_cavity beyond those is PARITY UNPINNED.
+ *     phases inside step_cavity beyond those is PARITY UNPINNED;
+ *   - the dynamic mixed-scale SGS per level (src/octree/smag.rs, the model of
+ *     the uniform oracle on each level's arrays, orc_dynamic_model) and its
+ *     place in step_cavity (solver.rs:647-947: the model before every
+ *     collision, the per-node relaxation times, the interface factors xi
+ *     recomputed from the levels' mean nu_t / alpha_t): the reference's V5
+ *     tests (src/octree/tests.rs:572-786), the level model against the
+ *     independent numpy model on the coarse level, the per-level floors, and
+ *     the per-node collision against the uniform collide_node.
  */
 #include "octree_oracle.h"
 
@@ -381,10 +388,12 @@ static void refresh_moments_nodes(level *l, const coinc_entry *e, size_t n) {
 }
 
 /* ---- per-level phases (src/octree/cavity.rs) ----------------------------- */
-/* collide_node_cavity (cavity.rs:68-161), fixed-tau path (no dynamic SGS) */
+/* collide_node_cavity (cavity.rs:68-161).  pre: the dynamic model's nu_t /
+ * alpha_t of the node set the relaxation times (cavity.rs:104-113), else the
+ * level's RelaxTimes. */
 static void collide_node_cavity(double *f, double *g, double *fy_out, double rho, double ux,
                                 double uy, double temp, const relax *tau, double beta_eff,
-                                int geq_order) {
+                                int geq_order, int pre, double nu_t, double alpha_t) {
   double fy = rho * beta_eff * (temp - 0.0); /* boussinesq_force(rho, beta, 1, T, 0) */
   *fy_out = fy;
   double f_eq[9], g_eq[9];
@@ -392,7 +401,12 @@ static void collide_node_cavity(double *f, double *g, double *fy_out, double rho
     f_eq[i] = orc_feq(i, rho, ux, uy);
     g_eq[i] = orc_geq(i, temp, ux, uy, geq_order);
   }
-  const double tau_plus = tau->tau_plus, tau_minus = tau->tau_minus, tau_t = tau->tau_t;
+  double tau_plus = tau->tau_plus, tau_minus = tau->tau_minus, tau_t = tau->tau_t;
+  if (pre) {
+    tau_plus = 0.5 + 3.0 * (tau->tau_plus - 0.5) / 3.0 + 3.0 * nu_t;
+    tau_minus = 0.5 + LAMBDA_TRT / (tau_plus - 0.5);
+    tau_t = 0.5 + 3.0 * (tau->tau_t - 0.5) / 3.0 + 3.0 * alpha_t;
+  }
   double a_f = 0.5 * (1.0 / tau_plus + 1.0 / tau_minus);
   double b_f = 0.5 * (1.0 / tau_plus - 1.0 / tau_minus);
   double a_g = 1.0 / tau_t;
@@ -418,9 +432,11 @@ static void collide_node_cavity(double *f, double *g, double *fy_out, double rho
   }
 }
 
-/* collide_cavity (cavity.rs:167-243): window [1, n-1)^2 when interior_only */
+/* collide_cavity (cavity.rs:167-243): window [1, n-1)^2 when interior_only;
+ * nu_t_pre / alpha_t_pre per node of the level, or NULL (model off). */
 static void collide_cavity(level *l, const relax *tau, double beta_eff, double *fy_out,
-                           int interior_only, int geq_order) {
+                           int interior_only, int geq_order, const double *nu_t_pre,
+                           const double *alpha_t_pre) {
   tables();
   size_t nx = l->nx;
   size_t ix0 = interior_only ? 1 : 0, ix1 = interior_only ? nx - 1 : nx;
@@ -428,8 +444,10 @@ static void collide_cavity(level *l, const relax *tau, double beta_eff, double *
   for (size_t iy = iy0; iy < iy1; ++iy)
     for (size_t ix = ix0; ix < ix1; ++ix) {
       size_t k = iy * nx + ix;
+      const int pre = nu_t_pre != NULL && alpha_t_pre != NULL;
       collide_node_cavity(l->f[k].f, l->g[k].f, &fy_out[k], l->rho[k], l->ux[k], l->uy[k],
-                          l->temp[k], tau, beta_eff, geq_order);
+                          l->temp[k], tau, beta_eff, geq_order, pre, pre ? nu_t_pre[k] : 0.0,
+                          pre ? alpha_t_pre[k] : 0.0);
     }
 }
 
@@ -584,6 +602,7 @@ typedef struct {
   xi_t xi_down;
   double *fy;
   int hot, cold;
+  double *nu_t_pre, *alpha_t_pre; /* dynamic model of the level (solver.rs:172-175) */
 } nested_t;
 
 typedef struct {
@@ -592,6 +611,7 @@ typedef struct {
   double *fy;
   int hot, cold;
   nested_t *sub;
+  double *nu_t_pre, *alpha_t_pre; /* dynamic model of the level (solver.rs:137-140) */
 } block_t;
 
 struct orc_octree {
@@ -611,6 +631,11 @@ struct orc_octree {
   size_t jobs;
   uint64_t rng;
   int geq_order;
+  /* dynamic mixed-scale SGS (solver.rs:229-248) */
+  int dynamic_smag;
+  double smag_alpha, nu_sm_min, eta_test;
+  double *nu_t_pre_coarse, *alpha_t_pre_coarse;
+  double cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max, nu_t_max;
 };
 
 static double rng_f64(uint64_t *state) { return orc_rng_next_f64(state); }
@@ -744,6 +769,8 @@ orc_octree *orc_oct_new(size_t nx0, size_t ny0, size_t bl_width, double ra, doub
     b->hot = hot;
     b->cold = cold;
     b->sub = NULL;
+    b->nu_t_pre = (double *)xcalloc(b->grid.nx1 * b->grid.ny1, sizeof(double));
+    b->alpha_t_pre = (double *)xcalloc(b->grid.nx1 * b->grid.ny1, sizeof(double));
     if (levels >= 3) {
       nested_t *s = (nested_t *)xcalloc(1, sizeof(nested_t));
       size_t sbx0 = hot ? 1 : b->grid.nx1 - 4, sbx1 = hot ? 3 : b->grid.nx1 - 2;
@@ -754,6 +781,8 @@ orc_octree *orc_oct_new(size_t nx0, size_t ny0, size_t bl_width, double ra, doub
       s->fy = (double *)xcalloc(s->grid.nx1 * s->grid.ny1, sizeof(double));
       s->hot = hot;
       s->cold = cold;
+      s->nu_t_pre = (double *)xcalloc(s->grid.nx1 * s->grid.ny1, sizeof(double));
+      s->alpha_t_pre = (double *)xcalloc(s->grid.nx1 * s->grid.ny1, sizeof(double));
       b->sub = s;
     }
   }
@@ -769,6 +798,13 @@ orc_octree *orc_oct_new(size_t nx0, size_t ny0, size_t bl_width, double ra, doub
   o->fy_coarse = (double *)xcalloc(nx0 * ny0, sizeof(double));
   o->jobs = jobs ? jobs : 1;
   o->rng = seed | 1ULL;
+  /* solver.rs:455-467 */
+  o->dynamic_smag = 0;
+  o->smag_alpha = 0.5;
+  o->nu_sm_min = -0.1;
+  o->eta_test = 2.0;
+  o->nu_t_pre_coarse = (double *)xcalloc(nx0 * ny0, sizeof(double));
+  o->alpha_t_pre_coarse = (double *)xcalloc(nx0 * ny0, sizeof(double));
   init_cavity(o);
   return o;
 }
@@ -781,14 +817,20 @@ void orc_oct_free(orc_octree *o) {
     grid_free(&b->grid);
     level_free(&b->fine);
     free(b->fy);
+    free(b->nu_t_pre);
+    free(b->alpha_t_pre);
     if (b->sub) {
       grid_free(&b->sub->grid);
       level_free(&b->sub->lvl);
       free(b->sub->fy);
+      free(b->sub->nu_t_pre);
+      free(b->sub->alpha_t_pre);
       free(b->sub);
     }
   }
   free(o->fy_coarse);
+  free(o->nu_t_pre_coarse);
+  free(o->alpha_t_pre_coarse);
   free(o);
 }
 
@@ -809,18 +851,70 @@ static double ramp_beta_factor(const orc_octree *o, uint64_t step) {
   return (double)step / (double)o->spin_up_steps;
 }
 
-/* step_cavity (solver.rs:647-947), fixed-Cs-off path (dynamic SGS off) */
+/* mean_field (solver.rs:1176-1178): sequential sum over the buffer, then / n */
+static double mean_field(const double *v, size_t n) {
+  double sum = 0.0;
+  for (size_t k = 0; k < n; ++k) sum += v[k];
+  return sum / (double)n;
+}
+
+/* compute_dynamic_sgs_level (smag.rs:144-416) on a level: the uniform model
+ * on the level's arrays with its molecular tau, nu and alpha. */
+static void level_model(const orc_octree *o, const level *l, double tau_mol, double nu,
+                        double alpha, double *nu_t, double *alpha_t, double st[4]) {
+  orc_dynamic_model(l->nx, l->ny, &l->f[0].f[0], l->rho, l->ux, l->uy, l->temp, tau_mol,
+                    o->smag_alpha, o->eta_test, nu, alpha, o->nu_sm_min, nu_t, alpha_t, st);
+}
+
+static double max_abs(const double *v, size_t n, double m) {
+  for (size_t k = 0; k < n; ++k) m = fmax(m, fabs(v[k]));
+  return m;
+}
+
+/* step_cavity (solver.rs:647-947) */
 void orc_oct_step(orc_octree *o) {
   tables();
   const int go = o->geq_order;
+  const int dyn = o->dynamic_smag;
   double nu_eff, alpha_eff;
   ramp_nu_alpha(o, o->time_step, &nu_eff, &alpha_eff);
   relax tau_coarse = relax_from(nu_eff, alpha_eff);
   relax tau_fine = relax_from(2.0 * nu_eff, 2.0 * alpha_eff);
   relax tau_sub_eff = relax_from(4.0 * nu_eff, 4.0 * alpha_eff);
 
+  /* the model on L0 before its collision, then xi A from the effective
+   * relaxation times: the ramped molecular ones plus the mean nu_t / alpha_t
+   * of L0 (this step) and of the blocks (their last sub-step)
+   * (solver.rs:660-723) */
+  const double *nu_l0 = NULL, *al_l0 = NULL;
+  if (dyn) {
+    const size_t n0 = o->nx0 * o->ny0;
+    double st[4];
+    level_model(o, &o->coarse, 0.5 + 3.0 * nu_eff, nu_eff, alpha_eff, o->nu_t_pre_coarse,
+                o->alpha_t_pre_coarse, st);
+    o->cs_dyn_mean = st[0];
+    o->cs_dyn_max = st[1];
+    o->kappa_dyn_mean = st[2];
+    o->kappa_dyn_max = st[3];
+    nu_l0 = o->nu_t_pre_coarse;
+    al_l0 = o->alpha_t_pre_coarse;
+    const double nt0 = mean_field(o->nu_t_pre_coarse, n0);
+    const double at0 = mean_field(o->alpha_t_pre_coarse, n0);
+    double nt1 = 0.0, at1 = 0.0;
+    for (int bi = 0; bi < 2; ++bi) {
+      const block_t *b = &o->blocks[bi];
+      const size_t n1 = b->grid.nx1 * b->grid.ny1;
+      nt1 += mean_field(b->nu_t_pre, n1);
+      at1 += mean_field(b->alpha_t_pre, n1);
+    }
+    const double nb = 2.0;
+    relax tce = relax_from(nu_eff + nt0, alpha_eff + at0);
+    relax tfe = relax_from(2.0 * nu_eff + nt1 / nb, 2.0 * alpha_eff + at1 / nb);
+    o->xi = rescaling(tce, tfe, 2, o->odd_exact);
+  }
+
   double beta_c = o->beta_eff_coarse * ramp_beta_factor(o, o->time_step);
-  collide_cavity(&o->coarse, &tau_coarse, beta_c, o->fy_coarse, 0, go);
+  collide_cavity(&o->coarse, &tau_coarse, beta_c, o->fy_coarse, 0, go, nu_l0, al_l0);
   stream_level(&o->coarse);
   thermal_bc(&o->coarse, go);
   update_macroscopic_guo(&o->coarse, o->fy_coarse);
@@ -832,20 +926,42 @@ void orc_oct_step(orc_octree *o) {
   double beta_ramp = ramp_beta_factor(o, o->time_step);
   double beta_f = o->beta_eff_fine * beta_ramp;
   double beta_sub = 0.25 * o->beta_eff_coarse * beta_ramp;
+  const double tau_mol_l1 = 0.5 + 3.0 * 2.0 * nu_eff;
+  const double tau_mol_l2 = 0.5 + 3.0 * 4.0 * nu_eff;
+  double st[4];
   for (int bi = 0; bi < 2; ++bi) {
     block_t *b = &o->blocks[bi];
     int wl = b->hot, wr = b->cold;
-    collide_cavity(&b->fine, &tau_fine, beta_f, b->fy, 1, go);
+    /* the model on L1 before sub-step 1 (solver.rs:767-786) */
+    if (dyn)
+      level_model(o, &b->fine, tau_mol_l1, 2.0 * nu_eff, 2.0 * alpha_eff, b->nu_t_pre,
+                  b->alpha_t_pre, st);
+    collide_cavity(&b->fine, &tau_fine, beta_f, b->fy, 1, go, dyn ? b->nu_t_pre : NULL,
+                   dyn ? b->alpha_t_pre : NULL);
     stream_block(&b->fine, wl, wr, 1, 1);
     thermal_bc_block(&b->fine, wl, wr, go);
     update_macroscopic_guo(&b->fine, b->fy);
     walls_block(&b->fine, wl, wr);
     if (b->sub) {
       nested_t *s = b->sub;
+      const size_t n1 = b->grid.nx1 * b->grid.ny1, n2 = s->grid.nx1 * s->grid.ny1;
+      /* xi B from L1 (this sub-step) and L2 (its last sub-step) (solver.rs:807-830) */
+      if (dyn) {
+        const double nt1b = mean_field(b->nu_t_pre, n1), at1b = mean_field(b->alpha_t_pre, n1);
+        const double nt2 = mean_field(s->nu_t_pre, n2), at2 = mean_field(s->alpha_t_pre, n2);
+        relax l1e = relax_from(2.0 * nu_eff + nt1b, 2.0 * alpha_eff + at1b);
+        relax l2e = relax_from(4.0 * nu_eff + nt2, 4.0 * alpha_eff + at2);
+        s->xi_down = rescaling(l1e, l2e, 2, o->odd_exact);
+      }
       xi_t xi_down = s->xi_down;
       coarse_to_fine(&s->grid, &b->fine, &s->lvl, &xi_down, go);
       for (int sp = 0; sp < 2; ++sp) {
-        collide_cavity(&s->lvl, &tau_sub_eff, beta_sub, s->fy, 1, go);
+        /* the model on L2 before each of its sub-steps (solver.rs:834-850) */
+        if (dyn)
+          level_model(o, &s->lvl, tau_mol_l2, 4.0 * nu_eff, 4.0 * alpha_eff, s->nu_t_pre,
+                      s->alpha_t_pre, st);
+        collide_cavity(&s->lvl, &tau_sub_eff, beta_sub, s->fy, 1, go, dyn ? s->nu_t_pre : NULL,
+                       dyn ? s->alpha_t_pre : NULL);
         stream_block(&s->lvl, wl, wr, 1, 1);
         thermal_bc_block(&s->lvl, wl, wr, go);
         update_macroscopic_guo(&s->lvl, s->fy);
@@ -854,16 +970,71 @@ void orc_oct_step(orc_octree *o) {
       fine_to_coarse(&s->grid, &s->lvl, &b->fine, &xi_down, go);
       refresh_moments_nodes(&b->fine, s->grid.coinc, s->grid.n_coinc);
     }
-    collide_cavity(&b->fine, &tau_fine, beta_f, b->fy, 1, go);
+    /* the model on L1 before sub-step 2; the reference passes the unramped
+     * molecular nu / alpha here (solver.rs:879-892) */
+    if (dyn)
+      level_model(o, &b->fine, tau_mol_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu, b->nu_t_pre,
+                  b->alpha_t_pre, st);
+    collide_cavity(&b->fine, &tau_fine, beta_f, b->fy, 1, go, dyn ? b->nu_t_pre : NULL,
+                   dyn ? b->alpha_t_pre : NULL);
     stream_block(&b->fine, wl, wr, 1, 1);
     thermal_bc_block(&b->fine, wl, wr, go);
     update_macroscopic_guo(&b->fine, b->fy);
     walls_block(&b->fine, wl, wr);
     fine_to_coarse(&b->grid, &b->fine, &o->coarse, &xi, go);
   }
+  /* nu_t max over the levels (solver.rs:918-931) */
+  if (dyn) {
+    double m = max_abs(o->nu_t_pre_coarse, o->nx0 * o->ny0, 0.0);
+    for (int bi = 0; bi < 2; ++bi) {
+      const block_t *b = &o->blocks[bi];
+      m = max_abs(b->nu_t_pre, b->grid.nx1 * b->grid.ny1, m);
+      if (b->sub) m = max_abs(b->sub->nu_t_pre, b->sub->grid.nx1 * b->sub->grid.ny1, m);
+    }
+    o->nu_t_max = m;
+  }
   for (int bi = 0; bi < 2; ++bi)
     refresh_moments_nodes(&o->coarse, o->blocks[bi].grid.coinc, o->blocks[bi].grid.n_coinc);
   o->time_step += 1;
+}
+
+/* set_dynamic_sgs (solver.rs:476-481) */
+void orc_oct_set_dynamic_sgs(orc_octree *o, int on, double smag_alpha, double nu_sm_min,
+                             double eta_test) {
+  o->dynamic_smag = on ? 1 : 0;
+  o->smag_alpha = smag_alpha;
+  o->nu_sm_min = nu_sm_min;
+  o->eta_test = eta_test;
+}
+
+void orc_oct_dynamic_stats(const orc_octree *o, double out[5]) {
+  out[0] = o->cs_dyn_mean;
+  out[1] = o->cs_dyn_max;
+  out[2] = o->kappa_dyn_mean;
+  out[3] = o->kappa_dyn_max;
+  out[4] = o->nu_t_max;
+}
+
+double *orc_oct_sgs_pre(orc_octree *o, size_t gi, int which) {
+  if (gi == 0) return which == 0 ? o->nu_t_pre_coarse : o->alpha_t_pre_coarse;
+  if (gi <= 2) return which == 0 ? o->blocks[gi - 1].nu_t_pre : o->blocks[gi - 1].alpha_t_pre;
+  if (gi <= 4 && o->levels == 3) {
+    nested_t *s = o->blocks[gi - 3].sub;
+    return which == 0 ? s->nu_t_pre : s->alpha_t_pre;
+  }
+  return NULL;
+}
+
+void orc_oct_xi(const orc_octree *o, double out[9]) {
+  out[0] = o->xi.xi_even;
+  out[1] = o->xi.xi_odd;
+  out[2] = o->xi.xi_thermal;
+  for (int bi = 0; bi < 2; ++bi) {
+    const nested_t *s = o->blocks[bi].sub;
+    out[3 + 3 * bi] = s ? s->xi_down.xi_even : 0.0;
+    out[4 + 3 * bi] = s ? s->xi_down.xi_odd : 0.0;
+    out[5 + 3 * bi] = s ? s->xi_down.xi_thermal : 0.0;
+  }
 }
 
 void orc_oct_run(orc_octree *o, uint64_t n) {
@@ -1121,7 +1292,17 @@ void orc_oct_coincident(const orc_octree *o, size_t gi, size_t j, size_t *k_coar
 void orc_oct_collide_level(orc_octree *o, size_t gi, const double tau[3], double beta,
                            int interior_only) {
   relax r = {tau[0], tau[1], tau[2]};
-  collide_cavity(grid_level(o, gi), &r, beta, grid_fy(o, gi), interior_only, o->geq_order);
+  collide_cavity(grid_level(o, gi), &r, beta, grid_fy(o, gi), interior_only, o->geq_order,
+                 NULL, NULL);
+}
+
+void orc_oct_collide_node(double f[9], double g[9], double *fy_out, double rho, double ux,
+                          double uy, double temp, const double tau[3], double beta,
+                          int geq_order, int pre, double nu_t, double alpha_t) {
+  tables();
+  relax r = {tau[0], tau[1], tau[2]};
+  collide_node_cavity(f, g, fy_out, rho, ux, uy, temp, &r, beta, geq_order == 1 ? 1 : 2, pre,
+                      nu_t, alpha_t);
 }
 void orc_oct_stream_level(orc_octree *o, size_t gi) { stream_level(grid_level(o, gi)); }
 void orc_oct_stream_block_level(orc_octree *o, size_t gi, int wl, int wr, int wb, int wt) {

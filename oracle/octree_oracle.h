@@ -44,6 +44,16 @@ uint64_t orc_oct_time_step(const orc_octree *o);
  * tau+ / tau- / tau_t of L0, L1, L2 (9), xi_even / xi_odd / xi_thermal of
  * interface A (3) and of interface B (3), n_nodes_total  -> 22 values */
 void orc_oct_derived(const orc_octree *o, double out[22]);
+/* Dynamic mixed-scale SGS (solver.rs:476-481, :647-947, smag.rs).  stats:
+ * cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max (of L0, this step),
+ * nu_t_max (all levels).  sgs_pre: the level's nu_t_pre (0) / alpha_t_pre (1)
+ * in grid order.  xi: interface A (even, odd, thermal), then interface B of
+ * the hot and the cold block (zeros with two levels). */
+void orc_oct_set_dynamic_sgs(orc_octree *o, int on, double smag_alpha, double nu_sm_min,
+                             double eta_test);
+void orc_oct_dynamic_stats(const orc_octree *o, double out[5]);
+double *orc_oct_sgs_pre(orc_octree *o, size_t gi, int which);
+void orc_oct_xi(const orc_octree *o, double out[9]);
 /* noise_stats (solver.rs:949-1000): max_rho_dev, max_rho_dev_ifaces, masse */
 void orc_oct_noise_stats(const orc_octree *o, double out[3]);
 double orc_oct_max_u_all(const orc_octree *o);
@@ -73,6 +83,11 @@ void orc_oct_coincident(const orc_octree *o, size_t gi, size_t j, size_t *k_coar
  * uniform oracle.  `tau` is (tau+, tau-, tau_t). */
 void orc_oct_collide_level(orc_octree *o, size_t gi, const double tau[3], double beta,
                            int interior_only);
+/* collide_node_cavity (cavity.rs:68-161) on caller-owned node arrays; pre:
+ * the per-node nu_t / alpha_t of the dynamic model set the relaxation times */
+void orc_oct_collide_node(double f[9], double g[9], double *fy_out, double rho, double ux,
+                          double uy, double temp, const double tau[3], double beta,
+                          int geq_order, int pre, double nu_t, double alpha_t);
 void orc_oct_stream_level(orc_octree *o, size_t gi);                     /* stream_cavity */
 void orc_oct_stream_block_level(orc_octree *o, size_t gi, int wl, int wr, int wb, int wt);
 void orc_oct_bc_level(orc_octree *o, size_t gi);                         /* uniform BCs */

@@ -84,6 +84,12 @@ def lib():
         "orc_oct_macro_level": (None, [vp, sz]),
         "orc_oct_walls_level": (None, [vp, sz]),
         "orc_oct_walls_block_level": (None, [vp, sz, i, i]),
+        "orc_oct_set_dynamic_sgs": (None, [vp, i, d, d, d]),
+        "orc_oct_dynamic_stats": (None, [vp, dp]),
+        "orc_oct_sgs_pre": (dp, [vp, sz, i]),
+        "orc_oct_xi": (None, [vp, dp]),
+        "orc_oct_collide_node": (None, [dp, dp, dp, d, d, d, d, dp, d, i, i, d, d]),
+        "orc_dynamic_model": (None, [sz, sz, dp, dp, dp, dp, dp, d, d, d, d, d, d, dp, dp, dp]),
         "orc_oct2_new": (vp, [sz, sz, sz, sz, sz, sz, d, d, i]),
         "orc_oct2_free": (None, [vp]),
         "orc_oct2_dims": (None, [vp, C.POINTER(sz), C.POINTER(sz)]),
@@ -354,6 +360,29 @@ def _view(ptr, shape):
     return np.ctypeslib.as_array(ptr, shape=shape)
 
 
+def dynamic_model(f9, rho, ux, uy, temp, tau, smag_alpha, eta_test, nu_mol, alpha_mol,
+                  nu_sm_min):
+    """The oracle's model on given arrays (ny x nx fields, f as [k][9]):
+    (nu_t, alpha_t, (cs_mean, cs_max, kappa_mean, kappa_max))."""
+    ny, nx = rho.shape
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (f9, rho, ux, uy, temp)]
+    nu_t, al_t, st = np.zeros(nx * ny), np.zeros(nx * ny), np.zeros(4)
+    lib().orc_dynamic_model(nx, ny, *[_dp(a) for a in arr], tau, smag_alpha, eta_test, nu_mol,
+                            alpha_mol, nu_sm_min, _dp(nu_t), _dp(al_t), _dp(st))
+    return nu_t.reshape(ny, nx), al_t.reshape(ny, nx), tuple(float(v) for v in st)
+
+
+def oct_collide_node(f, g, rho, ux, uy, temp, tau3, beta, geq_order=2, pre=None):
+    """collide_node_cavity on copies of f, g; pre = (nu_t, alpha_t) or None."""
+    f, g = np.array(f, dtype=np.float64), np.array(g, dtype=np.float64)
+    fy = np.zeros(1)
+    t = np.array(tau3, dtype=np.float64)
+    nt, at = pre if pre is not None else (0.0, 0.0)
+    lib().orc_oct_collide_node(_dp(f), _dp(g), _dp(fy), rho, ux, uy, temp, _dp(t), beta,
+                               geq_order, 0 if pre is None else 1, nt, at)
+    return f, g, float(fy[0])
+
+
 class OracleLevel:
     """numpy views of one level: fields [n], pops [n, 9] (k = iy*nx + ix)."""
 
@@ -397,6 +426,27 @@ class OracleOctree:
         if h:
             self._L.orc_oct_free(h)
             self._h = None
+
+    # -- dynamic mixed-scale SGS (solver.rs:476-481, :647-947) ---------------
+    def set_dynamic_sgs(self, on=True, smag_alpha=0.5, nu_sm_min=-0.1, eta_test=2.0):
+        self._L.orc_oct_set_dynamic_sgs(self._h, int(on), smag_alpha, nu_sm_min, eta_test)
+
+    def dynamic_stats(self):
+        """(cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max, nu_t_max)."""
+        out = np.zeros(5)
+        self._L.orc_oct_dynamic_stats(self._h, _dp(out))
+        return tuple(float(v) for v in out)
+
+    def sgs_pre(self, gi, which=0):
+        """Copy of grid gi's nu_t_pre (0) or alpha_t_pre (1)."""
+        g = self.grids[gi]
+        return _view(self._L.orc_oct_sgs_pre(self._h, gi, which), (g.nx * g.ny,)).copy()
+
+    def xi(self):
+        """Interface A (even, odd, thermal), then B of the hot and cold block."""
+        out = np.zeros(9)
+        self._L.orc_oct_xi(self._h, _dp(out))
+        return out
 
     def step(self):
         self._L.orc_oct_step(self._h)
