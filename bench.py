@@ -217,25 +217,29 @@ def dynamic_form(steps: int, warmup: int, cpu_s: float):
     return out
 
 
-def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8)):
+def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8), dynamic=False):
     """The octree cavity (src/octree, the reference CLI's default 64x256 base
-    grid with bl_width 8, Ra=1e6) for 2 or 3 levels: coarse steps per second
-    and node updates per second (a coarse step updates L0 once, the L1 blocks
-    twice and the L2 strips four times), against the octree oracle (one
-    thread) on a bounded CPU sample."""
+    grid with bl_width 8, Ra=1e6) for 2 or 3 levels, optionally with the
+    dynamic mixed-scale SGS: coarse steps per second and node updates per
+    second (a coarse step updates L0 once, the L1 blocks twice and the L2
+    strips four times), against the octree oracle (one thread) on a bounded
+    CPU sample."""
     from cfd_cavity_amd import OctreeCavitySolver
     from oracle import pyoracle
 
     nx0, ny0, bl = cfg
     g = OctreeCavitySolver(nx0, ny0, bl, 1.0e6, levels=levels)
+    g.set_dynamic_sgs(dynamic)
     upd = sum(nx * ny * (1, 2, 2, 4, 4)[gi] for gi, (nx, ny) in enumerate(g.dims))
     g.run(20)
     ms = g.time_run(steps)
-    out = {"config": f"{nx0}x{ny0}, bl_width {bl}, Ra=1e6, levels {levels}",
+    out = {"config": f"{nx0}x{ny0}, bl_width {bl}, Ra=1e6, levels {levels}"
+                     + (", dynamic SGS" if dynamic else ""),
            "us_per_coarse_step": ms / steps * 1e3, "node_updates_per_step": upd,
            "mnups": upd * steps / ms / 1e3}
     if cpu_s > 0:
         o = pyoracle.OracleOctree(nx0, ny0, bl, 1.0e6, levels=levels)
+        o.set_dynamic_sgs(dynamic)
         o.run(1)
         n, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < cpu_s:
@@ -405,6 +409,8 @@ def main():
             for lv in (int(x) for x in a.octree_levels.split(",")):
                 print(f"[bench] octree, {lv} levels", file=sys.stderr, flush=True)
                 extra[f"octree_levels{lv}"] = octree_form(lv, 1000, cs)
+            print("[bench] octree, 2 levels, dynamic SGS", file=sys.stderr, flush=True)
+            extra["octree_levels2_dynamic"] = octree_form(2, 500, cs, dynamic=True)
 
     precision = None
     if "uniform" in paths and a.precision_steps > 0:

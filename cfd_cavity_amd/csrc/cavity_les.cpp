@@ -10,7 +10,7 @@
 //   6 snapshot read / restart failure.
 // Outputs in --out: progress.log, snapshot.bin (CAVSNAP1; OCTSNAP1 with
 // --mode octree), summary.json (uniform).  The CSV / VTK field exporters of
-// post.rs are not reproduced; the octree's dynamic SGS is not on the GPU path.
+// post.rs are not reproduced.
 // Extra flags: --storage fp64|fp32|fp16_offset, --geq-order 1|2, --device N.
 #include <sys/stat.h>
 
@@ -313,10 +313,12 @@ int run_octree(const Args& a) {
     cavlbm::OctreeCavitySolver sim(a.nx0, a.ny0, a.bl_width, a.ra, a.pr, a.utarget, a.seed,
                                    a.jobs, a.odd_exact, (int)a.levels, a.nu_start_mult, a.nu,
                                    a.geq_order, a.device);
-    if (a.dynamic_smag) {
-      std::fprintf(stderr, "Erreur : --dynamic-smag en mode octree n'est pas sur le chemin GPU\n");
-      return 1;
-    }
+    // main.rs:515-523
+    sim.set_dynamic_sgs(a.dynamic_smag, a.smag_alpha, a.nu_sm_min, a.eta_test);
+    if (a.dynamic_smag)
+      std::printf("LES dynamique (Étape D) : alpha_mixte = %s | nu_sm_min = %s | eta_test = %s\n",
+                  rust_display(a.smag_alpha).c_str(), rust_display(a.nu_sm_min).c_str(),
+                  rust_display(a.eta_test).c_str());
     const int ng = sim.n_grids();
     if (!a.restart.empty()) {  // main.rs:526-546
       uint32_t lv = 0, sng = 0;
@@ -420,13 +422,23 @@ int run_octree(const Args& a) {
       if (is_progress(step)) {
         noise_max = std::fmax(noise_max, dg.max_rho_dev);
         noise_iface_max = std::fmax(noise_iface_max, dg.max_rho_dev_ifaces);
-        char line[512];
+        // main.rs:627-641: the model's diagnostics as a suffix
+        std::string dyn_suffix;
+        if (a.dynamic_smag) {
+          const std::vector<double> st = sim.dynamic_stats();
+          char b[160];
+          std::snprintf(b, sizeof b, " | cs_dyn = %.4f | kap_dyn = %.4f | nu_t_max = %s", st[0],
+                        st[2], rust_exp(st[4], 3).c_str());
+          dyn_suffix = b;
+        }
+        char line[640];
         std::snprintf(line, sizeof line,
                       "step %7zu | max_rho_dev = %s | max_rho_dev_ifaces = %s | masse = %.6f | "
-                      "Nuc = %.4f | Nuc_link0 = %.4f | Nuf = %.4f | Nu_fin = %.4f | max_u = %.5f",
+                      "Nuc = %.4f | Nuc_link0 = %.4f | Nuf = %.4f | Nu_fin = %.4f | max_u = %.5f%s",
                       step, rust_exp(dg.max_rho_dev, 3).c_str(),
                       rust_exp(dg.max_rho_dev_ifaces, 3).c_str(), dg.masse, dg.nusselt_hot,
-                      dg.nusselt_hot_link0, dg.nusselt_cold, dg.nusselt_hot_fine, dg.max_u_all);
+                      dg.nusselt_hot_link0, dg.nusselt_cold, dg.nusselt_hot_fine, dg.max_u_all,
+                      dyn_suffix.c_str());
         std::printf("%s\n", line);
         log_lines.push_back(line);
         if (dg.guard != 0) {  // check_guards (solver.rs:1018-1041)

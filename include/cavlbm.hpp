@@ -204,6 +204,12 @@ class OctreeCavitySolver {
   void set_dynamic_sgs(bool on, double a = 0.5, double m = -0.1, double e = 2.0) {
     check(cavlbm_octree_set_dynamic_sgs(o_, on ? 1 : 0, a, m, e));
   }
+  // cs_dyn_mean, cs_dyn_max, kappa_dyn_mean, kappa_dyn_max, nu_t_max
+  std::vector<double> dynamic_stats() {
+    std::vector<double> v(5);
+    check(cavlbm_octree_dynamic_stats(o_, v.data()));
+    return v;
+  }
   cavlbm_octree* raw() { return o_; }
 
  private:

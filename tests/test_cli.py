@@ -138,3 +138,27 @@ def test_cli_octree_run_snapshot_and_restart(tmp_path):
     bad = run("--mode", "octree", "--nx0", 32, "--ny0", 64, "--bl-width", 6, "--steps", 5,
               "--restart", out / "snapshot.bin", "--out", tmp_path / "oct3")
     assert bad.returncode == 6 and "incompatible" in bad.stderr
+
+
+@pytest.mark.gpu
+def test_cli_octree_dynamic_smag(tmp_path):
+    """--mode octree --dynamic-smag (main.rs:515-523, :627-641): the
+    announcement and the cs_dyn / kap_dyn / nu_t_max suffix of every progress
+    line, with the values of the library's octree stepped alike."""
+    from cfd_cavity_amd import OctreeCavitySolver
+
+    out = tmp_path / "octdyn"
+    r = run("--mode", "octree", "--nx0", 32, "--ny0", 64, "--bl-width", 4, "--levels", 3,
+            "--steps", 120, "--sample-start", 0, "--sample-every", 40, "--progress-every", 40,
+            "--dynamic-smag", "--out", out)
+    assert r.returncode == 0, r.stderr
+    assert "LES dynamique (Étape D) : alpha_mixte = 0.5 | nu_sm_min = -0.1 | eta_test = 2" \
+        in r.stdout
+    lines = (out / "progress.log").read_text().splitlines()[:4]
+    assert all(" | cs_dyn = " in ln and " | kap_dyn = " in ln and " | nu_t_max = " in ln
+               for ln in lines)
+    s = OctreeCavitySolver(32, 64, 4, 1.0e6, levels=3)
+    s.set_dynamic_sgs(True)
+    s.run(120)
+    cs, _, kap, _, _ = s.dynamic_stats()
+    assert f"cs_dyn = {cs:.4f} | kap_dyn = {kap:.4f}" in lines[-1]
