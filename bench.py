@@ -361,6 +361,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the dynamic-SGS and octree measurements")
     ap.add_argument("--octree-levels", default="2,3", help="octree configurations to measure")
+    ap.add_argument("--octree-forms", default="plain,dynamic",
+                    help="octree forms: plain (each level count) and dynamic (2 levels)")
     ap.add_argument("--precision-steps", type=int, default=200000,
                     help="steps of the FP64/FP32/FP16-offset comparison (0: skip)")
     ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree",
@@ -406,11 +408,14 @@ def main():
             print("[bench] dynamic SGS 256x1024", file=sys.stderr, flush=True)
             extra["dynamic_sgs"] = dynamic_form(500, 20, cs)
         if "octree" in paths:
+            oforms = set(a.octree_forms.split(","))
             for lv in (int(x) for x in a.octree_levels.split(",")):
-                print(f"[bench] octree, {lv} levels", file=sys.stderr, flush=True)
-                extra[f"octree_levels{lv}"] = octree_form(lv, 1000, cs)
-            print("[bench] octree, 2 levels, dynamic SGS", file=sys.stderr, flush=True)
-            extra["octree_levels2_dynamic"] = octree_form(2, 500, cs, dynamic=True)
+                if "plain" in oforms:
+                    print(f"[bench] octree, {lv} levels", file=sys.stderr, flush=True)
+                    extra[f"octree_levels{lv}"] = octree_form(lv, 1000, cs)
+            if "dynamic" in oforms:
+                print("[bench] octree, 2 levels, dynamic SGS", file=sys.stderr, flush=True)
+                extra["octree_levels2_dynamic"] = octree_form(2, 500, cs, dynamic=True)
 
     precision = None
     if "uniform" in paths and a.precision_steps > 0:

@@ -595,34 +595,76 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
 inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kOctBlock - 1) / kOctBlock)); }
 
 // Interface factors from the levels' mean nu_t / alpha_t: one workgroup per
-// update.  A mean is a fixed-order tree sum: thread t adds elements t, t +
-// kXiThreads, ... from +0, then the partials are halved pairwise in LDS.
+// update.  A mean is a fixed-order sum: thread t adds elements t, t +
+// kXiThreads, ... from +0 in that order (each thread issues kXiBatch loads
+// before adding them), each wave adds its 64 partials by a butterfly of lane
+// exchanges (a + b and b + a are the same IEEE sum, so every lane holds the
+// same total), and the first wave adds the waves' totals the same way.  A
+// level's nu_t and alpha_t are reduced together.
 constexpr int kXiThreads = 1024;
+constexpr int kXiWaves = kXiThreads / 64;
+constexpr int kXiBatch = 16;
 
-__device__ double block_mean(const double* v, long long n, double* red) {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ void block_mean2(const double* a, const double* b, long long n, double* red,
+                            double& ma, double& mb) {
   const int t = (int)threadIdx.x;
-  double s = 0.0;
-  for (long long k = t; k < n; k += kXiThreads) s += v[k];
-  red[t] = s;
-  __syncthreads();
-  for (int h = kXiThreads / 2; h > 0; h >>= 1) {
-    if (t < h) red[t] = red[t] + red[t + h];
-    __syncthreads();
+  double sa = 0.0, sb = 0.0;
+  for (long long k0 = t; k0 < n; k0 += (long long)kXiBatch * kXiThreads) {
+    double va[kXiBatch], vb[kXiBatch];
+#pragma unroll
+    for (int j = 0; j < kXiBatch; ++j) {
+      const long long k = k0 + (long long)j * kXiThreads;
+      va[j] = k < n ? a[k] : 0.0;
+      vb[j] = k < n ? b[k] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kXiBatch; ++j) {
+      if (k0 + (long long)j * kXiThreads < n) {
+        sa += va[j];
+        sb += vb[j];
+      }
+    }
   }
-  const double m = red[0] / (double)n;
+  sa = wave_sum(sa);
+  sb = wave_sum(sb);
+  const int w = t / 64, lane = t % 64;
+  if (lane == 0) {
+    red[w] = sa;
+    red[kXiWaves + w] = sb;
+  }
   __syncthreads();
-  return m;
+  double ta = 0.0, tb = 0.0;
+  if (w == 0) {
+    ta = wave_sum(lane < kXiWaves ? red[lane] : 0.0);
+    tb = wave_sum(lane < kXiWaves ? red[kXiWaves + lane] : 0.0);
+    if (lane == 0) {
+      red[2 * kXiWaves] = ta;
+      red[2 * kXiWaves + 1] = tb;
+    }
+  }
+  __syncthreads();
+  ma = red[2 * kXiWaves] / (double)n;
+  mb = red[2 * kXiWaves + 1] / (double)n;
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(kXiThreads) void k_oct_xi_update(OctXiUpdate u0, OctXiUpdate u1) {
-  __shared__ double red[kXiThreads];
+  __shared__ double red[2 * kXiWaves + 2];
   const OctXiUpdate& u = blockIdx.x == 0 ? u0 : u1;
-  const double nt_c = block_mean(u.nut_c, u.n_c, red);
-  const double at_c = block_mean(u.at_c, u.n_c, red);
+  double nt_c, at_c;
+  block_mean2(u.nut_c, u.at_c, u.n_c, red, nt_c, at_c);
   double nt_f = 0.0, at_f = 0.0;
   for (int i = 0; i < u.nf; ++i) {
-    nt_f += block_mean(u.nut_f[i], u.n_f[i], red);
-    at_f += block_mean(u.at_f[i], u.n_f[i], red);
+    double nt, at;
+    block_mean2(u.nut_f[i], u.at_f[i], u.n_f[i], red, nt, at);
+    nt_f += nt;
+    at_f += at;
   }
   if (threadIdx.x != 0) return;
   const double nf = (double)u.nf;
