@@ -252,6 +252,40 @@ def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8), dynamic
     return out
 
 
+def adjoint_form():
+    """The adjoint form's workload: 96x384, Ra=1e7, FP64, J = Nu_hot averaged
+    over steps 8k-10k from the seeded spun-up state, theta = 0.5 +- 0.01
+    (seeded), Cs = 0.10, checkpoints every 100 steps.  Wall time of one
+    gradient (dJ/dtheta over 384 nodes and dJ/dCs by the adjoint), of the
+    forward window alone, and of one tangent direction."""
+    import numpy as np
+    from cfd_cavity_amd import CavitySolver
+    from cfd_cavity_amd.adjoint import adjoint_nu_window, nu_window, tangent_nu_window
+
+    s = CavitySolver(96, 384, 1e7, 0.71, 0.10, 0.9, 0.02, 42, 1)
+    s.run(8000)
+    s.synchronize()
+    theta = 0.5 + 0.01 * np.random.default_rng(42).uniform(-1, 1, s.ny())
+
+    def timed(f):
+        t0 = time.perf_counter()
+        r = f()
+        return r, time.perf_counter() - t0
+
+    nu_window(s, theta, 0.10, 2000)                       # warm-up (graphs, buffers)
+    (j, tf) = timed(lambda: nu_window(s, theta, 0.10, 2000))
+    adjoint_nu_window(s, theta, 0.10, 2000, 100)
+    (ja, ta) = timed(lambda: adjoint_nu_window(s, theta, 0.10, 2000, 100))
+    dth = np.random.default_rng(1).standard_normal(s.ny())
+    tangent_nu_window(s, theta, 0.10, dth, 0.0, 2000)
+    (_, tt) = timed(lambda: tangent_nu_window(s, theta, 0.10, dth, 0.0, 2000))
+    return {"config": "96x384, Ra=1e7, FP64, J = mean Nu_hot over steps 8k-10k, "
+                      "checkpoints every 100 steps",
+            "forward_window_ms": tf * 1e3, "adjoint_gradient_ms": ta * 1e3,
+            "adjoint_over_forward": ta / tf, "tangent_direction_ms": tt * 1e3,
+            "gradient_entries": s.ny() + 1, "j": j, "j_adjoint": ja[0]}
+
+
 def cpu_rate(threads: int, budget_s: float):
     """Steps of the oracle on the 256x1024 workload from the reference init
     until the time budget is spent: (MLUPS, steps, seconds)."""
@@ -365,7 +399,7 @@ def main():
                     help="octree forms: plain (each level count) and dynamic (2 levels)")
     ap.add_argument("--precision-steps", type=int, default=200000,
                     help="steps of the FP64/FP32/FP16-offset comparison (0: skip)")
-    ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree",
+    ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree,adjoint",
                     help="paths to measure (a subset prints a partial line, for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -416,6 +450,9 @@ def main():
             if "dynamic" in oforms:
                 print("[bench] octree, 2 levels, dynamic SGS", file=sys.stderr, flush=True)
                 extra["octree_levels2_dynamic"] = octree_form(2, 500, cs, dynamic=True)
+        if "adjoint" in paths:
+            print("[bench] adjoint form 96x384", file=sys.stderr, flush=True)
+            extra["adjoint"] = adjoint_form()
 
     precision = None
     if "uniform" in paths and a.precision_steps > 0:
