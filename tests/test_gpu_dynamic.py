@@ -299,3 +299,28 @@ def test_wide_filter_keeps_four_launches():
     f32 = CavitySolver(*args_of(SMALL), 1, storage="fp32")
     f32.set_dynamic_sgs(True)
     assert f32.dynamic_launches == 4
+
+
+@pytest.mark.parametrize("nx,ny,eta,geq,hot", [
+    (3, 40, 2.0, 2, False),      # the narrowest lattice: every pulled node is a wall or next to one
+    (17, 3, 3.0, 2, False),      # three rows
+    (5, 7, 1.0, 1, False),       # linear g equilibrium, unit filter
+    (40, 33, 2.0, 2, True),      # hot-wall profile, 33 rows: a last tile of one row
+    (33, 130, 3.0, 1, True),
+])
+def test_one_launch_step_edge_lattices(nx, ny, eta, geq, hot):
+    """The one-launch dynamic step on lattices that exercise its clamped
+    windows and partial tiles, with the linear g equilibrium and a hot-wall
+    profile: bitwise the four-launch step."""
+    runs = []
+    for one in (True, False):
+        s = CavitySolver(nx, ny, 1e5, 0.71, 0.1, 0.9, 0.02, 42, 1, geq_order=geq)
+        if hot:
+            s.set_hot_wall(0.5 + 0.05 * np.sin(np.linspace(0.0, 3.0, ny)))
+        s.run(30)
+        s.set_dynamic_fusion(one)
+        s.set_dynamic_sgs(True, 0.5, -0.1, eta)
+        assert s.dynamic_launches == (1 if one else 4)
+        s.run(25)
+        runs.append(s)
+    assert_same_dynamic_state(runs[0], runs[1], f"{nx}x{ny} eta={eta} geq={geq} hot={hot}")
