@@ -9,8 +9,8 @@
 //   3 no averaging samples, 4 output directory, 5 write failure,
 //   6 snapshot read / restart failure.
 // Outputs in --out: progress.log, snapshot.bin (CAVSNAP1; OCTSNAP1 with
-// --mode octree), summary.json (uniform).  The CSV / VTK field exporters of
-// post.rs are not reproduced.
+// --mode octree); uniform also summary.json, fields_mean.csv, profiles_w.csv
+// and snapshot.vti (post.rs:202-316).
 // Extra flags: --storage fp64|fp32|fp16_offset, --geq-order 1|2, --device N.
 #include <sys/stat.h>
 
@@ -235,6 +235,89 @@ std::string rust_display(double v) {
 }
 
 // main.rs:740-753
+// stream_function_mean (main.rs:723-737): trapezoidal integration of the mean
+// u_x up each column, psi = 0 on the bottom row.
+std::vector<double> stream_function_mean(const std::vector<double>& ux, size_t nx, size_t ny) {
+  std::vector<double> psi(nx * ny, 0.0);
+  for (size_t ix = 0; ix < nx; ++ix) {
+    double acc = 0.0;
+    for (size_t iy = 1; iy < ny; ++iy) {
+      acc += 0.5 * (ux[(iy - 1) * nx + ix] + ux[iy * nx + ix]);
+      psi[iy * nx + ix] = acc;
+    }
+  }
+  return psi;
+}
+
+// post::write_fields_csv (post.rs:202-232): x, y, ux, uy, T, psi, nu_t of the
+// mean fields, row by row.
+std::string fields_csv(size_t nx, size_t ny, const std::vector<double>& ux,
+                       const std::vector<double>& uy, const std::vector<double>& t,
+                       const std::vector<double>& psi, const std::vector<double>& nut) {
+  std::string out = "x,y,ux,uy,T,psi,nu_t\n";
+  out.reserve(out.size() + nx * ny * 96);
+  char b[64];
+  for (size_t iy = 0; iy < ny; ++iy)
+    for (size_t ix = 0; ix < nx; ++ix) {
+      const size_t k = iy * nx + ix;
+      std::snprintf(b, sizeof b, "%.6f,%.6f", (double)ix, (double)iy);
+      out += b;
+      for (const std::vector<double>* v : {&ux, &uy, &t, &psi, &nut}) {
+        out += ',';
+        out += rust_exp((*v)[k], 8);
+      }
+      out += '\n';
+    }
+  return out;
+}
+
+// post::write_profiles_csv (post.rs:235-249): w at z = 0.5 and 0.75 of the
+// height along x / (nx - 1).
+std::string profiles_csv(const std::vector<double>& mid, const std::vector<double>& q75,
+                         size_t nx) {
+  std::string out = "x_norm,w_z05,w_z075\n";
+  char b[32];
+  for (size_t ix = 0; ix < nx; ++ix) {
+    std::snprintf(b, sizeof b, "%.6f", (double)ix / (double)(nx - 1));
+    out += b;
+    out += ',' + rust_exp(mid[ix], 8) + ',' + rust_exp(q75[ix], 8) + '\n';
+  }
+  return out;
+}
+
+// post::write_vtk (post.rs:252-316): VTK XML ImageData of the final T, ux, uy,
+// x varying fastest (k = iy*nx + ix), one line per row.
+std::string vtk_image(size_t nx, size_t ny, const std::vector<double>& t,
+                      const std::vector<double>& ux, const std::vector<double>& uy) {
+  std::string out = "<?xml version=\"1.0\"?>\n"
+                    "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+  char b[160];
+  std::snprintf(b, sizeof b,
+                "  <ImageData WholeExtent=\"0 %zu 0 %zu 0 0\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n"
+                "    <Piece Extent=\"0 %zu 0 %zu 0 0\">\n",
+                nx - 1, ny - 1, nx - 1, ny - 1);
+  out += b;
+  auto block = [&](const std::vector<double>& v) {
+    std::string s;
+    s.reserve(nx * ny * 14);
+    for (size_t iy = 0; iy < ny; ++iy) {
+      for (size_t ix = 0; ix < nx; ++ix) s += rust_exp(v[iy * nx + ix], 6) + ' ';
+      s += '\n';
+    }
+    return s;
+  };
+  out += "      <PointData Scalars=\"Temperature\">\n";
+  out += "        <DataArray type=\"Float64\" Name=\"Temperature\" format=\"ascii\">\n";
+  out += block(t) + "        </DataArray>\n";
+  out += "        <DataArray type=\"Float64\" Name=\"Ux\" format=\"ascii\">\n";
+  out += block(ux) + "        </DataArray>\n";
+  out += "        <DataArray type=\"Float64\" Name=\"Uy\" format=\"ascii\">\n";
+  out += block(uy) + "        </DataArray>\n";
+  out += "      </PointData>\n      <CellData>\n      </CellData>\n    </Piece>\n"
+         "  </ImageData>\n</VTKFile>\n";
+  return out;
+}
+
 double nusselt_from_mean(const std::vector<double>& t, size_t nx, size_t ny, double lz, bool hot) {
   double sum = 0.0;
   for (size_t iy = 0; iy < ny; ++iy) {
@@ -678,7 +761,14 @@ int main(int argc, char** argv) {
                   wm.second, w75.first, w75.second, dg.y_plus_hot, s_half);
     std::string text;
     for (auto& l : log_lines) text += l + "\n";
-    if (cavlbm_snapshot_write(snap_path.c_str(), a.nx, a.ny, rho.data(), ux.data(), uy.data(),
+    // main.rs:317-392: fields_mean.csv, profiles_w.csv, snapshot.vti,
+    // snapshot.bin, summary.json, progress.log
+    const std::vector<double> psi_m = stream_function_mean(ux_m, a.nx, a.ny);
+    if (!write_text(a.out + "/fields_mean.csv",
+                    fields_csv(a.nx, a.ny, ux_m, uy_m, t_m, psi_m, nut_m)) ||
+        !write_text(a.out + "/profiles_w.csv", profiles_csv(mid, q75, a.nx)) ||
+        !write_text(a.out + "/snapshot.vti", vtk_image(a.nx, a.ny, t, ux, uy)) ||
+        cavlbm_snapshot_write(snap_path.c_str(), a.nx, a.ny, rho.data(), ux.data(), uy.data(),
                               t.data()) ||
         !write_text(a.out + "/summary.json", js) || !write_text(a.out + "/progress.log", text)) {
       std::fprintf(stderr, "Erreur d'écriture dans %s\n", a.out.c_str());

@@ -68,6 +68,25 @@ def test_cli_run_averaging_snapshot_and_log(tmp_path):
     assert (nx, ny) == (32, 128) and np.array_equal(t, s.get_field("temperature"))
     d = s.diagnostics()
     assert f"Nuc = {d['nusselt_hot']:.4f}" in lines[-1]
+    # post.rs:202-316 exports: mean fields with the stream function, the w
+    # profiles, the final fields as VTK ImageData
+    fm = np.loadtxt(out / "fields_mean.csv", delimiter=",", skiprows=1)
+    assert fm.shape == (nx * ny, 7)
+    assert np.array_equal(fm[:, 0], np.tile(np.arange(nx), ny))
+    ux_m = fm[:, 2].reshape(ny, nx)
+    psi = fm[:, 5].reshape(ny, nx)
+    ref_psi = np.vstack([np.zeros((1, nx)), np.cumsum(0.5 * (ux_m[1:] + ux_m[:-1]), axis=0)])
+    # the file holds 9 significant digits of ux and psi: the column sums of
+    # the rounded ux agree to ~1e-8 of the largest |psi|
+    assert np.max(np.abs(psi - ref_psi)) < 1e-6 * np.max(np.abs(ref_psi))
+    pw = np.loadtxt(out / "profiles_w.csv", delimiter=",", skiprows=1)
+    assert pw.shape == (nx, 3) and pw[0, 0] == 0.0 and pw[-1, 0] == 1.0
+    assert np.allclose(pw[:, 1], fm[:, 3].reshape(ny, nx)[ny // 2], rtol=1e-7, atol=1e-14)
+    vti = (out / "snapshot.vti").read_text()
+    assert vti.startswith('<?xml version="1.0"?>') and f'WholeExtent="0 {nx - 1} 0 {ny - 1} 0 0"' in vti
+    tv = np.array(vti.split('Name="Temperature" format="ascii">')[1].split("</DataArray>")[0]
+                  .split(), dtype=float)
+    assert np.allclose(tv, s.get_field("temperature"), rtol=1e-6, atol=1e-12)
 
 
 @pytest.mark.gpu
