@@ -124,7 +124,7 @@ StepCtx make_ctx(const cavlbm_solver* s, uint64_t step) {
 }
 
 StepBufs bufs(cavlbm_solver* s) {
-  StepBufs b;
+  StepBufs b{};
   b.src = s->cpop[s->cur];
   b.dst = s->cpop[s->cur ^ 1];
   b.fy = s->fy;
@@ -1658,20 +1658,29 @@ StepCtx window_ctx(const cavlbm_solver* s, uint64_t step, double cs) {
 
 // One forward collision of the window into (dst, fy); src == nullptr: the
 // first collision, from the solver's M-form state S_0.
+// One window step: X_m from X_{m-1} (src) or, for m = 0, from the M-form.  A
+// step from src also forms the Nusselt row terms of the state it pulls
+// (MODE_WINDOW): accumulated into acc and / or their signs into sgn; fy_in is
+// the force plane of X_{m-1} when it is not fy itself.
 hipError_t window_step(cavlbm_solver* s, const double* src, double* dst, double* fy,
-                       double* nut, const double* theta, const StepCtx& ctx) {
-  StepBufs b;
+                       double* nut, const double* theta, const StepCtx& ctx,
+                       const double* fy_in = nullptr, double* acc = nullptr,
+                       double* sgn = nullptr) {
+  StepBufs b{};
   b.src = src;
   b.dst = dst;
   b.fy = fy;
+  b.fy_in = fy_in ? fy_in : fy;
+  b.nu_acc = acc;
+  b.nu_sgn = sgn;
   b.nut = nut;
   b.mpops = s->mpops;
   b.rho = s->fld[0];
   b.ux = s->fld[1];
   b.uy = s->fld[2];
   b.temp = s->fld[3];
-  return launch_step_fp64(s->geq, src ? MODE_FUSED : MODE_FIRST, s->geo, ctx, Walls{theta}, b, 0,
-                          s->st);
+  return launch_step_fp64(s->geq, src ? MODE_WINDOW : MODE_FIRST, s->geo, ctx, Walls{theta}, b,
+                          0, s->st);
 }
 
 int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K,
@@ -1708,9 +1717,11 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
   int cur = 0;
   for (uint64_t m = 0; m < K; ++m) {
     // X_m = Collide(S_m): S_0 from the M-form, then the fused step from X_{m-1}
+    // (from m = 1 the step also accumulates Nu(S_m), pulled from X_{m-1})
     const double* src = m == 0 ? nullptr : X[cur];
     double* dst = m == 0 ? X[cur] : X[cur ^ 1];
-    HIP_TRY(s, window_step(s, src, dst, fy, nut, theta, window_ctx(s, t0 + m, cs)));
+    HIP_TRY(s, window_step(s, src, dst, fy, nut, theta, window_ctx(s, t0 + m, cs), nullptr,
+                           m == 0 ? nullptr : acc));
     if (m > 0) cur ^= 1;
     if (ckpt && m % C == 0) {
       double* slot = w.alloc<double>(19 * P, s->st);
@@ -1721,9 +1732,9 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
                                 s->st));
       ckpt->push_back(slot);
     }
-    // Nu(S_{m+1}) from X_m
-    HIP_TRY(s, launch_nu_rows(s->geo, s->geq, X[cur], theta, acc, nullptr, s->st));
   }
+  // Nu(S_K) from X_{K-1}
+  HIP_TRY(s, launch_nu_rows(s->geo, s->geq, X[cur], theta, acc, nullptr, s->st));
   std::vector<double> h(s->p.ny);
   HIP_TRY(s, hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                             s->st));
@@ -1771,7 +1782,8 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
   double* lam[2] = {w.alloc<double>(19 * P, s->st), w.alloc<double>(19 * P, s->st)};
   double* mu = w.alloc<double>(19 * P, s->st);
   double* nut = w.alloc<double>(P, s->st);
-  double* sgn = w.alloc<double>(s->p.ny, s->st);
+  // sign of T(1,iy) - T(2,iy) of S_{m+1} for each m of a segment
+  double* sgn = w.alloc<double>(seg_len * s->p.ny, s->st);
   double* th_acc = w.alloc<double>(s->p.ny, s->st);
   double* cs_acc = w.alloc<double>(P, s->st);
   double* cs_sum = w.alloc<double>(1, s->st);
@@ -1793,16 +1805,19 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
     // recompute X_m, fy_m for m in [m_lo, m_hi) from the checkpoint
     HIP_TRY(s, hipMemcpyAsync(seg[0], ckpt[q], 19 * P * sizeof(double), hipMemcpyDeviceToDevice,
                               s->st));
+    // (step m pulls X_{m-1} and stores the signs of S_m, used at m - 1)
+    const uint64_t ny = s->p.ny;
     for (uint64_t m = m_lo + 1; m < m_hi; ++m) {
       double* prev = seg[m - 1 - m_lo];
       double* next = seg[m - m_lo];
-      HIP_TRY(s, hipMemcpyAsync(next + 18 * P, prev + 18 * P, P * sizeof(double),
-                                hipMemcpyDeviceToDevice, s->st));
-      HIP_TRY(s, window_step(s, prev, next, next + 18 * P, nut, theta, window_ctx(s, t0 + m, cs)));
+      HIP_TRY(s, window_step(s, prev, next, next + 18 * P, nut, theta, window_ctx(s, t0 + m, cs),
+                             prev + 18 * P, nullptr, sgn + (m - 1 - m_lo) * ny));
     }
+    HIP_TRY(s, launch_nu_rows(s->geo, s->geq, seg[m_hi - 1 - m_lo], theta, nullptr,
+                              sgn + (m_hi - 1 - m_lo) * ny, s->st));
     for (uint64_t m = m_hi; m-- > m_lo;) {
       double* Xm = seg[m - m_lo];
-      HIP_TRY(s, launch_nu_rows(s->geo, s->geq, Xm, theta, nullptr, sgn, s->st));
+      a.sgn = sgn + (m - m_lo) * ny;
       a.X = Xm;
       a.fyp = Xm + 18 * P;
       a.lam = lam_zero ? nullptr : lam[lc];

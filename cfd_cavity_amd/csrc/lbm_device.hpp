@@ -50,7 +50,8 @@ enum Mode {
   MODE_COLLIDE_M = 3,    // M-form pops+fields -> collide -> M-form pops (in place)
   MODE_FIRST_PRE = 4,    // MODE_FIRST with per-node nu_t / alpha_t given (dynamic SGS)
   MODE_MATERIALIZE_F = 5,  // MODE_MATERIALIZE writing the f planes and fields only
-  MODE_FUSED_DYN = 6     // MODE_FUSED with nu_t / alpha_t from the model's coefficient stage
+  MODE_FUSED_DYN = 6,    // MODE_FUSED with nu_t / alpha_t from the model's coefficient stage
+  MODE_WINDOW = 7        // MODE_FUSED reading fy_in, with the Nusselt row term of the pulled state
 };
 
 struct Geom {
@@ -176,6 +177,12 @@ struct StepBufs {
   const double* nut_pre;  // MODE_FIRST_PRE: nu_t per node (FP64), else unused
   const double* at_pre;   // MODE_FIRST_PRE: alpha_t per node (FP64), else unused
   DynCoef dc;             // MODE_FUSED_DYN: the model's coefficient stage
+  // MODE_WINDOW (FP64): the previous force plane (may equal fy), and per row
+  // iy the accumulated |T(1,iy) - T(2,iy)| and / or its sign, of the state
+  // this step pulls (either may be null)
+  const void* fy_in;
+  double* nu_acc;
+  double* nu_sgn;
 };
 
 constexpr int kTileX = 64;
@@ -607,10 +614,30 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   R rho, ux, uy, temp;
 
   if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE || MODE == MODE_MATERIALIZE_F ||
-      MODE == MODE_FUSED_DYN) {
-    const R fyp = static_cast<const R*>(b.fy)[k];
+      MODE == MODE_FUSED_DYN || MODE == MODE_WINDOW) {
+    const R fyp = static_cast<const R*>(MODE == MODE_WINDOW ? b.fy_in : b.fy)[k];
     pull_node<S, GEQ>(geo, walls, static_cast<const T*>(b.src), fyp, ix, iy, f, g, rho, ux, uy,
                       temp);
+    if (MODE == MODE_WINDOW && ix == 1) {
+      // the hot-wall Nusselt term of row iy (post.rs:68-78) for the pulled
+      // state: T(1, iy) is this node's, T(2, iy) the same pull and sum at
+      // column 2 (the isothermal value when column 2 is the cold wall)
+      const R t1 = temp;
+      R t2 = R(0.0);
+      if (geo.nx == 3) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) t2 += R(kW[i]) * R(kTCold);
+      } else {
+        const int gy = iy == 0 ? 1 : (iy == geo.ny - 1 ? geo.ny - 2 : iy);
+        const T* src = static_cast<const T*>(b.src);
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+          t2 += R(src[(unsigned)(9 + i) * P32 + (unsigned)((gy - kCy[i]) * geo.pitch + (2 - kCx[i]))]);
+      }
+      const R d = t1 - t2;
+      if (b.nu_acc) b.nu_acc[iy] += fabs((double)d);
+      if (b.nu_sgn) b.nu_sgn[iy] = d > R(0.0) ? 1.0 : (d < R(0.0) ? -1.0 : 0.0);
+    }
     if (MODE == MODE_MATERIALIZE || MODE == MODE_MATERIALIZE_F) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
