@@ -83,6 +83,35 @@ def test_create_rejects_like_reference_asserts(kw, needle):
     assert rc == _lib.E_PARAM and needle in msg and not h.value
 
 
+def _zero_args(args):
+    out = []
+    for a in args:
+        if a is C.c_double:
+            out.append(0.0)
+        elif a in (C.c_int32, C.c_uint32, C.c_uint64):
+            out.append(0)
+        else:
+            out.append(None)
+    return out
+
+
+def test_every_handle_entry_point_rejects_a_null_handle():
+    """A null solver / batch / octree handle gives CAVLBM_E_PARAM before any
+    device work (destroy of null is a no-op returning 0, like free)."""
+    L = _lib.lib()
+    checked = 0
+    for name, (res, args) in _lib.SIGNATURES.items():
+        if res is not C.c_int or not args or args[0] is not C.c_void_p:
+            continue
+        rc = getattr(L, name)(*_zero_args(args))
+        want = 0 if name.endswith("_destroy") else _lib.E_PARAM
+        assert rc == want, name
+        checked += 1
+    assert checked >= 50
+    for name in ("cavlbm_last_error", "cavlbm_batch_last_error", "cavlbm_octree_last_error"):
+        assert isinstance(getattr(L, name)(None), bytes)
+
+
 def test_create_without_gpu_fails_loudly():
     import torch
 
