@@ -33,6 +33,11 @@ thread_local std::string g_create_error;
 // even launch count, so a replay ends on the buffers it started from.
 constexpr int kGraphSteps = 48;
 
+struct WinSlot {
+  void* p;
+  size_t bytes;
+};
+
 }  // namespace
 
 struct cavlbm_solver {
@@ -77,6 +82,9 @@ struct cavlbm_solver {
   double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
   double* dyn = nullptr;  // kDynPlanes FP64 planes, allocated on first enable
   bool dyn_fuse = true;   // one launch per dynamic step where it applies
+  // Window objective / adjoint / tangent buffers (states, checkpoints, the
+  // recomputed segment, adjoint planes), kept between calls until destroy.
+  std::vector<WinSlot> win;
   std::string err;
 };
 
@@ -487,6 +495,8 @@ void free_all(cavlbm_solver* s) {
   f(s->red_out);
   f(s->avg);
   f(s->dyn);
+  for (auto& w : s->win) f(w.p);
+  s->win.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -1635,18 +1645,36 @@ const char* cavlbm_batch_last_error(const cavlbm_batch* b) {
 namespace {
 
 // Device work area of one window evaluation; freed on destruction.
+// The device buffers of one window call, handed out in request order from
+// the solver's window arena.  Every call of an entry point requests the same
+// sequence, so a repeated call finds its buffers there and allocates nothing
+// (a slot grows only when a call needs more than it holds).  alloc: contents
+// undefined (the caller overwrites every node before reading); zeroed: an
+// accumulator, cleared on the stream.
 struct WindowWork {
-  std::vector<void*> allocs;
-  ~WindowWork() {
-    for (void* p : allocs) (void)hipFree(p);
+  cavlbm_solver* s;
+  size_t next = 0;
+  template <class T>
+  T* alloc(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (next == s->win.size()) s->win.push_back({nullptr, 0});
+    WinSlot& slot = s->win[next++];
+    if (slot.bytes < bytes) {
+      if (slot.p) (void)hipFree(slot.p);  // synchronizes: no launch still uses it
+      slot = {nullptr, 0};
+      if (hipMalloc(&slot.p, bytes) != hipSuccess) {
+        slot.p = nullptr;
+        return nullptr;
+      }
+      slot.bytes = bytes;
+    }
+    return static_cast<T*>(slot.p);
   }
   template <class T>
-  T* alloc(size_t n, hipStream_t st) {
-    void* p = nullptr;
-    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-    allocs.push_back(p);
-    if (hipMemsetAsync(p, 0, n * sizeof(T), st) != hipSuccess) return nullptr;
-    return static_cast<T*>(p);
+  T* zeroed(size_t n) {
+    T* p = alloc<T>(n);
+    if (p && hipMemsetAsync(p, 0, n * sizeof(T), s->st) != hipSuccess) return nullptr;
+    return p;
   }
 };
 
@@ -1695,7 +1723,7 @@ int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_
   HIP_TRY(s, hipSetDevice(s->p.device));
   int rc = materialize(s);  // S_0 in M-form (the logical state is unchanged)
   if (rc) return rc;
-  theta = w.alloc<double>(s->p.ny, s->st);
+  theta = w.alloc<double>(s->p.ny);
   if (!theta) return fail(s, CAVLBM_E_HIP, "window allocation failed");
   std::vector<double> h(s->p.ny, 0.5);
   if (theta_hot) std::copy(theta_hot, theta_hot + s->p.ny, h.begin());
@@ -1708,10 +1736,10 @@ int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_
 int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K, uint64_t C,
                    WindowWork& w, std::vector<double*>* ckpt, double* j_out) {
   const size_t P = (size_t)s->geo.plane;
-  double* X[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
-  double* fy = w.alloc<double>(P, s->st);
-  double* nut = w.alloc<double>(P, s->st);
-  double* acc = w.alloc<double>(s->p.ny, s->st);
+  double* X[2] = {w.alloc<double>(18 * P), w.alloc<double>(18 * P)};
+  double* fy = w.alloc<double>(P);
+  double* nut = w.alloc<double>(P);
+  double* acc = w.zeroed<double>(s->p.ny);
   if (!X[0] || !X[1] || !fy || !nut || !acc) return fail(s, CAVLBM_E_HIP, "window allocation failed");
   const uint64_t t0 = s->time_step;
   int cur = 0;
@@ -1724,7 +1752,7 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
                            m == 0 ? nullptr : acc));
     if (m > 0) cur ^= 1;
     if (ckpt && m % C == 0) {
-      double* slot = w.alloc<double>(19 * P, s->st);
+      double* slot = w.alloc<double>(19 * P);
       if (!slot) return fail(s, CAVLBM_E_HIP, "checkpoint allocation failed");
       HIP_TRY(s, hipMemcpyAsync(slot, X[cur], 18 * P * sizeof(double), hipMemcpyDeviceToDevice,
                                 s->st));
@@ -1752,7 +1780,7 @@ extern "C" {
 int cavlbm_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny, double cs,
                      uint64_t window_steps, double* j_out) {
   if (!s || !j_out) return CAVLBM_E_PARAM;
-  WindowWork w;
+  WindowWork w{s};
   double* theta = nullptr;
   int rc = window_setup(s, theta_hot, ny, window_steps, w, theta);
   if (rc) return rc;
@@ -1765,7 +1793,7 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
   if (!s || !j_out || !grad_theta || !grad_cs) return CAVLBM_E_PARAM;
   const uint64_t K = window_steps;
   const uint64_t C = checkpoint_every ? checkpoint_every : 100;
-  WindowWork w;
+  WindowWork w{s};
   double* theta = nullptr;
   int rc = window_setup(s, theta_hot, ny, K, w, theta);
   if (rc) return rc;
@@ -1778,16 +1806,16 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
   const uint64_t seg_len = std::min<uint64_t>(C, K);
   std::vector<double*> seg(seg_len);
   for (auto& q : seg)
-    if (!(q = w.alloc<double>(19 * P, s->st))) return fail(s, CAVLBM_E_HIP, "segment allocation failed");
-  double* lam[2] = {w.alloc<double>(19 * P, s->st), w.alloc<double>(19 * P, s->st)};
-  double* mu = w.alloc<double>(19 * P, s->st);
-  double* nut = w.alloc<double>(P, s->st);
+    if (!(q = w.alloc<double>(19 * P))) return fail(s, CAVLBM_E_HIP, "segment allocation failed");
+  // mu_m of the backward step m, alternating (step m gathers mu_{m+1})
+  double* mu[2] = {w.alloc<double>(19 * P), w.alloc<double>(19 * P)};
+  double* nut = w.alloc<double>(P);
   // sign of T(1,iy) - T(2,iy) of S_{m+1} for each m of a segment
-  double* sgn = w.alloc<double>(seg_len * s->p.ny, s->st);
-  double* th_acc = w.alloc<double>(s->p.ny, s->st);
-  double* cs_acc = w.alloc<double>(P, s->st);
-  double* cs_sum = w.alloc<double>(1, s->st);
-  if (!lam[0] || !lam[1] || !mu || !nut || !sgn || !th_acc || !cs_acc || !cs_sum)
+  double* sgn = w.alloc<double>(seg_len * s->p.ny);
+  double* th_acc = w.zeroed<double>(s->p.ny);
+  double* cs_acc = w.zeroed<double>(P);
+  double* cs_sum = w.alloc<double>(1);
+  if (!mu[0] || !mu[1] || !nut || !sgn || !th_acc || !cs_acc || !cs_sum)
     return fail(s, CAVLBM_E_HIP, "adjoint allocation failed");
   const double obj_w = s->d.lz / ((double)s->p.ny * 1.0 * (double)K);
 
@@ -1797,8 +1825,8 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
   a.cs_acc = cs_acc;
   a.obj_w = obj_w;
   a.sgn = sgn;
-  int lc = 0;              // lam[lc] holds the adjoint of X_{m+1}
-  bool lam_zero = true;    // X_K is not used by J
+  int mc = 0;              // step m writes mu[mc]; mu[mc ^ 1] holds mu_{m+1}
+  bool lam_zero = true;    // X_K is not used by J: its adjoint is zero
   const uint64_t nseg = (K + C - 1) / C;
   for (uint64_t q = nseg; q-- > 0;) {
     const uint64_t m_lo = q * C, m_hi = std::min<uint64_t>(m_lo + C, K);
@@ -1820,14 +1848,12 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
       a.sgn = sgn + (m - m_lo) * ny;
       a.X = Xm;
       a.fyp = Xm + 18 * P;
-      a.lam = lam_zero ? nullptr : lam[lc];
-      a.lam_fy = lam_zero ? nullptr : lam[lc] + 18 * P;
-      a.mu = mu;
-      a.mu_fy = mu + 18 * P;
+      a.mu_next = lam_zero ? nullptr : mu[mc ^ 1];
+      a.mu_next_fy = lam_zero ? nullptr : mu[mc ^ 1] + 18 * P;
+      a.mu = mu[mc];
+      a.mu_fy = mu[mc] + 18 * P;
       HIP_TRY(s, launch_adj_step(s->geo, s->geq, false, window_ctx(s, t0 + m + 1, cs), a, s->st));
-      HIP_TRY(s, launch_adj_gather(s->geo, mu, mu + 18 * P, lam[lc ^ 1], lam[lc ^ 1] + 18 * P,
-                                   s->st));
-      lc ^= 1;
+      mc ^= 1;
       lam_zero = false;
     }
   }
@@ -1839,8 +1865,10 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
   a.mux = s->fld[1];
   a.muy = s->fld[2];
   a.mtemp = s->fld[3];
-  a.lam = lam[lc];
-  a.lam_fy = lam[lc] + 18 * P;
+  a.mu_next = mu[mc ^ 1];
+  a.mu_next_fy = mu[mc ^ 1] + 18 * P;
+  a.mu = nullptr;  // the first collision has no pulled values
+  a.mu_fy = nullptr;
   a.sgn = nullptr;
   HIP_TRY(s, launch_adj_step(s->geo, s->geq, true, window_ctx(s, t0, cs), a, s->st));
   HIP_TRY(s, launch_sum_nodes(s->geo, cs_acc, cs_sum, s->st));
@@ -1869,19 +1897,19 @@ extern "C" int cavlbm_tangent_nu_window(cavlbm_solver* s, const double* theta_ho
                                         uint64_t window_steps, double* j_out, double* dj_out) {
   if (!s || !j_out || !dj_out) return CAVLBM_E_PARAM;
   const uint64_t K = window_steps;
-  WindowWork w;
+  WindowWork w{s};
   double* theta = nullptr;
   int rc = window_setup(s, theta_hot, ny, K, w, theta);
   if (rc) return rc;
   if (dtheta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "dtheta_hot must have ny values");
   const size_t P = (size_t)s->geo.plane;
-  double* dtheta = w.alloc<double>(s->p.ny, s->st);
-  double* X[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
-  double* Xd[2] = {w.alloc<double>(18 * P, s->st), w.alloc<double>(18 * P, s->st)};
-  double* fy = w.alloc<double>(P, s->st);
-  double* fyd = w.alloc<double>(P, s->st);
-  double* acc = w.alloc<double>(s->p.ny, s->st);
-  double* dacc = w.alloc<double>(s->p.ny, s->st);
+  double* dtheta = w.zeroed<double>(s->p.ny);
+  double* X[2] = {w.alloc<double>(18 * P), w.alloc<double>(18 * P)};
+  double* Xd[2] = {w.alloc<double>(18 * P), w.alloc<double>(18 * P)};
+  double* fy = w.alloc<double>(P);
+  double* fyd = w.alloc<double>(P);
+  double* acc = w.zeroed<double>(s->p.ny);
+  double* dacc = w.zeroed<double>(s->p.ny);
   if (!dtheta || !X[0] || !X[1] || !Xd[0] || !Xd[1] || !fy || !fyd || !acc || !dacc)
     return fail(s, CAVLBM_E_HIP, "tangent allocation failed");
   if (dtheta_hot)

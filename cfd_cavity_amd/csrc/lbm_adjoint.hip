@@ -10,16 +10,19 @@
 // population set of collision m, fy_m its Guo force; S_{m+1} = Mat(X_m, fy_m)
 // is stream + thermal BCs + moments + wall velocity, X_{m+1} = Collide(S_{m+1}).
 //
-// Per backward step m:
-//   k_adj_node   (one thread per node k): recompute S_{m+1} at k from X_m and
-//                the collision at k, then propagate the adjoint of X_{m+1}(k)
-//                (lambda: 18 populations + fy) and the objective's dJ/dT(k)
-//                back to the adjoints mu(k) of the 18 values node k pulled,
-//                of fy_m(k), of theta_hot (column 0) and of Cs.
-//   k_adj_gather (one thread per source node j): lambda_m(j) = the sum of the
-//                mu of every destination that read X_m(j): the pull partner
-//                j + c_i, the bounce-back partner, and for g the adiabatic
-//                rows 0 / ny-1, which copy the values pulled by rows 1 / ny-2.
+// One launch per backward step m, k_adj_node (one thread per node k):
+//   1. lambda(k), the adjoint of X_{m+1}(k) (18 populations + fy), gathered
+//      from mu_{m+1}, the output of the previous backward step: the sum of the
+//      mu of every destination that read X_{m+1}(k) (the pull partner k + c_i,
+//      the bounce-back partner, and for g the adiabatic rows 0 / ny-1, which
+//      copy the values pulled by rows 1 / ny-2) -- the transpose of the pull,
+//      as the forward step pulls its own inputs;
+//   2. recompute S_{m+1} at k from X_m and the collision at k, then propagate
+//      lambda(k) and the objective's dJ/dT(k) back to the adjoints mu_m(k) of
+//      the 18 values node k pulled, of fy_m(k), of theta_hot (column 0) and
+//      of Cs.
+// mu alternates between two buffers (neighbours read mu_{m+1} while mu_m is
+// written).
 // The adjoint of sqrt(pi:pi) at pi = 0 is taken as 0, and that of |x| at 0.
 #include "lbm_launch.hpp"
 
@@ -76,6 +79,31 @@ __global__ void k_nu_rows(Geom geo, const double* X, const double* theta, double
   const double d = t1 - t2;
   if (acc) acc[iy] += fabs(d);
   if (sgn) sgn[iy] = d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : 0.0);
+}
+
+// lambda(j), the adjoint of the populations X(j) of a pulled state: the
+// transpose of the pull with bounce-back and of the adiabatic row copies,
+// applied to mu (the adjoints of the values each node pulled).
+__device__ __forceinline__ void gather_lambda(const Geom& geo, const double* mu, int jx, int jy,
+                                              double (&F)[9], double (&G)[9]) {
+  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
+  const long long P = geo.plane;
+  const long long j = (long long)jy * pitch + jx;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int kx = jx + kCx[i], ky = jy + kCy[i];
+    const bool in = kx >= 0 && kx < nx && ky >= 0 && ky < ny;
+    // f: read by k = j + c_i (direction i), or, when that falls outside, by
+    // j itself through the bounce-back of direction opp(i)
+    F[i] = in ? mu[i * P + (long long)ky * pitch + kx] : mu[kOpp[i] * P + j];
+    double s = 0.0;
+    if (in && kx >= 1 && kx <= nx - 2 && ky >= 1 && ky <= ny - 2) {
+      s = mu[(9 + i) * P + (long long)ky * pitch + kx];
+      if (ky == 1) s += mu[(9 + i) * P + kx];
+      if (ky == ny - 2) s += mu[(9 + i) * P + (long long)(ny - 1) * pitch + kx];
+    }
+    G[i] = s;
+  }
 }
 
 template <int GEQ, bool FIRST>
@@ -173,13 +201,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_node(Geom geo, StepCtx c, AdjArg
   }
 
   // ---- reverse: collision ---------------------------------------------------
-  double F[9], G[9];
+  double F[9], G[9], fyb = 0.0;
+  if (a.mu_next) {
+    gather_lambda(geo, a.mu_next, ix, iy, F, G);
+    fyb = a.mu_next_fy[k];
+  } else {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    F[i] = a.lam ? a.lam[i * P + k] : 0.0;
-    G[i] = a.lam ? a.lam[(9 + i) * P + k] : 0.0;
+    for (int i = 0; i < 9; ++i) F[i] = G[i] = 0.0;
   }
-  double fyb = a.lam_fy ? a.lam_fy[k] : 0.0;
   double fb[9], gb[9], db[9], eb[9];
   double ab = 0.0, bb = 0.0, kmb = 0.0, kpb = 0.0, agb = 0.0, uxb = 0.0, uyb = 0.0;
 #pragma unroll
@@ -290,33 +319,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_node(Geom geo, StepCtx c, AdjArg
   a.mu_fy[k] = fypb;
 }
 
-// lambda_m(j): transpose of the pull with bounce-back and of the row copies.
-__global__ __launch_bounds__(kBlock) void k_adj_gather(Geom geo, const double* mu,
-                                                        const double* mu_fy, double* lam,
-                                                        double* lam_fy) {
-  int jx, jy;
-  if (!node_of(geo, jx, jy)) return;
-  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
-  const long long P = geo.plane;
-  const long long j = (long long)jy * pitch + jx;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    const int kx = jx + kCx[i], ky = jy + kCy[i];
-    const bool in = kx >= 0 && kx < nx && ky >= 0 && ky < ny;
-    // f: read by k = j + c_i (direction i), or, when that falls outside, by
-    // j itself through the bounce-back of direction opp(i)
-    lam[i * P + j] = in ? mu[i * P + (long long)ky * pitch + kx] : mu[kOpp[i] * P + j];
-    double s = 0.0;
-    if (in && kx >= 1 && kx <= nx - 2 && ky >= 1 && ky <= ny - 2) {
-      s = mu[(9 + i) * P + (long long)ky * pitch + kx];
-      if (ky == 1) s += mu[(9 + i) * P + kx];
-      if (ky == ny - 2) s += mu[(9 + i) * P + (long long)(ny - 1) * pitch + kx];
-    }
-    lam[(9 + i) * P + j] = s;
-  }
-  lam_fy[j] = mu_fy[j];
-}
-
 __global__ void k_sum_nodes(Geom geo, const double* v, double* partial) {
   __shared__ double lds[1024 / 64];
   double acc = 0.0;
@@ -362,13 +364,6 @@ hipError_t launch_adj_step(const Geom& geo, int geq, bool first, const StepCtx& 
     else
       hipLaunchKernelGGL((k_adj_node<2, false>), grid, block, 0, st, geo, ctx, a);
   }
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_gather(const Geom& geo, const double* mu, const double* mu_fy, double* lam,
-                             double* lam_fy, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(nblocks_of(geo)), dim3(kBlock), 0, st, geo, mu, mu_fy,
-                     lam, lam_fy);
   return hipGetLastError();
 }
 

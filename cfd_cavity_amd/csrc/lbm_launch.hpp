@@ -183,8 +183,8 @@ struct AdjArgs {
   const double* mux;
   const double* muy;
   const double* mtemp;
-  const double* lam;     // adjoint of X_{m+1} (18 planes), null when zero
-  const double* lam_fy;  // adjoint of fy_{m+1}, null when zero
+  const double* mu_next;     // mu of the previous backward step (19 planes), null: zero
+  const double* mu_next_fy;  // its fy plane (mu_next + 18 planes)
   const double* sgn;     // sign(T(1,iy) - T(2,iy)) of S_{m+1}, null: no objective term
   double obj_w;          // Lz / (ny dT K)
   const double* theta;   // hot-wall profile (ny)
@@ -197,8 +197,6 @@ hipError_t launch_nu_rows(const Geom& geo, int geq, const double* X, const doubl
                           double* acc, double* sgn, hipStream_t st);
 hipError_t launch_adj_step(const Geom& geo, int geq, bool first, const StepCtx& ctx,
                            const AdjArgs& a, hipStream_t st);
-hipError_t launch_adj_gather(const Geom& geo, const double* mu, const double* mu_fy, double* lam,
-                             double* lam_fy, hipStream_t st);
 hipError_t launch_sum_nodes(const Geom& geo, const double* v, double* out, hipStream_t st);
 
 // Tangent-linear (forward mode) step and Nusselt rows, see lbm_tangent.hip.

@@ -251,7 +251,11 @@ const char *cavlbm_snapshot_last_error(void);
  * the isothermal condition (solver.rs:957-963) and the Smagorinsky constant
  * cs in collide_node (solver.rs:1120) during the window.  No counterpart in
  * the reference (which has no derivatives); for calibration / inverse
- * problems on the reference's model. */
+ * problems on the reference's model.
+ * The device buffers of these calls (window states; for the adjoint the
+ * checkpoints, one recomputed segment of checkpoint_every states and the
+ * adjoint planes, about (K / C + C + 4) * 19 * nx * ny doubles) stay with
+ * the solver until cavlbm_destroy, so a repeated call allocates nothing. */
 int cavlbm_nu_window(cavlbm_solver *s, const double *theta_hot, uint64_t ny,
                      double cs, uint64_t window_steps, double *j_out);
 /* J and its gradient by the hand-written adjoint of the fused step, with

@@ -87,3 +87,33 @@ def test_adjoint_96x384_ra1e7_window_8k_10k():
     rng = np.random.default_rng(42)
     theta = 0.5 + 0.01 * rng.uniform(-1, 1, s.ny())
     fd_check(s, theta, 0.10, 2000, 100, directions(s.ny(), 16, 1234))
+
+
+def test_window_buffers_reused_across_calls():
+    """The window calls keep their device buffers in the solver between calls:
+    repeated calls, calls of other shapes (window length, checkpoint interval)
+    and of the other entry points in between give bitwise the results of a
+    fresh solver, and the accumulators start from zero on every call."""
+    theta = 0.5 + 0.01 * np.random.default_rng(3).uniform(-1, 1, 40)
+    dth, dcs = directions(40, 1, 8)[0]
+    fresh = {}
+    for key, f in [("adj", lambda s: adjoint_nu_window(s, theta, 0.1, 50, 9)),
+                   ("adj_small", lambda s: adjoint_nu_window(s, theta, 0.1, 21, 4)),
+                   ("tan", lambda s: tangent_nu_window(s, theta, 0.1, dth, dcs, 50)),
+                   ("tan_zero", lambda s: tangent_nu_window(s, theta, 0.1, None, 0.0, 33)),
+                   ("j", lambda s: nu_window(s, theta, 0.1, 64))]:
+        fresh[key] = f(spun_up(16, 40, 1e5, 200))
+    s = spun_up(16, 40, 1e5, 200)
+    order = ["adj", "adj", "j", "adj_small", "tan", "adj", "tan_zero", "j", "adj_small", "tan"]
+    calls = {"adj": lambda: adjoint_nu_window(s, theta, 0.1, 50, 9),
+             "adj_small": lambda: adjoint_nu_window(s, theta, 0.1, 21, 4),
+             "tan": lambda: tangent_nu_window(s, theta, 0.1, dth, dcs, 50),
+             "tan_zero": lambda: tangent_nu_window(s, theta, 0.1, None, 0.0, 33),
+             "j": lambda: nu_window(s, theta, 0.1, 64)}
+    for key in order:
+        got, want = calls[key](), fresh[key]
+        if key.startswith("adj"):
+            assert got[0] == want[0] and got[2] == want[2] and bits_equal(got[1], want[1]), key
+        else:
+            assert got == want, key
+    assert fresh["tan_zero"][1] == 0.0
