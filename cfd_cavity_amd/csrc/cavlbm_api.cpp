@@ -1920,12 +1920,15 @@ extern "C" int cavlbm_tangent_nu_window(cavlbm_solver* s, const double* theta_ho
   for (uint64_t m = 0; m < K; ++m) {
     const bool first = m == 0;
     const int out = first ? cur : cur ^ 1;
+    // (a step from X_{m-1} also accumulates Nu(S_m) and its tangent)
     HIP_TRY(s, launch_tangent_step(s->geo, s->geq, first, window_ctx(s, t0 + m, cs), dcs, theta,
                                    dtheta, X[cur], Xd[cur], fy, fyd, X[out], Xd[out], s->mpops,
-                                   s->fld[0], s->fld[1], s->fld[2], s->fld[3], s->st));
+                                   s->fld[0], s->fld[1], s->fld[2], s->fld[3],
+                                   first ? nullptr : acc, first ? nullptr : dacc, s->st));
     cur = out;
-    HIP_TRY(s, launch_nu_rows_dual(s->geo, X[cur], Xd[cur], theta, dtheta, acc, dacc, s->st));
   }
+  // Nu(S_K) from X_{K-1}
+  HIP_TRY(s, launch_nu_rows_dual(s->geo, X[cur], Xd[cur], theta, dtheta, acc, dacc, s->st));
   std::vector<double> h(s->p.ny), hd(s->p.ny);
   HIP_TRY(s, hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(s, hipMemcpyAsync(hd.data(), dacc, hd.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -205,7 +205,7 @@ hipError_t launch_tangent_step(const Geom& geo, int geq, bool first, const StepC
                                const double* X, const double* Xd, double* fy, double* fyd,
                                double* Y, double* Yd, const double* mpops, const double* mrho,
                                const double* mux, const double* muy, const double* mtemp,
-                               hipStream_t st);
+                               double* acc, double* dacc, hipStream_t st);
 hipError_t launch_nu_rows_dual(const Geom& geo, const double* X, const double* Xd,
                                const double* theta, const double* dtheta, double* acc,
                                double* dacc, hipStream_t st);

@@ -66,6 +66,14 @@ __device__ __forceinline__ void pulled_g_dual(const Geom& geo, const double* X,
   }
 }
 
+// acc[iy] += |T1 - T2|, dacc[iy] += d|T1 - T2| for dlt = T1 - T2.
+__device__ __forceinline__ void nu_row_add(Dual dlt, double* acc, double* dacc, int iy) {
+  acc[iy] += fabs(dlt.v);
+  dacc[iy] += dlt.v > 0.0 ? dlt.d : (dlt.v < 0.0 ? -dlt.d : 0.0);
+}
+
+// One tangent step; with acc (a step that pulls, m >= 1) it also accumulates
+// the Nusselt row terms of the state S_m it pulls, as the window step does.
 template <int GEQ, bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_tangent_step(Geom geo, StepCtx ctx, double dcs,
                                                           const double* theta,
@@ -75,7 +83,8 @@ __global__ __launch_bounds__(kBlock) void k_tangent_step(Geom geo, StepCtx ctx, 
                                                           double* Yd, const double* mpops,
                                                           const double* mrho, const double* mux,
                                                           const double* muy,
-                                                          const double* mtemp) {
+                                                          const double* mtemp, double* acc,
+                                                          double* dacc) {
   int bx, by;
   tile_of_block(geo, bx, by);
   const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
@@ -127,6 +136,16 @@ __global__ __launch_bounds__(kBlock) void k_tangent_step(Geom geo, StepCtx ctx, 
     ux = un;
     uy = vn;
     temp = tn;
+    // Nusselt row term of the pulled state (same sums as k_nu_rows_dual):
+    // node (1, iy) adds |T(1,iy) - T(2,iy)| and its tangent
+    if (acc && ix == 1) {
+      Dual g2[9];
+      pulled_g_dual(geo, X, Xd, theta, dtheta, 2, iy, g2);
+      Dual t2(0.0);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) t2 += g2[i];
+      nu_row_add(tn - t2, acc, dacc, iy);
+    }
   }
   Dual fy_new, nut_new;
   collide_node<Dual, GEQ>(f, g, rho, ux, uy, temp, ctx, Dual(ctx.cs, dcs), fy_new, nut_new);
@@ -155,9 +174,7 @@ __global__ void k_nu_rows_dual(Geom geo, const double* X, const double* Xd, cons
     t1 += g1[i];
     t2 += g2[i];
   }
-  const Dual dlt = t1 - t2;
-  acc[iy] += fabs(dlt.v);
-  dacc[iy] += dlt.v > 0.0 ? dlt.d : (dlt.v < 0.0 ? -dlt.d : 0.0);
+  nu_row_add(t1 - t2, acc, dacc, iy);
 }
 
 }  // namespace
@@ -167,11 +184,11 @@ hipError_t launch_tangent_step(const Geom& geo, int geq, bool first, const StepC
                                const double* X, const double* Xd, double* fy, double* fyd,
                                double* Y, double* Yd, const double* mpops, const double* mrho,
                                const double* mux, const double* muy, const double* mtemp,
-                               hipStream_t st) {
+                               double* acc, double* dacc, hipStream_t st) {
   const dim3 grid(nblocks_of(geo)), block(kBlock);
 #define CAVLBM_T(G, F)                                                                      \
   hipLaunchKernelGGL((k_tangent_step<G, F>), grid, block, 0, st, geo, ctx, dcs, theta, dtheta, \
-                     X, Xd, fy, fyd, Y, Yd, mpops, mrho, mux, muy, mtemp)
+                     X, Xd, fy, fyd, Y, Yd, mpops, mrho, mux, muy, mtemp, acc, dacc)
   if (geq == 1) {
     if (first) CAVLBM_T(1, true); else CAVLBM_T(1, false);
   } else {
