@@ -85,6 +85,8 @@ struct cavlbm_solver {
   // Window objective / adjoint / tangent buffers (states, checkpoints, the
   // recomputed segment, adjoint planes), kept between calls until destroy.
   std::vector<WinSlot> win;
+  hipStream_t st2 = nullptr;  // adjoint: recomputation beside the backward sweep
+  hipEvent_t wev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::string err;
 };
 
@@ -497,6 +499,9 @@ void free_all(cavlbm_solver* s) {
   f(s->dyn);
   for (auto& w : s->win) f(w.p);
   s->win.clear();
+  for (hipEvent_t e : s->wev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->st2) (void)hipStreamDestroy(s->st2);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -1693,7 +1698,7 @@ StepCtx window_ctx(const cavlbm_solver* s, uint64_t step, double cs) {
 hipError_t window_step(cavlbm_solver* s, const double* src, double* dst, double* fy,
                        double* nut, const double* theta, const StepCtx& ctx,
                        const double* fy_in = nullptr, double* acc = nullptr,
-                       double* sgn = nullptr) {
+                       double* sgn = nullptr, hipStream_t st = nullptr) {
   StepBufs b{};
   b.src = src;
   b.dst = dst;
@@ -1708,8 +1713,23 @@ hipError_t window_step(cavlbm_solver* s, const double* src, double* dst, double*
   b.uy = s->fld[2];
   b.temp = s->fld[3];
   return launch_step_fp64(s->geq, src ? MODE_WINDOW : MODE_FIRST, s->geo, ctx, Walls{theta}, b,
-                          0, s->st);
+                          0, st ? st : s->st);
 }
+
+// The second stream and the events of the adjoint's overlapped recomputation,
+// created on first use (default priority, like the solver stream).
+int window_streams(cavlbm_solver* s) {
+  if (!s->st2) HIP_TRY(s, hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking));
+  for (hipEvent_t& e : s->wev)
+    if (!e) HIP_TRY(s, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return 0;
+}
+
+// Waits for the second stream when leaving the adjoint, on every return path.
+struct AuxStreamJoin {
+  hipStream_t st;
+  ~AuxStreamJoin() { (void)hipStreamSynchronize(st); }
+};
 
 int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K,
                  WindowWork& w, double*& theta) {
@@ -1803,49 +1823,81 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
 
   const size_t P = (size_t)s->geo.plane;
   const uint64_t t0 = s->time_step;
+  const uint64_t rows = s->p.ny;
   const uint64_t seg_len = std::min<uint64_t>(C, K);
-  std::vector<double*> seg(seg_len);
-  for (auto& q : seg)
-    if (!(q = w.alloc<double>(19 * P))) return fail(s, CAVLBM_E_HIP, "segment allocation failed");
+  const uint64_t nseg = (K + C - 1) / C;
+  // Two segment buffers (states and signs): the recomputation of segment q-1
+  // runs on the second stream while the backward sweep of segment q reads the
+  // other buffer on the solver stream (both are small launches that leave
+  // most CUs idle on their own).
+  std::vector<double*> seg[2] = {std::vector<double*>(seg_len), std::vector<double*>(seg_len)};
+  double* sgn[2] = {nullptr, nullptr};
+  for (int b = 0; b < (nseg > 1 ? 2 : 1); ++b) {
+    for (auto& q : seg[b])
+      if (!(q = w.alloc<double>(19 * P))) return fail(s, CAVLBM_E_HIP, "segment allocation failed");
+    // sign of T(1,iy) - T(2,iy) of S_{m+1} for each m of a segment
+    if (!(sgn[b] = w.alloc<double>(seg_len * rows)))
+      return fail(s, CAVLBM_E_HIP, "segment allocation failed");
+  }
   // mu_m of the backward step m, alternating (step m gathers mu_{m+1})
   double* mu[2] = {w.alloc<double>(19 * P), w.alloc<double>(19 * P)};
   double* nut = w.alloc<double>(P);
-  // sign of T(1,iy) - T(2,iy) of S_{m+1} for each m of a segment
-  double* sgn = w.alloc<double>(seg_len * s->p.ny);
-  double* th_acc = w.zeroed<double>(s->p.ny);
+  double* th_acc = w.zeroed<double>(rows);
   double* cs_acc = w.zeroed<double>(P);
   double* cs_sum = w.alloc<double>(1);
-  if (!mu[0] || !mu[1] || !nut || !sgn || !th_acc || !cs_acc || !cs_sum)
+  if (!mu[0] || !mu[1] || !nut || !th_acc || !cs_acc || !cs_sum)
     return fail(s, CAVLBM_E_HIP, "adjoint allocation failed");
-  const double obj_w = s->d.lz / ((double)s->p.ny * 1.0 * (double)K);
+  rc = window_streams(s);
+  if (rc) return rc;
+  hipStream_t st2 = s->st2;
+  hipEvent_t* ev = s->wev;  // 0: forward done; 1 + b: segment b recomputed; 3 + b: b swept
+  AuxStreamJoin join{st2};   // no work of this call outlives it, also on an error return
+  const double obj_w = s->d.lz / ((double)rows * 1.0 * (double)K);
+
+  // X_m, fy_m for m in [m_lo, m_hi) of segment q from its checkpoint into
+  // buffer b, on the second stream (step m pulls X_{m-1} and stores the signs
+  // of S_m, used at m - 1)
+  auto recompute = [&](uint64_t q, int b) -> int {
+    const uint64_t m_lo = q * C, m_hi = std::min<uint64_t>(m_lo + C, K);
+    std::vector<double*>& sb = seg[b];
+    HIP_TRY(s, hipMemcpyAsync(sb[0], ckpt[q], 19 * P * sizeof(double), hipMemcpyDeviceToDevice,
+                              st2));
+    for (uint64_t m = m_lo + 1; m < m_hi; ++m) {
+      double* prev = sb[m - 1 - m_lo];
+      double* next = sb[m - m_lo];
+      HIP_TRY(s, window_step(s, prev, next, next + 18 * P, nut, theta, window_ctx(s, t0 + m, cs),
+                             prev + 18 * P, nullptr, sgn[b] + (m - 1 - m_lo) * rows, st2));
+    }
+    HIP_TRY(s, launch_nu_rows(s->geo, s->geq, sb[m_hi - 1 - m_lo], theta, nullptr,
+                              sgn[b] + (m_hi - 1 - m_lo) * rows, st2));
+    HIP_TRY(s, hipEventRecord(ev[1 + b], st2));
+    return 0;
+  };
 
   AdjArgs a{};
   a.theta = theta;
   a.theta_acc = th_acc;
   a.cs_acc = cs_acc;
   a.obj_w = obj_w;
-  a.sgn = sgn;
   int mc = 0;              // step m writes mu[mc]; mu[mc ^ 1] holds mu_{m+1}
   bool lam_zero = true;    // X_K is not used by J: its adjoint is zero
-  const uint64_t nseg = (K + C - 1) / C;
+  HIP_TRY(s, hipEventRecord(ev[0], s->st));  // checkpoints and buffers ready
+  HIP_TRY(s, hipStreamWaitEvent(st2, ev[0], 0));
+  rc = recompute(nseg - 1, (int)((nseg - 1) & 1));
+  if (rc) return rc;
   for (uint64_t q = nseg; q-- > 0;) {
-    const uint64_t m_lo = q * C, m_hi = std::min<uint64_t>(m_lo + C, K);
-    // recompute X_m, fy_m for m in [m_lo, m_hi) from the checkpoint
-    HIP_TRY(s, hipMemcpyAsync(seg[0], ckpt[q], 19 * P * sizeof(double), hipMemcpyDeviceToDevice,
-                              s->st));
-    // (step m pulls X_{m-1} and stores the signs of S_m, used at m - 1)
-    const uint64_t ny = s->p.ny;
-    for (uint64_t m = m_lo + 1; m < m_hi; ++m) {
-      double* prev = seg[m - 1 - m_lo];
-      double* next = seg[m - m_lo];
-      HIP_TRY(s, window_step(s, prev, next, next + 18 * P, nut, theta, window_ctx(s, t0 + m, cs),
-                             prev + 18 * P, nullptr, sgn + (m - 1 - m_lo) * ny));
+    const int b = (int)(q & 1);
+    HIP_TRY(s, hipStreamWaitEvent(s->st, ev[1 + b], 0));
+    if (q > 0) {
+      // buffer b ^ 1 was last read by the sweep of segment q + 1
+      if (q + 1 < nseg) HIP_TRY(s, hipStreamWaitEvent(st2, ev[3 + (b ^ 1)], 0));
+      rc = recompute(q - 1, b ^ 1);
+      if (rc) return rc;
     }
-    HIP_TRY(s, launch_nu_rows(s->geo, s->geq, seg[m_hi - 1 - m_lo], theta, nullptr,
-                              sgn + (m_hi - 1 - m_lo) * ny, s->st));
+    const uint64_t m_lo = q * C, m_hi = std::min<uint64_t>(m_lo + C, K);
     for (uint64_t m = m_hi; m-- > m_lo;) {
-      double* Xm = seg[m - m_lo];
-      a.sgn = sgn + (m - m_lo) * ny;
+      double* Xm = seg[b][m - m_lo];
+      a.sgn = sgn[b] + (m - m_lo) * rows;
       a.X = Xm;
       a.fyp = Xm + 18 * P;
       a.mu_next = lam_zero ? nullptr : mu[mc ^ 1];
@@ -1856,6 +1908,7 @@ int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t
       mc ^= 1;
       lam_zero = false;
     }
+    HIP_TRY(s, hipEventRecord(ev[3 + b], s->st));
   }
   // the first collision X_0 = Collide(S_0): its Cs dependence
   a.X = nullptr;

@@ -253,8 +253,9 @@ const char *cavlbm_snapshot_last_error(void);
  * the reference (which has no derivatives); for calibration / inverse
  * problems on the reference's model.
  * The device buffers of these calls (window states; for the adjoint the
- * checkpoints, one recomputed segment of checkpoint_every states and the
- * adjoint planes, about (K / C + C + 4) * 19 * nx * ny doubles) stay with
+ * checkpoints, two recomputed segments of checkpoint_every states -- the
+ * recomputation of one overlaps the backward sweep of the other -- and the
+ * adjoint planes, about (K / C + 2 C + 4) * 19 * nx * ny doubles) stay with
  * the solver until cavlbm_destroy, so a repeated call allocates nothing. */
 int cavlbm_nu_window(cavlbm_solver *s, const double *theta_hot, uint64_t ny,
                      double cs, uint64_t window_steps, double *j_out);
