@@ -290,7 +290,7 @@ hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgs
   }
   hipError_t e = hipSuccess;
   if (!s->m_valid) e = launch(s, MODE_MATERIALIZE_F, ctx, 0);
-  if (e == hipSuccess) e = launch_dynamic_sgs(s->geo, args, false, s->st);
+  if (e == hipSuccess) e = launch_dynamic_sgs(&s->geo, &args, 1, false, s->st);
   if (e != hipSuccess) return e;
   StepBufs b = bufs(s);
   b.dc = dyn_coef_of(args);
@@ -318,7 +318,8 @@ int dyn_step(cavlbm_solver* s, int write_aux) {
   int rc = materialize(s);
   if (rc) return rc;
   if (s->dyn_on) {
-    HIP_TRY(s, launch_dynamic_sgs(s->geo, dyn_args(s, ctx), true, s->st));
+    const DynSgsArgs da = dyn_args(s, ctx);
+    HIP_TRY(s, launch_dynamic_sgs(&s->geo, &da, 1, true, s->st));
     s->dyn_pre = true;
   }
   HIP_TRY(s, launch(s, MODE_FIRST_PRE, ctx, write_aux));

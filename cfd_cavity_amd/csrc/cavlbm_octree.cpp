@@ -156,6 +156,7 @@ struct cavlbm_octree {
   bool dyn_on = false;
   double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
   double* dyn[5] = {};
+  double* dyn_part[5] = {};  // per level: the coefficient pass's per-workgroup sums
   OctXi* xi_dev = nullptr;
   std::string err;
 };
@@ -311,7 +312,9 @@ hipError_t double_substep(cavlbm_octree* o, int g0, int count, const OctCollide&
 // compute_dynamic_sgs_level (smag.rs:144-416) on grid gi's current M-form:
 // the uniform model's filter, contractions and coefficient passes with the
 // level's molecular tau, nu and alpha; nu_t / alpha_t land in planes 18 / 19.
-hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, double alpha) {
+// The model passes' geometry of level gi (flat layout, kTileX x kTileY
+// workgroups).
+Geom model_geom(const cavlbm_octree* o, int gi) {
   const OctLevel& L = o->lv[gi];
   Geom geo{};
   geo.nx = L.nx;
@@ -320,8 +323,15 @@ hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, doub
   geo.nbx = (L.nx + kTileX - 1) / kTileX;
   geo.nby = (L.ny + kTileY - 1) / kTileY;
   geo.plane = L.plane;
+  return geo;
+}
+
+void level_model_args(const cavlbm_octree* o, int gi, double tau_mol, double nu, double alpha,
+                      Geom& geo, DynSgsArgs& a) {
+  const OctLevel& L = o->lv[gi];
+  geo = model_geom(o, gi);
   const size_t P = (size_t)L.plane;
-  DynSgsArgs a{};
+  a = DynSgsArgs{};
   a.mpops = L.m;
   a.rho = L.m + kOctRho * P;
   a.ux = L.m + kOctUx * P;
@@ -356,7 +366,17 @@ hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, doub
   a.ck = q + 17 * P;
   a.nut_pre = q + 18 * P;
   a.at_pre = q + 19 * P;
-  return launch_dynamic_sgs(geo, a, true, o->st);
+  a.part = o->dyn_part[gi];
+}
+
+// The model on level gi, or on levels gi and gi + 1 (the hot and cold blocks
+// or strips, same molecular nu / alpha) in the same launches.
+hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, double alpha,
+                       int count = 1) {
+  Geom geo[2];
+  DynSgsArgs a[2];
+  for (int i = 0; i < count; ++i) level_model_args(o, gi + i, tau_mol, nu, alpha, geo[i], a[i]);
+  return launch_dynamic_sgs(geo, a, count, true, o->st);
 }
 
 const double* nut_of(const cavlbm_octree* o, int gi) {
@@ -381,12 +401,12 @@ void with_model(const cavlbm_octree* o, OctCollide& c, int g0, int count, const 
 OctXiUpdate xi_update(const cavlbm_octree* o, int gc, const int* gf, int nf, double nu_c,
                       double alpha_c, double nu_f, double alpha_f, OctXi* out) {
   OctXiUpdate u{};
-  u.nut_c = nut_of(o, gc);
-  u.at_c = at_of(o, gc);
+  u.part_c = o->dyn_part[gc];
+  u.nb_c = nblocks_of(model_geom(o, gc));
   u.n_c = o->lv[gc].plane;
   for (int i = 0; i < nf; ++i) {
-    u.nut_f[i] = nut_of(o, gf[i]);
-    u.at_f[i] = at_of(o, gf[i]);
+    u.part_f[i] = o->dyn_part[gf[i]];
+    u.nb_f[i] = nblocks_of(model_geom(o, gf[i]));
     u.n_f[i] = o->lv[gf[i]].plane;
   }
   u.nf = nf;
@@ -461,8 +481,7 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
   pair_of(lf, 1);
   OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[1], 2, o->xi_a, xi_a, o->geq, o->st));
   if (dyn) {  // the blocks' model before sub-step 1 (solver.rs:767-786)
-    OCT_Q(level_model(o, 1, tau_l1, 2.0 * nu_eff, 2.0 * alpha_eff));
-    OCT_Q(level_model(o, 2, tau_l1, 2.0 * nu_eff, 2.0 * alpha_eff));
+    OCT_Q(level_model(o, 1, tau_l1, 2.0 * nu_eff, 2.0 * alpha_eff, 2));
   }
   if (o->levels == 3) {
     OCT_Q(substep(o, 1, 2, c1));
@@ -480,8 +499,7 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
     OCT_Q(launch_oct_coarse_to_fine(lc, lf, &o->gt[3], 2, o->xi_b, xi_b, o->geq, o->st));
     if (dyn) {
       for (int sp = 0; sp < 2; ++sp) {  // the strips' model before each sub-step (:834-850)
-        OCT_Q(level_model(o, 3, tau_l2, 4.0 * nu_eff, 4.0 * alpha_eff));
-        OCT_Q(level_model(o, 4, tau_l2, 4.0 * nu_eff, 4.0 * alpha_eff));
+        OCT_Q(level_model(o, 3, tau_l2, 4.0 * nu_eff, 4.0 * alpha_eff, 2));
         OCT_Q(substep(o, 3, 2, c2));
       }
     } else {
@@ -491,14 +509,12 @@ hipError_t enqueue_step(cavlbm_octree* o, uint64_t step) {
     pair_of(lc, 1);
     OCT_Q(launch_oct_fine_to_coarse(lf, lc, &o->ct[3], 2, o->xi_b, xi_b, o->geq, o->st));
     if (dyn) {  // before sub-step 2 the reference passes the unramped nu / alpha (:879-892)
-      OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
-      OCT_Q(level_model(o, 2, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
+      OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu, 2));
     }
     OCT_Q(substep(o, 1, 2, c1));
   } else if (dyn) {
     OCT_Q(substep(o, 1, 2, c1));
-    OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
-    OCT_Q(level_model(o, 2, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu));
+    OCT_Q(level_model(o, 1, tau_l1, 2.0 * o->nu_lu, 2.0 * o->alpha_lu, 2));
     OCT_Q(substep(o, 1, 2, c1));
   } else {
     OCT_Q(double_substep(o, 1, 2, c1));
@@ -1041,8 +1057,12 @@ int cavlbm_octree_set_dynamic_sgs(cavlbm_octree* o, int32_t enable, double smag_
     for (int gi = 0; gi < o->ngrids; ++gi) {
       const size_t n = (size_t)kDynPlanes * (size_t)o->lv[gi].plane;
       o->dyn[gi] = dalloc<double>(o, n);
-      if (!o->dyn[gi]) return ofail(o, CAVLBM_E_HIP, "hipMalloc of the dynamic SGS planes");
+      const size_t np = 2 * (size_t)nblocks_of(model_geom(o, gi));
+      o->dyn_part[gi] = dalloc<double>(o, np);
+      if (!o->dyn[gi] || !o->dyn_part[gi])
+        return ofail(o, CAVLBM_E_HIP, "hipMalloc of the dynamic SGS planes");
       OHIP_TRY(o, hipMemsetAsync(o->dyn[gi], 0, n * sizeof(double), o->st));
+      OHIP_TRY(o, hipMemsetAsync(o->dyn_part[gi], 0, np * sizeof(double), o->st));
     }
     OctXi* x = dalloc<OctXi>(o, 3);
     if (!x) return ofail(o, CAVLBM_E_HIP, "hipMalloc of the interface factors");

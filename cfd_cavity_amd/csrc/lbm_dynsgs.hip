@@ -220,16 +220,36 @@ struct Plane {
   __device__ __forceinline__ double operator()(int x, int y) const { return p[y * pitch + x]; }
 };
 
-__device__ __forceinline__ int node_of_thread(const Geom& geo, int& ix, int& iy) {
-  const int bx = (int)blockIdx.x % geo.nbx, by = (int)blockIdx.x / geo.nbx;
+// The three passes take one or two lattices per launch (the octree's hot and
+// cold blocks): blocks [0, nb0) work on item 0, the others on item 1.
+struct DynSgsPair {
+  Geom geo[2];
+  DynSgsArgs a[2];
+  int nb0;
+};
+
+// This block's item, and the node of this thread in it.
+__device__ __forceinline__ int pair_node(const DynSgsPair& p, int& ix, int& iy, bool& in,
+                                         int& blk) {
+  const int b = (int)blockIdx.x;
+  const int it = b < p.nb0 ? 0 : 1;
+  blk = it ? b - p.nb0 : b;
+  const Geom& geo = p.geo[it];
+  const int bx = blk % geo.nbx, by = blk / geo.nbx;
   ix = bx * kTileX + (int)(threadIdx.x % kTileX);
   iy = by * kTileY + (int)(threadIdx.x / kTileX);
-  return ix < geo.nx && iy < geo.ny;
+  in = ix < geo.nx && iy < geo.ny;
+  return it;
 }
 
-__global__ __launch_bounds__(kBlock) void k_dyn_filter(Geom geo, DynSgsArgs a) {
+__global__ __launch_bounds__(kBlock) void k_dyn_filter(DynSgsPair p) {
   int ix, iy;
-  if (!node_of_thread(geo, ix, iy)) return;
+  bool in;
+  int blk;
+  const int it = pair_node(p, ix, iy, in, blk);
+  if (!in) return;
+  const Geom& geo = p.geo[it];
+  const DynSgsArgs& a = p.a[it];
   const int k = iy * geo.pitch + ix;
   const DynWin w = dyn_window(geo, a.width, a.half_left, ix, iy);
   double s_u = 0.0, s_v = 0.0, s_t = 0.0, s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0,
@@ -259,9 +279,14 @@ __global__ __launch_bounds__(kBlock) void k_dyn_filter(Geom geo, DynSgsArgs a) {
   dyn_similarity(a.ux[k], a.uy[k], a.temp[k], ubx, uby, tb, a.qsm[k], a.phi[k]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
+__global__ __launch_bounds__(kBlock) void k_dyn_terms(DynSgsPair p) {
   int ix, iy;
-  if (!node_of_thread(geo, ix, iy)) return;
+  bool in;
+  int blk;
+  const int it = pair_node(p, ix, iy, in, blk);
+  if (!in) return;
+  const Geom& geo = p.geo[it];
+  const DynSgsArgs& a = p.a[it];
   const int k = iy * geo.pitch + ix;
   const long long P = geo.plane;
   double fp[9];
@@ -282,11 +307,40 @@ __global__ __launch_bounds__(kBlock) void k_dyn_terms(Geom geo, DynSgsArgs a) {
   a.kap_mix_g[k] = r.kap_mix_g;
 }
 
-__global__ __launch_bounds__(kBlock) void k_dyn_coeffs(Geom geo, DynSgsArgs a) {
+__global__ __launch_bounds__(kBlock) void k_dyn_coeffs(DynSgsPair p) {
   int ix, iy;
-  if (!node_of_thread(geo, ix, iy)) return;
-  double nut, at;
-  dyn_coef_node(geo, dyn_coef_of(a), ix, iy, true, nut, at);
+  bool in;
+  int blk;
+  const int it = pair_node(p, ix, iy, in, blk);
+  const Geom& geo = p.geo[it];
+  const DynSgsArgs& a = p.a[it];
+  double nut = 0.0, at = 0.0;
+  if (in) dyn_coef_node(geo, dyn_coef_of(a), ix, iy, true, nut, at);
+  if (!a.part) return;  // uniform over the workgroup
+  // the workgroup's sums: each wave's by a butterfly of lane exchanges, then
+  // the waves' in order
+  constexpr int kWaves = kBlock / 64;
+  __shared__ double red[2 * kWaves];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nut += __shfl_xor(nut, off, 64);
+    at += __shfl_xor(at, off, 64);
+  }
+  const int w = (int)threadIdx.x / 64;
+  if (threadIdx.x % 64 == 0) {
+    red[w] = nut;
+    red[kWaves + w] = at;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sn = 0.0, sa = 0.0;
+    for (int i = 0; i < kWaves; ++i) {
+      sn += red[i];
+      sa += red[kWaves + i];
+    }
+    a.part[blk] = sn;
+    a.part[geo.nbx * geo.nby + blk] = sa;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -542,11 +596,21 @@ hipError_t by_width(int width, const Geom& geo, const StepCtx& ctx, const Walls&
 
 }  // namespace
 
-hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs, hipStream_t st) {
-  const dim3 grid(nblocks_of(geo)), block(kBlock);
-  hipLaunchKernelGGL(k_dyn_filter, grid, block, 0, st, geo, a);
-  hipLaunchKernelGGL(k_dyn_terms, grid, block, 0, st, geo, a);
-  if (coeffs) hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, geo, a);
+hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, bool coeffs,
+                              hipStream_t st) {
+  if (count < 1 || count > 2) return hipErrorInvalidValue;
+  DynSgsPair p{};
+  int nb = 0;
+  for (int i = 0; i < count; ++i) {
+    p.geo[i] = geo[i];
+    p.a[i] = a[i];
+    nb += nblocks_of(geo[i]);
+  }
+  p.nb0 = nblocks_of(geo[0]);
+  const dim3 grid(nb), block(kBlock);
+  hipLaunchKernelGGL(k_dyn_filter, grid, block, 0, st, p);
+  hipLaunchKernelGGL(k_dyn_terms, grid, block, 0, st, p);
+  if (coeffs) hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, p);
   return hipGetLastError();
 }
 

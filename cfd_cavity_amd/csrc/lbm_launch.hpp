@@ -103,12 +103,17 @@ struct DynSgsArgs {
   double* ck;     // out: C_kappa per node
   double* nut_pre;  // out: nu_t per node
   double* at_pre;   // out: alpha_t per node
+  // out (coefficient pass, when set): per workgroup of the pass, the sum of
+  // its nodes' nu_t at part[b] and of alpha_t at part[nblocks + b]
+  double* part;
 };
 constexpr int kDynPlanes = 20;  // planes of scratch + outputs above
 // The model's passes: the filter and the terms, then (coeffs) the
 // coefficient pass; without it the collision runs that stage itself
-// (MODE_FUSED_DYN with the DynCoef of dyn_coef_of).
-hipError_t launch_dynamic_sgs(const Geom& geo, const DynSgsArgs& a, bool coeffs, hipStream_t st);
+// (MODE_FUSED_DYN with the DynCoef of dyn_coef_of).  count (1 or 2)
+// independent lattices share each launch.
+hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, bool coeffs,
+                              hipStream_t st);
 // One launch per dynamic step from the FP64 C-form (src -> dst): streaming,
 // the model's passes, the coefficient stage and the collision, reading the
 // force plane fy_in and writing fy_out (tiles read their halo's fy).  With

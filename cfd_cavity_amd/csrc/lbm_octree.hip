@@ -595,15 +595,17 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_f2c(OctXferGroup grp, OctXi x
 inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kOctBlock - 1) / kOctBlock)); }
 
 // Interface factors from the levels' mean nu_t / alpha_t: one workgroup per
-// update.  A mean is a fixed-order sum: thread t adds elements t, t +
-// kXiThreads, ... from +0 in that order (each thread issues kXiBatch loads
-// before adding them), each wave adds its 64 partials by a butterfly of lane
-// exchanges (a + b and b + a are the same IEEE sum, so every lane holds the
-// same total), and the first wave adds the waves' totals the same way.  A
-// level's nu_t and alpha_t are reduced together.
-constexpr int kXiThreads = 1024;
+// update, over the per-workgroup sums of the coefficient pass.  A level's
+// total is a fixed-order sum: thread t adds entries t, t + kXiThreads, ...
+// from +0 in that order, each wave adds its 64 partials by a butterfly of
+// lane exchanges (a + b and b + a are the same IEEE sum, so every lane holds
+// the same total), and the first wave adds the waves' totals the same way.
+// The coarse level and the fine levels of an update (up to kXiSets, each with
+// nu_t and alpha_t) are reduced together, behind one barrier.
+constexpr int kXiThreads = 256;
 constexpr int kXiWaves = kXiThreads / 64;
-constexpr int kXiBatch = 16;
+constexpr int kXiSets = 3;
+constexpr int kXiBatch = 4;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -611,60 +613,83 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__device__ void block_mean2(const double* a, const double* b, long long n, double* red,
-                            double& ma, double& mb) {
+// Sums of a[s][0..len[s]) and b[s][0..len[s]) for s < ns, divided by div[s].
+__device__ void block_means(const double* const (&a)[kXiSets], const double* const (&b)[kXiSets],
+                            const long long (&len)[kXiSets], const long long (&div)[kXiSets],
+                            int ns, double* red, double (&ma)[kXiSets], double (&mb)[kXiSets]) {
   const int t = (int)threadIdx.x;
-  double sa = 0.0, sb = 0.0;
-  for (long long k0 = t; k0 < n; k0 += (long long)kXiBatch * kXiThreads) {
-    double va[kXiBatch], vb[kXiBatch];
+  double sa[kXiSets], sb[kXiSets];
+  long long nmax = 0;
 #pragma unroll
-    for (int j = 0; j < kXiBatch; ++j) {
-      const long long k = k0 + (long long)j * kXiThreads;
-      va[j] = k < n ? a[k] : 0.0;
-      vb[j] = k < n ? b[k] : 0.0;
+  for (int s = 0; s < kXiSets; ++s) {
+    sa[s] = 0.0;
+    sb[s] = 0.0;
+    if (s < ns && len[s] > nmax) nmax = len[s];
+  }
+  for (long long k0 = t; k0 < nmax; k0 += (long long)kXiBatch * kXiThreads) {
+    double va[kXiSets][kXiBatch], vb[kXiSets][kXiBatch];
+#pragma unroll
+    for (int s = 0; s < kXiSets; ++s)
+#pragma unroll
+      for (int j = 0; j < kXiBatch; ++j) {
+        const long long k = k0 + (long long)j * kXiThreads;
+        const bool in = s < ns && k < len[s];
+        va[s][j] = in ? a[s][k] : 0.0;
+        vb[s][j] = in ? b[s][k] : 0.0;
+      }
+#pragma unroll
+    for (int s = 0; s < kXiSets; ++s)
+#pragma unroll
+      for (int j = 0; j < kXiBatch; ++j) {
+        if (s < ns && k0 + (long long)j * kXiThreads < len[s]) {
+          sa[s] += va[s][j];
+          sb[s] += vb[s][j];
+        }
+      }
+  }
+  const int w = t / 64, lane = t % 64;
+#pragma unroll
+  for (int s = 0; s < kXiSets; ++s) {
+    const double ta = wave_sum(sa[s]), tb = wave_sum(sb[s]);
+    if (lane == 0) {
+      red[(2 * s) * kXiWaves + w] = ta;
+      red[(2 * s + 1) * kXiWaves + w] = tb;
     }
-#pragma unroll
-    for (int j = 0; j < kXiBatch; ++j) {
-      if (k0 + (long long)j * kXiThreads < n) {
-        sa += va[j];
-        sb += vb[j];
+  }
+  __syncthreads();
+  double* tot = red + 2 * kXiSets * kXiWaves;
+  if (w == 0) {
+    for (int s = 0; s < ns; ++s) {
+      const double ta = wave_sum(lane < kXiWaves ? red[(2 * s) * kXiWaves + lane] : 0.0);
+      const double tb = wave_sum(lane < kXiWaves ? red[(2 * s + 1) * kXiWaves + lane] : 0.0);
+      if (lane == 0) {
+        tot[2 * s] = ta;
+        tot[2 * s + 1] = tb;
       }
     }
   }
-  sa = wave_sum(sa);
-  sb = wave_sum(sb);
-  const int w = t / 64, lane = t % 64;
-  if (lane == 0) {
-    red[w] = sa;
-    red[kXiWaves + w] = sb;
+  __syncthreads();
+  for (int s = 0; s < ns; ++s) {
+    ma[s] = tot[2 * s] / (double)div[s];
+    mb[s] = tot[2 * s + 1] / (double)div[s];
   }
-  __syncthreads();
-  double ta = 0.0, tb = 0.0;
-  if (w == 0) {
-    ta = wave_sum(lane < kXiWaves ? red[lane] : 0.0);
-    tb = wave_sum(lane < kXiWaves ? red[kXiWaves + lane] : 0.0);
-    if (lane == 0) {
-      red[2 * kXiWaves] = ta;
-      red[2 * kXiWaves + 1] = tb;
-    }
-  }
-  __syncthreads();
-  ma = red[2 * kXiWaves] / (double)n;
-  mb = red[2 * kXiWaves + 1] / (double)n;
-  __syncthreads();
 }
 
 __global__ __launch_bounds__(kXiThreads) void k_oct_xi_update(OctXiUpdate u0, OctXiUpdate u1) {
-  __shared__ double red[2 * kXiWaves + 2];
+  __shared__ double red[2 * kXiSets * kXiWaves + 2 * kXiSets];
   const OctXiUpdate& u = blockIdx.x == 0 ? u0 : u1;
-  double nt_c, at_c;
-  block_mean2(u.nut_c, u.at_c, u.n_c, red, nt_c, at_c);
+  const double* const a[kXiSets] = {u.part_c, u.part_f[0], u.part_f[1]};
+  const double* const b[kXiSets] = {u.part_c + u.nb_c, u.part_f[0] + u.nb_f[0],
+                                    u.part_f[1] + u.nb_f[1]};
+  const long long len[kXiSets] = {u.nb_c, u.nb_f[0], u.nb_f[1]};
+  const long long div[kXiSets] = {u.n_c, u.n_f[0], u.n_f[1]};
+  double ma[kXiSets], mb[kXiSets];
+  block_means(a, b, len, div, 1 + u.nf, red, ma, mb);
+  const double nt_c = ma[0], at_c = mb[0];
   double nt_f = 0.0, at_f = 0.0;
   for (int i = 0; i < u.nf; ++i) {
-    double nt, at;
-    block_mean2(u.nut_f[i], u.at_f[i], u.n_f[i], red, nt, at);
-    nt_f += nt;
-    at_f += at;
+    nt_f += ma[1 + i];
+    at_f += mb[1 + i];
   }
   if (threadIdx.x != 0) return;
   const double nf = (double)u.nf;

@@ -81,15 +81,15 @@ struct OctXi {
 //                  relax(m_f nu + <nu_t>_f, m_f alpha + <alpha_t>_f))
 // where <.>_c is the mean over the coarse level and <.>_f the average over
 // the fine levels given (the two blocks for interface A: their means added
-// in order, then / 2).  Means are deterministic tree sums (not the
-// reference's sequential order).
+// in order, then / 2).  A level's mean is the sum of the coefficient pass's
+// per-workgroup sums (part: nu_t at [0, nb), alpha_t at [nb, 2 nb)), added
+// by a fixed tree, over the level's n nodes: deterministic, not the
+// reference's sequential order.
 struct OctXiUpdate {
-  const double* nut_c;
-  const double* at_c;
-  long long n_c;
-  const double* nut_f[2];
-  const double* at_f[2];
-  long long n_f[2];
+  const double* part_c;
+  long long nb_c, n_c;
+  const double* part_f[2];
+  long long nb_f[2], n_f[2];
   int nf;                 // fine levels averaged (1 or 2)
   double nu_c, alpha_c;   // molecular (ramped) nu / alpha of each side
   double nu_f, alpha_f;
