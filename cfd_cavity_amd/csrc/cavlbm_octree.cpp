@@ -326,6 +326,17 @@ Geom model_geom(const cavlbm_octree* o, int gi) {
   return geo;
 }
 
+// smag.rs:170-175: eta = max(eta_test, 1), width = round(eta) nodes (from
+// 2 max(nx, ny) + 1 on, every width averages the whole level).
+int level_width(const cavlbm_octree* o, int gi) {
+  const OctLevel& L = o->lv[gi];
+  const double eta = std::fmax(o->dyn_eta, 1.0);
+  const double cap = 2.0 * (double)std::max(L.nx, L.ny) + 1.0;
+  const double wr = std::round(eta);
+  const int w = (int)(wr < cap ? wr : cap);
+  return w < 1 ? 1 : w;
+}
+
 void level_model_args(const cavlbm_octree* o, int gi, double tau_mol, double nu, double alpha,
                       Geom& geo, DynSgsArgs& a) {
   const OctLevel& L = o->lv[gi];
@@ -338,15 +349,9 @@ void level_model_args(const cavlbm_octree* o, int gi, double tau_mol, double nu,
   a.uy = L.m + kOctUy * P;
   a.temp = L.m + kOctT * P;
   a.tau = tau_mol;
-  // smag.rs:170-175: eta = max(eta_test, 1), width = round(eta) nodes (from
-  // 2 max(nx, ny) + 1 on, every width averages the whole level)
-  const double eta = std::fmax(o->dyn_eta, 1.0);
-  const double cap = 2.0 * (double)std::max(L.nx, L.ny) + 1.0;
-  const double wr = std::round(eta);
-  a.width = (int)(wr < cap ? wr : cap);
-  if (a.width < 1) a.width = 1;
+  a.width = level_width(o, gi);
   a.half_left = (a.width - 1) / 2;
-  a.dtest = eta * 1.0;
+  a.dtest = std::fmax(o->dyn_eta, 1.0) * 1.0;
   a.alpha_exp = o->dyn_alpha;
   a.beta_exp = 1.0 - o->dyn_alpha;
   const double margin = (kTauTrtFloor - 0.5) / 3.0;
@@ -376,7 +381,7 @@ hipError_t level_model(cavlbm_octree* o, int gi, double tau_mol, double nu, doub
   Geom geo[2];
   DynSgsArgs a[2];
   for (int i = 0; i < count; ++i) level_model_args(o, gi + i, tau_mol, nu, alpha, geo[i], a[i]);
-  return launch_dynamic_sgs(geo, a, count, true, o->st);
+  return launch_dynamic_model(geo, a, count, o->st);
 }
 
 const double* nut_of(const cavlbm_octree* o, int gi) {
@@ -402,11 +407,11 @@ OctXiUpdate xi_update(const cavlbm_octree* o, int gc, const int* gf, int nf, dou
                       double alpha_c, double nu_f, double alpha_f, OctXi* out) {
   OctXiUpdate u{};
   u.part_c = o->dyn_part[gc];
-  u.nb_c = nblocks_of(model_geom(o, gc));
+  u.nb_c = dyn_model_blocks(model_geom(o, gc), level_width(o, gc));
   u.n_c = o->lv[gc].plane;
   for (int i = 0; i < nf; ++i) {
     u.part_f[i] = o->dyn_part[gf[i]];
-    u.nb_f[i] = nblocks_of(model_geom(o, gf[i]));
+    u.nb_f[i] = dyn_model_blocks(model_geom(o, gf[i]), level_width(o, gf[i]));
     u.n_f[i] = o->lv[gf[i]].plane;
   }
   u.nf = nf;
@@ -1057,7 +1062,9 @@ int cavlbm_octree_set_dynamic_sgs(cavlbm_octree* o, int32_t enable, double smag_
     for (int gi = 0; gi < o->ngrids; ++gi) {
       const size_t n = (size_t)kDynPlanes * (size_t)o->lv[gi].plane;
       o->dyn[gi] = dalloc<double>(o, n);
-      const size_t np = 2 * (size_t)nblocks_of(model_geom(o, gi));
+      // entries of either form of the model (tiled, or the three passes)
+      const Geom mg = model_geom(o, gi);
+      const size_t np = 2 * (size_t)std::max(nblocks_of(mg), dyn_model_blocks(mg, 1));
       o->dyn_part[gi] = dalloc<double>(o, np);
       if (!o->dyn[gi] || !o->dyn_part[gi])
         return ofail(o, CAVLBM_E_HIP, "hipMalloc of the dynamic SGS planes");

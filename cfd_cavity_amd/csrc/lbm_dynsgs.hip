@@ -388,26 +388,18 @@ struct LdsPlane {
   }
 };
 
-template <int GEQ, int W, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, StepCtx ctx,
-                                                      Walls walls, const double* __restrict__ src,
-                                                      double* __restrict__ dst,
-                                                      const double* __restrict__ fy_in,
-                                                      double* __restrict__ fy_out,
-                                                      double* __restrict__ nut, DynSgsArgs a,
-                                                      DynCoef dc, int write_aux) {
+// Steps 2 and 3 of a tile, once u, v, T of every lattice node of R and the
+// grid-level strain of every node of T are in LDS and the workgroup has
+// synchronised: the test filter on F (box_filter, solver.rs:1204-1224) and
+// the similarity scales on Q; then the contractions and mixed forms on T,
+// held in registers until R and F are no longer read and left in LDS planes
+// 0..5 over T.  Ends synchronised.
+template <int W, int TX, int TY>
+__device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs& a, double* sh,
+                                               int ox, int oy) {
   using DT = DynStepTile<W, TX, TY>;
-  __shared__ double sh[DT::kLds];
-  int bx, by;
-  tile_of_block(tiles, bx, by);
-  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
-  const unsigned P32 = (unsigned)geo.plane;
-  const int ox = bx * TX, oy = by * TY;  // first output node of the tile
+  const int nx = geo.nx, ny = geo.ny;
   const int t = (int)threadIdx.x;
-  // every pulled node at least one node inside the walls: no boundary handling
-  const bool interior = ox - DT::kRlo >= 1 && ox + TX - 1 + DT::kRhi <= nx - 2 &&
-                        oy - DT::kRlo >= 1 && oy + TY - 1 + DT::kRhi <= ny - 2;
-
   const int rx0 = ox - DT::kRlo, ry0 = oy - DT::kRlo;
   const int fx0 = ox - DT::kFlo, fy0 = oy - DT::kFlo;
   const int qx0 = ox - DT::kQlo, qy0 = oy - DT::kQlo;
@@ -421,43 +413,7 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
   const LdsPlane qs{sh + DT::kOffQ + 0 * DT::kQ, qx0, qy0, DT::kQW};
   const LdsPlane ph{sh + DT::kOffQ + 1 * DT::kQ, qx0, qy0, DT::kQW};
 
-  // Pulled node (x, y): its fields on R and, inside T, its grid-level strain.
-  auto put_node = [&](int x, int y, const double (&fp)[9], double r, double u, double v,
-                      double tt) {
-    const int ir = (y - ry0) * DT::kRW + (x - rx0);
-    sh[0 * DT::kR + ir] = u;
-    sh[1 * DT::kR + ir] = v;
-    sh[2 * DT::kR + ir] = tt;
-    const int lx = x - tx0, ly = y - ty0;
-    if (lx >= 0 && lx < DT::kTW && ly >= 0 && ly < DT::kTH) {
-      const DynStrain s = dyn_grid_strain(fp, r, u, v, a.tau);
-      const int it = ly * DT::kTW + lx;
-      sh[DT::kOffS + 0 * DT::kT + it] = s.xx;
-      sh[DT::kOffS + 1 * DT::kT + it] = s.xy;
-      sh[DT::kOffS + 2 * DT::kT + it] = s.yy;
-      sh[DT::kOffS + 3 * DT::kT + it] = s.mag;
-    }
-  };
-
-  // 1. streaming of every node of R that lies in the lattice
-  auto pull = [&](int x, int y, double (&pf)[9], double (&pg)[9], double& r, double& u,
-                  double& v, double& tt) {
-    const double fyp = fy_in[y * pitch + x];
-    if (interior)
-      pull_node<ST_FP64, GEQ, true>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
-    else
-      pull_node<ST_FP64, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
-  };
-  for (int idx = t; idx < DT::kR; idx += DT::kNT) {
-    const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
-    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
-    double pf[9], pg[9], r, u, v, tt;
-    pull(x, y, pf, pg, r, u, v, tt);
-    put_node(x, y, pf, r, u, v, tt);
-  }
-  __syncthreads();
-
-  // 2. test filter on F (box_filter, :1204-1224) and similarity scales on Q
+  // 2. test filter on F and similarity scales on Q
   for (int idx = t; idx < DT::kF; idx += DT::kNT) {
     const int x = fx0 + idx % DT::kFW, y = fy0 + idx / DT::kFW;
     if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
@@ -529,6 +485,68 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
     sh[5 * DT::kT + idx] = keep[j].kap_mix_g;
   }
   __syncthreads();
+}
+
+template <int GEQ, int W, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, StepCtx ctx,
+                                                      Walls walls, const double* __restrict__ src,
+                                                      double* __restrict__ dst,
+                                                      const double* __restrict__ fy_in,
+                                                      double* __restrict__ fy_out,
+                                                      double* __restrict__ nut, DynSgsArgs a,
+                                                      DynCoef dc, int write_aux) {
+  using DT = DynStepTile<W, TX, TY>;
+  __shared__ double sh[DT::kLds];
+  int bx, by;
+  tile_of_block(tiles, bx, by);
+  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
+  const unsigned P32 = (unsigned)geo.plane;
+  const int ox = bx * TX, oy = by * TY;  // first output node of the tile
+  const int t = (int)threadIdx.x;
+  // every pulled node at least one node inside the walls: no boundary handling
+  const bool interior = ox - DT::kRlo >= 1 && ox + TX - 1 + DT::kRhi <= nx - 2 &&
+                        oy - DT::kRlo >= 1 && oy + TY - 1 + DT::kRhi <= ny - 2;
+
+  const int rx0 = ox - DT::kRlo, ry0 = oy - DT::kRlo;
+  const int tx0 = ox - DT::kTlo, ty0 = oy - DT::kTlo;
+
+  // Pulled node (x, y): its fields on R and, inside T, its grid-level strain.
+  auto put_node = [&](int x, int y, const double (&fp)[9], double r, double u, double v,
+                      double tt) {
+    const int ir = (y - ry0) * DT::kRW + (x - rx0);
+    sh[0 * DT::kR + ir] = u;
+    sh[1 * DT::kR + ir] = v;
+    sh[2 * DT::kR + ir] = tt;
+    const int lx = x - tx0, ly = y - ty0;
+    if (lx >= 0 && lx < DT::kTW && ly >= 0 && ly < DT::kTH) {
+      const DynStrain s = dyn_grid_strain(fp, r, u, v, a.tau);
+      const int it = ly * DT::kTW + lx;
+      sh[DT::kOffS + 0 * DT::kT + it] = s.xx;
+      sh[DT::kOffS + 1 * DT::kT + it] = s.xy;
+      sh[DT::kOffS + 2 * DT::kT + it] = s.yy;
+      sh[DT::kOffS + 3 * DT::kT + it] = s.mag;
+    }
+  };
+
+  // 1. streaming of every node of R that lies in the lattice
+  auto pull = [&](int x, int y, double (&pf)[9], double (&pg)[9], double& r, double& u,
+                  double& v, double& tt) {
+    const double fyp = fy_in[y * pitch + x];
+    if (interior)
+      pull_node<ST_FP64, GEQ, true>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+    else
+      pull_node<ST_FP64, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+  };
+  for (int idx = t; idx < DT::kR; idx += DT::kNT) {
+    const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    double pf[9], pg[9], r, u, v, tt;
+    pull(x, y, pf, pg, r, u, v, tt);
+    put_node(x, y, pf, r, u, v, tt);
+  }
+  __syncthreads();
+
+  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy);
 
   // 4. coefficient stage (box of the contractions) and the collision with
   //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185); the
@@ -566,6 +584,123 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
 }
 
 constexpr int kDynTX = 16, kDynTY = 16;
+
+// The model alone on one or two lattices (the octree's hot and cold blocks)
+// from their M-form fields: steps 1-3 of the one-launch step with the fields
+// read in place of the streaming, then the coefficient stage, which stores
+// C_nu, C_kappa and the floored nu_t / alpha_t, and (part) the workgroup's
+// sums of nu_t and alpha_t.  Tiles of item 1 follow those of item 0; the
+// per-node arithmetic is the three passes' (k_dyn_filter, k_dyn_terms,
+// k_dyn_coeffs), bit for bit.
+struct DynModelPair {
+  Geom geo[2];
+  DynSgsArgs a[2];
+  int nt0;  // tiles of item 0
+};
+
+template <int W, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_dyn_model(DynModelPair p) {
+  using DT = DynStepTile<W, TX, TY>;
+  __shared__ double sh[DT::kLds];
+  const int b = (int)blockIdx.x;
+  const int item = b < p.nt0 ? 0 : 1;
+  const int tile = item ? b - p.nt0 : b;
+  const Geom& geo = p.geo[item];
+  const DynSgsArgs& a = p.a[item];
+  const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
+  const long long P = geo.plane;
+  const int tbx = (nx + TX - 1) / TX;
+  const int ox = (tile % tbx) * TX, oy = (tile / tbx) * TY;
+  const int t = (int)threadIdx.x;
+  const int rx0 = ox - DT::kRlo, ry0 = oy - DT::kRlo;
+  const int tx0 = ox - DT::kTlo, ty0 = oy - DT::kTlo;
+
+  // 1. u, v, T of every lattice node of R; the grid-level strain on T
+  for (int idx = t; idx < DT::kR; idx += DT::kNT) {
+    const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    const int k = y * pitch + x;
+    const double u = a.ux[k], v = a.uy[k];
+    sh[0 * DT::kR + idx] = u;
+    sh[1 * DT::kR + idx] = v;
+    sh[2 * DT::kR + idx] = a.temp[k];
+    const int lx = x - tx0, ly = y - ty0;
+    if (lx >= 0 && lx < DT::kTW && ly >= 0 && ly < DT::kTH) {
+      double fp[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) fp[i] = a.mpops[i * P + k];
+      const DynStrain s = dyn_grid_strain(fp, a.rho[k], u, v, a.tau);
+      const int it = ly * DT::kTW + lx;
+      sh[DT::kOffS + 0 * DT::kT + it] = s.xx;
+      sh[DT::kOffS + 1 * DT::kT + it] = s.xy;
+      sh[DT::kOffS + 2 * DT::kT + it] = s.yy;
+      sh[DT::kOffS + 3 * DT::kT + it] = s.mag;
+    }
+  }
+  __syncthreads();
+
+  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy);
+
+  // 4. coefficient stage (box of the contractions) of the output nodes
+  const int ix = ox + t % TX, iy = oy + t / TX;
+  double nt = 0.0, at = 0.0;
+  if (ix < nx && iy < ny) {
+    const int k = iy * pitch + ix;
+    const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
+    double num[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      num[m] = box_mean(w, LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW});
+    const int it = (iy - ty0) * DT::kTW + (ix - tx0);
+    double cnu, ck;
+    dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
+                  a.nu_t_min, a.alpha_t_min, cnu, ck, nt, at);
+    a.cnu[k] = cnu;
+    a.ck[k] = ck;
+    a.nut_pre[k] = nt;
+    a.at_pre[k] = at;
+  }
+  if (!a.part) return;  // uniform over the workgroup
+  constexpr int kWaves = TX * TY / 64;
+  double* red = sh;  // the planes are no longer read after the barrier
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nt += __shfl_xor(nt, off, 64);
+    at += __shfl_xor(at, off, 64);
+  }
+  __syncthreads();
+  const int wv = t / 64;
+  if (t % 64 == 0) {
+    red[wv] = nt;
+    red[kWaves + wv] = at;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double sn = 0.0, sa = 0.0;
+    for (int i = 0; i < kWaves; ++i) {
+      sn += red[i];
+      sa += red[kWaves + i];
+    }
+    const int ntiles = tbx * ((ny + TY - 1) / TY);
+    a.part[tile] = sn;
+    a.part[ntiles + tile] = sa;
+  }
+}
+
+template <int W>
+hipError_t go_dyn_model(const Geom* geo, const DynSgsArgs* a, int count, hipStream_t st) {
+  DynModelPair p{};
+  int nt = 0;
+  for (int i = 0; i < count; ++i) {
+    p.geo[i] = geo[i];
+    p.a[i] = a[i];
+    nt += dyn_model_blocks(geo[i], W);
+  }
+  p.nt0 = dyn_model_blocks(geo[0], W);
+  hipLaunchKernelGGL((k_dyn_model<W, kDynTX, kDynTY>), dim3(nt), dim3(kDynTX * kDynTY), 0, st,
+                     p);
+  return hipGetLastError();
+}
 
 template <int GEQ, int W>
 hipError_t go_dyn_step(const Geom& geo, const StepCtx& ctx, const Walls& walls, const void* src,
@@ -612,6 +747,23 @@ hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, b
   hipLaunchKernelGGL(k_dyn_terms, grid, block, 0, st, p);
   if (coeffs) hipLaunchKernelGGL(k_dyn_coeffs, grid, block, 0, st, p);
   return hipGetLastError();
+}
+
+int dyn_model_blocks(const Geom& geo, int width) {
+  if (!dyn_step_fits(width)) return nblocks_of(geo);
+  return ((geo.nx + kDynTX - 1) / kDynTX) * ((geo.ny + kDynTY - 1) / kDynTY);
+}
+
+hipError_t launch_dynamic_model(const Geom* geo, const DynSgsArgs* a, int count, hipStream_t st) {
+  if (count < 1 || count > 2) return hipErrorInvalidValue;
+  const int width = a[0].width;
+  if (!dyn_step_fits(width) || (count == 2 && a[1].width != width))
+    return launch_dynamic_sgs(geo, a, count, true, st);
+  switch (width) {
+    case 1: return go_dyn_model<1>(geo, a, count, st);
+    case 2: return go_dyn_model<2>(geo, a, count, st);
+    default: return go_dyn_model<3>(geo, a, count, st);
+  }
 }
 
 hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,

@@ -120,6 +120,13 @@ hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, b
 // write_aux the collision's nu_t and the coefficient planes are stored.
 // Test-filter widths 1..3.
 inline bool dyn_step_fits(int width) { return width >= 1 && width <= 3; }
+// The whole model (filter, terms and coefficient stage, writing C_nu, C_kappa,
+// nu_t_pre, alpha_t_pre and, with part, the per-workgroup sums) on count (1
+// or 2) lattices from their M-form: one tiled launch for widths 1..3, the
+// three passes otherwise.  dyn_model_blocks: the workgroups, i.e. the entries
+// of part per sum, the model writes for a lattice of that geometry.
+hipError_t launch_dynamic_model(const Geom* geo, const DynSgsArgs* a, int count, hipStream_t st);
+int dyn_model_blocks(const Geom& geo, int width);
 hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,
                                 const void* src, void* dst, const void* fy_in, void* fy_out,
                                 void* nut, const DynSgsArgs& a, int write_aux, hipStream_t st);
