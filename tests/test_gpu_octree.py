@@ -282,6 +282,10 @@ def assert_model_close(g, o, tol_pre, tol_stats, what, tol_xi=1e-14):
     (dict(levels=2), (0.5, -0.1, 2.0), 1e-10),
     (dict(levels=3), (0.5, -0.1, 2.0), 1e-10),
     (dict(levels=3, odd_exact=True), (0.3, -0.1, 3.0), 1e-10),
+    # test-filter width 1 (tiled model), and width 5: the three passes, whose
+    # coefficient pass writes the workgroup sums the factor update reads
+    (dict(levels=2), (0.5, -0.1, 1.0), 1e-10),
+    (dict(levels=3), (0.5, -0.1, 5.0), 1e-10),
     (dict(levels=2, nu_start_mult=3.0), (0.5, -0.1, 2.0), 1e-10),  # direct launches, ramped nu
     # the tau floor binds; at nu = 0.0005 the differences grow to ~2e-10 of
     # the strips' small velocities in 20 steps
