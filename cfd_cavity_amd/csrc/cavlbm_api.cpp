@@ -1758,27 +1758,51 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
                    WindowWork& w, std::vector<double*>* ckpt, double* j_out) {
   const size_t P = (size_t)s->geo.plane;
   double* X[2] = {w.alloc<double>(18 * P), w.alloc<double>(18 * P)};
-  double* fy = w.alloc<double>(P);
+  // fy_m; the N-step launches read one plane and write the other
+  double* fy[2] = {w.alloc<double>(P), w.alloc<double>(P)};
   double* nut = w.alloc<double>(P);
   double* acc = w.zeroed<double>(s->p.ny);
-  if (!X[0] || !X[1] || !fy || !nut || !acc) return fail(s, CAVLBM_E_HIP, "window allocation failed");
+  if (!X[0] || !X[1] || !fy[0] || !fy[1] || !nut || !acc)
+    return fail(s, CAVLBM_E_HIP, "window allocation failed");
   const uint64_t t0 = s->time_step;
-  int cur = 0;
-  for (uint64_t m = 0; m < K; ++m) {
-    // X_m = Collide(S_m): S_0 from the M-form, then the fused step from X_{m-1}
-    // (from m = 1 the step also accumulates Nu(S_m), pulled from X_{m-1})
-    const double* src = m == 0 ? nullptr : X[cur];
-    double* dst = m == 0 ? X[cur] : X[cur ^ 1];
-    HIP_TRY(s, window_step(s, src, dst, fy, nut, theta, window_ctx(s, t0 + m, cs), nullptr,
-                           m == 0 ? nullptr : acc));
-    if (m > 0) cur ^= 1;
-    if (ckpt && m % C == 0) {
+  const Walls walls{theta};
+  int cur = 0, fc = 0;
+  for (uint64_t m = 0; m < K;) {
+    // X_m = Collide(S_m): S_0 from the M-form, then steps from X_{m-1}, which
+    // also accumulate Nu(S_m) pulled from X_{m-1}: n of them per launch (the
+    // N-step window kernel), ending on every checkpoint step
+    int n = 1;
+    if (m > 0) {
+      uint64_t room = K - m;
+      if (ckpt) room = std::min<uint64_t>(room, (m + C - 1) / C * C - m + 1);
+      for (int q = (int)std::min<uint64_t>((uint64_t)s->tblock, room); q >= 2; --q)
+        if (s->fitsn[q]) {
+          n = q;
+          break;
+        }
+    }
+    if (m == 0) {
+      HIP_TRY(s, window_step(s, nullptr, X[cur], fy[fc], nut, theta, window_ctx(s, t0, cs)));
+    } else if (n == 1) {
+      HIP_TRY(s, window_step(s, X[cur], X[cur ^ 1], fy[fc], nut, theta,
+                             window_ctx(s, t0 + m, cs), nullptr, acc));
+      cur ^= 1;
+    } else {
+      StepCtxN cn{};
+      for (int j = 0; j < n; ++j) cn.c[j] = window_ctx(s, t0 + m + (uint64_t)j, cs);
+      HIP_TRY(s, launch_stepn_window_fp64(s->geq, n, s->geon[n], cn, walls, X[cur], X[cur ^ 1],
+                                          fy[fc], fy[fc ^ 1], acc, s->st));
+      cur ^= 1;
+      fc ^= 1;
+    }
+    m += (uint64_t)n;
+    if (ckpt && (m - 1) % C == 0) {  // X_{m-1}, the last state of this launch
       double* slot = w.alloc<double>(19 * P);
       if (!slot) return fail(s, CAVLBM_E_HIP, "checkpoint allocation failed");
       HIP_TRY(s, hipMemcpyAsync(slot, X[cur], 18 * P * sizeof(double), hipMemcpyDeviceToDevice,
                                 s->st));
-      HIP_TRY(s, hipMemcpyAsync(slot + 18 * P, fy, P * sizeof(double), hipMemcpyDeviceToDevice,
-                                s->st));
+      HIP_TRY(s, hipMemcpyAsync(slot + 18 * P, fy[fc], P * sizeof(double),
+                                hipMemcpyDeviceToDevice, s->st));
       ckpt->push_back(slot);
     }
   }

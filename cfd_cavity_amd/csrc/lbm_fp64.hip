@@ -95,4 +95,32 @@ hipError_t launch_stepn_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& 
   return hipErrorInvalidValue;
 }
 
+template <int GEQ, int N>
+static hipError_t gow(const Geom& geo_n, const StepCtxN& cn, const Walls& walls, const void* src,
+                      void* dst, const void* fy_in, void* fy_out, double* nu_acc,
+                      hipStream_t st) {
+  if (!stepn_fits<ST_FP64, N>(geo_n.nx, geo_n.ny)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_stepn_window<GEQ, N>), dim3(geo_n.nbx * geo_n.nby),
+                     dim3(TileN<ST_FP64, N>::kNT), 0, st, geo_n, cn, walls,
+                     static_cast<const double*>(src), static_cast<double*>(dst),
+                     static_cast<const double*>(fy_in), static_cast<double*>(fy_out), nu_acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_stepn_window_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+                                    const Walls& walls, const void* src, void* dst,
+                                    const void* fy_in, void* fy_out, double* nu_acc,
+                                    hipStream_t st) {
+#define CAVLBM_CASE(G, N) \
+  if (geq == G && n == N) return gow<G, N>(geo_n, cn, walls, src, dst, fy_in, fy_out, nu_acc, st);
+  CAVLBM_CASE(2, 2)
+  CAVLBM_CASE(2, 3)
+  CAVLBM_CASE(2, 4)
+  CAVLBM_CASE(1, 2)
+  CAVLBM_CASE(1, 3)
+  CAVLBM_CASE(1, 4)
+#undef CAVLBM_CASE
+  return hipErrorInvalidValue;
+}
+
 }  // namespace cavlbm

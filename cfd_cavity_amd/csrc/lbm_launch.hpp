@@ -26,6 +26,13 @@ hipError_t launch_step_lowp(int storage, int geq, int mode, const Geom& geo,
 hipError_t launch_stepn_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st);
+// The objective window's N steps (FP64): launch_stepn_fp64's steps, each of
+// which also adds the hot-wall Nusselt term of the state it pulls to
+// nu_acc[iy] (as MODE_WINDOW does for one step).
+hipError_t launch_stepn_window_fp64(int geq, int n, const Geom& geo_n, const StepCtxN& cn,
+                                    const Walls& walls, const void* src, void* dst,
+                                    const void* fy_in, void* fy_out, double* nu_acc,
+                                    hipStream_t st);
 hipError_t launch_stepn_lowp(int storage, int geq, int n, const Geom& geo_n, const StepCtxN& cn,
                              const Walls& walls, const void* src, void* dst, const void* fy_in,
                              void* fy_out, void* nut, int write_aux, hipStream_t st);

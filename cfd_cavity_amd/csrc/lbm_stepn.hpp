@@ -90,7 +90,13 @@ __host__ __device__ inline bool stepn_fits(int nx, int ny) {
 // One tile's N steps.  INTERIOR: every node of the first sub-step's region
 // lies in [1, nx-2] x [1, ny-2], so every pull is inside the lattice and no
 // boundary condition applies (the same arithmetic without the checks).
-template <int S, int GEQ, int N, bool INTERIOR>
+// NU (FP64, the objective window): every sub-step also adds the hot-wall
+// Nusselt term |T(1,iy) - T(2,iy)| of the state it pulls (post.rs:68-78) to
+// nu_acc[iy], for the rows of this tile's output region only, so each row of
+// each step is added once and in step order: T(1, iy) is the node's pulled
+// temperature, T(2, iy) the sum of the g that node (2, iy) pulls, from +0 in
+// direction order (the isothermal value when column 2 is the cold wall).
+template <int S, int GEQ, int N, bool INTERIOR, bool NU = false>
 __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
                                            const Walls& walls,
                                            const typename StorageT<S>::T* src,
@@ -98,15 +104,21 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
                                            const typename StorageT<S>::R* fy_in,
                                            typename StorageT<S>::R* fy_out,
                                            typename StorageT<S>::R* nut, int write_aux,
-                                           typename StorageT<S>::R* sh, int bx, int by) {
+                                           typename StorageT<S>::R* sh, int bx, int by,
+                                           double* nu_acc = nullptr) {
   using ST = StorageT<S>;
   using R = typename ST::R;
   constexpr int kTX = TileN<S, N>::kTX, kTY = TileN<S, N>::kTY;
   constexpr int kH = N - 1;                         // halo of sub-step 1
   constexpr int kE1X = kTX + 2 * kH;
   constexpr int kE1 = kE1X * (kTY + 2 * kH);        // sub-step 1 nodes
+  static_assert(!NU || (S == ST_FP64 && !INTERIOR), "the window form is FP64, wall tiles");
   const int nx = geo.nx, ny = geo.ny;
   const int t = (int)threadIdx.x;
+  // node (1, iy) of this tile's output rows: it adds the Nusselt term
+  auto nu_node = [&](int ix, int iy) {
+    return NU && ix == 1 && bx == 0 && iy >= by * kTY && iy < (by + 1) * kTY;
+  };
 
   // ---- sub-step 1: one thread per node of the grown tile, pulls from HBM
   R f[9], g[9], fyc = R(0.0), nutc = R(0.0), tc = R(0.0);
@@ -118,6 +130,23 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       R rho, ux, uy;
       pull_node<S, GEQ, INTERIOR>(geo, walls, src, fy_in[iy * geo.pitch + ix], ix, iy, f, g,
                                   rho, ux, uy, tc);
+      if constexpr (NU) {
+        if (nu_node(ix, iy)) {
+          R t2 = R(0.0);
+          if (nx == 3) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) t2 += R(kW[i]) * R(kTCold);
+          } else {
+            const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+            const unsigned P32 = (unsigned)geo.plane;
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+              t2 += R(src[(unsigned)(9 + i) * P32 +
+                          (unsigned)((gy - kCy[i]) * geo.pitch + (2 - kCx[i]))]);
+          }
+          nu_acc[iy] += fabs((double)(tc - t2));
+        }
+      }
       collide_node<R, GEQ>(f, g, rho, ux, uy, tc, cn.c[0], R(cn.c[0].cs), fyc, nutc);
       if (N > 1) storage_round<S>(f, g, tc);
     }
@@ -164,6 +193,23 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       const bool wall = !INTERIOR && (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1);
       R rho, ux, uy;
       node_moments<R>(f2, g2, fyp, wall, rho, ux, uy, tc);
+      if constexpr (NU) {
+        if (nu_node(ix, iy)) {
+          // node (2, iy) pulls from the previous region: its sources lie
+          // within one row of this tile's output rows (stepn_fits)
+          R t2 = R(0.0);
+          if (nx == 3) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) t2 += R(kW[i]) * R(kTCold);
+          } else {
+            const int gy = iy == 0 ? 1 : (iy == ny - 1 ? ny - 2 : iy);
+            const int e2 = (gy - py0) * px + (2 - px0);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) t2 += sh[(10 + i) * kE1 + e2 - kCy[i] * px - kCx[i]];
+          }
+          nu_acc[iy] += fabs((double)(tc - t2));
+        }
+      }
       collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, cn.c[s - 1], R(cn.c[s - 1].cs), fyc, nutc);
       if (s < N) storage_round<S>(f2, g2, tc);
 #pragma unroll
@@ -221,6 +267,28 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn(
   else
     stepn_tile<S, GEQ, N, false>(geo, cn, walls, src, dst, fy_in, fy_out, nut, write_aux, sh, bx,
                                  by);
+}
+
+// N steps of the objective window (FP64): the N-step kernel whose sub-steps
+// also add the hot-wall Nusselt terms of the states they pull to nu_acc.
+template <int GEQ, int N>
+__global__ __launch_bounds__((TileN<ST_FP64, N>::kNT)) void k_stepn_window(
+    Geom geo, StepCtxN cn, Walls walls, const double* src, double* dst, const double* fy_in,
+    double* fy_out, double* nu_acc) {
+  constexpr int kTX = TileN<ST_FP64, N>::kTX, kTY = TileN<ST_FP64, N>::kTY;
+  constexpr int kH = N - 1;
+  constexpr int kE1 = (kTX + 2 * kH) * (kTY + 2 * kH);
+  __shared__ double sh[19 * kE1];
+  int bx, by;
+  tile_of_block(geo, bx, by);
+  const bool interior = bx * kTX - kH >= 1 && (bx + 1) * kTX + kH - 1 <= geo.nx - 2 &&
+                        by * kTY - kH >= 1 && (by + 1) * kTY + kH - 1 <= geo.ny - 2;
+  if (interior)
+    stepn_tile<ST_FP64, GEQ, N, true>(geo, cn, walls, src, dst, fy_in, fy_out, nullptr, 0, sh,
+                                      bx, by);
+  else
+    stepn_tile<ST_FP64, GEQ, N, false, true>(geo, cn, walls, src, dst, fy_in, fy_out, nullptr, 0,
+                                             sh, bx, by, nu_acc);
 }
 
 }  // namespace cavlbm

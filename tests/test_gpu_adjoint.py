@@ -117,3 +117,19 @@ def test_window_buffers_reused_across_calls():
         else:
             assert got == want, key
     assert fresh["tan_zero"][1] == 0.0
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 40), (17, 35), (40, 33), (20, 49)])
+def test_window_blocked_steps_equal_single_steps(nx, ny):
+    """The forward window advances up to four steps per launch, each adding the
+    Nusselt rows of the state it pulls (blocks end on checkpoint steps); J and
+    the adjoint's gradients equal those of one step per launch bitwise.  The
+    heights cover every fit of the tilings: 40 and 35 take four steps, 33 two
+    and 49 none."""
+    s = spun_up(nx, ny, 1e5, 150)
+    theta = 0.5 + 0.01 * np.random.default_rng(7).uniform(-1, 1, ny)
+    blocked = (nu_window(s, theta, 0.1, 37), adjoint_nu_window(s, theta, 0.1, 37, 6))
+    s.time_blocking = 1
+    single = (nu_window(s, theta, 0.1, 37), adjoint_nu_window(s, theta, 0.1, 37, 6))
+    assert blocked[0] == single[0] == blocked[1][0] == single[1][0]
+    assert blocked[1][2] == single[1][2] and bits_equal(blocked[1][1], single[1][1])
