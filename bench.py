@@ -189,7 +189,8 @@ def dynamic_form(steps: int, warmup: int, cpu_s: float):
     """The dynamic mixed-scale SGS path (solver.rs:562-828) on the same
     256x1024 workload, FP64: one launch per step (streaming, the model and
     the collision per tile), against the oracle's dynamic step on a bounded
-    CPU sample."""
+    CPU sample; and the FP32-storage step in its fused (3) and four-launch
+    forms."""
     from oracle import pyoracle
 
     s = make_solver("fp64")
@@ -199,6 +200,16 @@ def dynamic_form(steps: int, warmup: int, cpu_s: float):
     ms = s.time_run(steps)
     out = {"steps": steps, "ms_per_step": ms / steps, "mlups": nodes * steps / ms / 1e3,
            "launches_per_step": s.dynamic_launches, "cs_dyn_mean": s.dynamic_stats()[0]}
+    # FP32 storage: the model in one tiled launch between the post-stream
+    # fields and the collision (3 launches), and the four-launch form
+    for fuse in (True, False):
+        f = make_solver("fp32")
+        f.set_dynamic_fusion(fuse)
+        f.set_dynamic_sgs(True)
+        f.run(warmup)
+        ms = f.time_run(steps)
+        out[f"fp32_{f.dynamic_launches}_launches_ms_per_step"] = ms / steps
+        del f
     if cpu_s > 0:
         threads = min(os.cpu_count() or 1, 16)
         w = WORKLOAD
