@@ -32,7 +32,6 @@ UNITS = {
     "lbm_aux.hip": ["-ffp-contract=off"],
     "lbm_adjoint.hip": ["-ffp-contract=off"],
     "lbm_tangent.hip": ["-ffp-contract=off"],
-    "lbm_dynsgs.hip": ["-ffp-contract=off"],
     "lbm_octree.hip": ["-ffp-contract=off"],
     "cavlbm_octree.cpp": ["-ffp-contract=off", "-x", "hip"],
     "cavlbm_api.cpp": ["-ffp-contract=off", "-x", "hip"],
@@ -44,6 +43,11 @@ UNITS = {
 VARIANTS = {
     "lbm_batch_fp64": ("lbm_batch.hip", ["-ffp-contract=off", "-DCAVLBM_BATCH_PRECISION=64"]),
     "lbm_batch_lowp": ("lbm_batch.hip", ["-ffp-contract=on", "-DCAVLBM_BATCH_PRECISION=32"]),
+    # the dynamic model (never contracted, by pragma) with the FP64 one-launch
+    # step, and the FP32-arithmetic one-launch steps whose pull and collision
+    # round like the FP32 single-step kernels
+    "lbm_dynsgs": ("lbm_dynsgs.hip", ["-ffp-contract=off", "-DCAVLBM_DYN_OBJECT=64"]),
+    "lbm_dynsgs_lowp": ("lbm_dynsgs.hip", ["-ffp-contract=on", "-DCAVLBM_DYN_OBJECT=32"]),
 }
 
 

@@ -81,7 +81,7 @@ struct cavlbm_solver {
   bool dyn_on = false, dyn_pre = false;
   double dyn_alpha = 0.5, dyn_min = -0.1, dyn_eta = 2.0;
   double* dyn = nullptr;  // kDynPlanes FP64 planes, allocated on first enable
-  bool dyn_fuse = true;   // fused dynamic steps (1 or 3 launches) where they apply
+  bool dyn_fuse = true;   // one launch per dynamic step where it applies
   // Window objective / adjoint / tangent buffers (states, checkpoints, the
   // recomputed segment, adjoint planes), kept between calls until destroy.
   std::vector<WinSlot> win;
@@ -272,42 +272,33 @@ DynSgsArgs dyn_args(const cavlbm_solver* s, const StepCtx& ctx) {
   return a;
 }
 
-// The launches of one dynamic step from the C-form.  FP64 storage with a
-// test filter of 1..3 nodes: one launch (k_dyn_step), which reads fy and
-// writes fy_alt; the caller swaps them (fy_swap).  Otherwise the post-stream
-// f planes and fields the model reads (MODE_MATERIALIZE_F, skipped when the
-// M-form is current), then
-//   - FP32 / FP16-offset storage, filter of 1..3 nodes (dyn_launches 3): the
-//     model in one tiled launch (k_dyn_model, which stores nu_t / alpha_t)
-//     and the collision with those values (MODE_FUSED_DYN without
-//     contraction planes);
-//   - otherwise (4): the model's filter and terms, and the collision, which
-//     pulls the C-form again, runs the coefficient stage per node and stores
-//     its planes with the auxiliary outputs (MODE_FUSED_DYN).
+// The launches of one dynamic step from the C-form.  A test filter of 1..3
+// nodes (dyn_launches 1): one launch (k_dyn_step for the storage), which
+// reads fy and writes fy_alt; the caller swaps them (fy_swap).  Otherwise
+// (4): the post-stream f planes and fields the model reads
+// (MODE_MATERIALIZE_F, skipped when the M-form is current), the model's
+// filter and terms, and the collision, which pulls the C-form again, runs the
+// coefficient stage per node and stores its planes with the auxiliary
+// outputs (MODE_FUSED_DYN).
 int dyn_launches(const cavlbm_solver* s, int width) {
-  if (!s->dyn_fuse || !dyn_step_fits(width)) return 4;
-  return s->storage == ST_FP64 ? 1 : 3;
+  return s->dyn_fuse && dyn_step_fits(width) ? 1 : 4;
 }
 
 hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgsArgs& args,
                               int write_aux, bool& fy_swap) {
   fy_swap = false;
-  const int nl = dyn_launches(s, args.width);
-  if (nl == 1) {
+  if (dyn_launches(s, args.width) == 1) {
     fy_swap = true;
-    return launch_dyn_step_fp64(s->geq, s->geo, ctx, Walls{s->theta}, s->cpop[s->cur],
-                                s->cpop[s->cur ^ 1], s->fy, s->fy_alt, s->nut, args, write_aux,
-                                s->st);
+    return launch_dyn_step(s->storage, s->geq, s->geo, ctx, Walls{s->theta}, s->cpop[s->cur],
+                           s->cpop[s->cur ^ 1], s->fy, s->fy_alt, s->nut, args, write_aux,
+                           s->st);
   }
   hipError_t e = hipSuccess;
   if (!s->m_valid) e = launch(s, MODE_MATERIALIZE_F, ctx, 0);
-  if (e == hipSuccess)
-    e = nl == 3 ? launch_dynamic_model(&s->geo, &args, 1, s->st)
-                : launch_dynamic_sgs(&s->geo, &args, 1, false, s->st);
+  if (e == hipSuccess) e = launch_dynamic_sgs(&s->geo, &args, 1, false, s->st);
   if (e != hipSuccess) return e;
   StepBufs b = bufs(s);
   b.dc = dyn_coef_of(args);
-  if (nl == 3) b.dc.terms = nullptr;  // nu_t / alpha_t from the model's launch
   return launch_b(s, MODE_FUSED_DYN, ctx, b, write_aux);
 }
 

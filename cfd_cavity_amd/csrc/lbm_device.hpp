@@ -669,16 +669,9 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   } else if (MODE == MODE_FUSED_DYN) {
     // the coefficient stage for this node; its planes are stored with the
     // auxiliary outputs (the values the solver keeps for its getters and for
-    // later steps without the model).  Without contraction planes (dc.terms
-    // null) the model's tiled launch has run that stage: its nu_t_pre /
-    // alpha_t_pre are this node's values.
+    // later steps without the model)
     double nt, at;
-    if (b.dc.terms) {
-      dyn_coef_node(geo, b.dc, ix, iy, write_aux != 0, nt, at);
-    } else {
-      nt = b.nut_pre[k];
-      at = b.at_pre[k];
-    }
+    dyn_coef_node(geo, b.dc, ix, iy, write_aux != 0, nt, at);
     collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new, R(nt),
                                R(at));
   } else

@@ -2,10 +2,11 @@
 // (reference: src/uniform/solver.rs:562-828 compute_dynamic_sgs, helpers
 // box_filter :1204-1224, ddx/ddy :1229-1242, lilly_coefficient :1251-1261).
 //
-// FP64, built with -ffp-contract=off so that every sum, product, division and
-// square root is the IEEE result in the reference's operand order.  The
-// model's per-node arithmetic is written once (dyn_similarity,
-// dyn_grid_strain, dyn_terms_node, dyn_coef_vals) and reached from two forms:
+// The model is FP64 with no contraction (the pragma below, whatever the
+// object's flag) so that every sum, product, division and square root is the
+// IEEE result in the reference's operand order.  The model's per-node
+// arithmetic is written once (dyn_similarity, dyn_grid_strain,
+// dyn_terms_node, dyn_coef_vals) and reached from two forms:
 //
 // Three passes over the M-form state (post-stream populations and moments),
 // one thread per node, planes in HBM:
@@ -22,7 +23,7 @@
 //                  floored (dyn_coef_node; a step from the C-form runs it
 //                  inside the collision instead, MODE_FUSED_DYN).
 //
-// One launch per dynamic step from the C-form (k_dyn_step, FP64 storage,
+// One launch per dynamic step from the C-form (k_dyn_step, any storage,
 // test-filter widths 1..3): each workgroup pulls the populations of its tile
 // grown by the model's stencil (the filter windows, the centred differences
 // and the box of the contractions), runs the filter, the similarity scales,
@@ -36,7 +37,21 @@
 // library's pow, which is not correctly rounded, so this path agrees with a
 // CPU evaluation to a few ulp per node rather than bitwise.  For alpha = 0.5
 // (the default) the factors are correctly rounded square roots.
+//
+// Two objects are built from this file: CAVLBM_DYN_OBJECT 64 (no
+// contraction) holds everything but the low-precision one-launch steps;
+// CAVLBM_DYN_OBJECT 32 (-ffp-contract=on) holds k_dyn_step for FP32 and
+// FP16-offset storage, whose pull, collision and coefficient values (the
+// functions of lbm_device.hpp, included before the pragma) then round as in
+// the single-step kernels of those storages, while the model defined here
+// stays uncontracted.
 #include "lbm_launch.hpp"
+
+#ifndef CAVLBM_DYN_OBJECT
+#define CAVLBM_DYN_OBJECT 64
+#endif
+
+#pragma clang fp contract(off)
 
 namespace cavlbm {
 
@@ -487,14 +502,18 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
   __syncthreads();
 }
 
-template <int GEQ, int W, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, StepCtx ctx,
-                                                      Walls walls, const double* __restrict__ src,
-                                                      double* __restrict__ dst,
-                                                      const double* __restrict__ fy_in,
-                                                      double* __restrict__ fy_out,
-                                                      double* __restrict__ nut, DynSgsArgs a,
-                                                      DynCoef dc, int write_aux) {
+// S: the storage of the populations (the pull, collision and stores run in
+// its arithmetic type R); the model is FP64 in every storage, on the pulled
+// values widened to FP64 exactly as MODE_MATERIALIZE_F stores them.
+template <int S, int GEQ, int W, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_dyn_step(
+    Geom geo, Geom tiles, StepCtx ctx, Walls walls,
+    const typename StorageT<S>::T* __restrict__ src, typename StorageT<S>::T* __restrict__ dst,
+    const typename StorageT<S>::R* __restrict__ fy_in,
+    typename StorageT<S>::R* __restrict__ fy_out, typename StorageT<S>::R* __restrict__ nut,
+    DynSgsArgs a, DynCoef dc, int write_aux) {
+  using ST = StorageT<S>;
+  using R = typename ST::R;
   using DT = DynStepTile<W, TX, TY>;
   __shared__ double sh[DT::kLds];
   int bx, by;
@@ -529,20 +548,22 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
   };
 
   // 1. streaming of every node of R that lies in the lattice
-  auto pull = [&](int x, int y, double (&pf)[9], double (&pg)[9], double& r, double& u,
-                  double& v, double& tt) {
-    const double fyp = fy_in[y * pitch + x];
+  auto pull = [&](int x, int y, R (&pf)[9], R (&pg)[9], R& r, R& u, R& v, R& tt) {
+    const R fyp = fy_in[y * pitch + x];
     if (interior)
-      pull_node<ST_FP64, GEQ, true>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+      pull_node<S, GEQ, true>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
     else
-      pull_node<ST_FP64, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
+      pull_node<S, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
   };
   for (int idx = t; idx < DT::kR; idx += DT::kNT) {
     const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
     if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
-    double pf[9], pg[9], r, u, v, tt;
+    R pf[9], pg[9], r, u, v, tt;
     pull(x, y, pf, pg, r, u, v, tt);
-    put_node(x, y, pf, r, u, v, tt);
+    double pd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pd[i] = (double)pf[i];
+    put_node(x, y, pd, (double)r, (double)u, (double)v, (double)tt);
   }
   __syncthreads();
 
@@ -555,7 +576,7 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
   const int ix = ox + t % TX, iy = oy + t / TX;
   if (ix >= nx || iy >= ny) return;
   const int k = iy * pitch + ix;
-  double f[9], g[9], rho, ux, uy, temp;
+  R f[9], g[9], rho, ux, uy, temp;
   pull(ix, iy, f, g, rho, ux, uy, temp);
   const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
   double num[4];
@@ -566,8 +587,9 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
   double cnu, ck, nt, at;
   dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
                 dc.nu_t_min, dc.alpha_t_min, cnu, ck, nt, at);
-  double fy_new, nut_new;
-  collide_node<double, GEQ, true>(f, g, rho, ux, uy, temp, ctx, ctx.cs, fy_new, nut_new, nt, at);
+  R fy_new, nut_new;
+  collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new, R(nt),
+                             R(at));
   fy_out[k] = fy_new;
   if (write_aux) {
     nut[k] = nut_new;
@@ -578,9 +600,10 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(Geom geo, Geom tiles, Step
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    dst[(unsigned)i * P32 + (unsigned)k] = f[i];
-    dst[(unsigned)(9 + i) * P32 + (unsigned)k] = g[i];
+    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
+    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp);
   }
+  if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)temp;
 }
 
 constexpr int kDynTX = 16, kDynTY = 16;
@@ -702,35 +725,49 @@ hipError_t go_dyn_model(const Geom* geo, const DynSgsArgs* a, int count, hipStre
   return hipGetLastError();
 }
 
-template <int GEQ, int W>
+template <int S, int GEQ, int W>
 hipError_t go_dyn_step(const Geom& geo, const StepCtx& ctx, const Walls& walls, const void* src,
                        void* dst, const void* fy_in, void* fy_out, void* nut,
                        const DynSgsArgs& a, int write_aux, hipStream_t st) {
+  using ST = StorageT<S>;
+  using T = typename ST::T;
+  using R = typename ST::R;
   Geom tiles = geo;
   tiles.nbx = (geo.nx + kDynTX - 1) / kDynTX;
   tiles.nby = (geo.ny + kDynTY - 1) / kDynTY;
-  hipLaunchKernelGGL((k_dyn_step<GEQ, W, kDynTX, kDynTY>), dim3(tiles.nbx * tiles.nby),
+  hipLaunchKernelGGL((k_dyn_step<S, GEQ, W, kDynTX, kDynTY>), dim3(tiles.nbx * tiles.nby),
                      dim3(kDynTX * kDynTY), 0, st, geo, tiles, ctx, walls,
-                     static_cast<const double*>(src), static_cast<double*>(dst),
-                     static_cast<const double*>(fy_in), static_cast<double*>(fy_out),
-                     static_cast<double*>(nut), a, dyn_coef_of(a), write_aux);
+                     static_cast<const T*>(src), static_cast<T*>(dst),
+                     static_cast<const R*>(fy_in), static_cast<R*>(fy_out), static_cast<R*>(nut),
+                     a, dyn_coef_of(a), write_aux);
   return hipGetLastError();
 }
 
-template <int GEQ>
+template <int S, int GEQ>
 hipError_t by_width(int width, const Geom& geo, const StepCtx& ctx, const Walls& walls,
                     const void* src, void* dst, const void* fy_in, void* fy_out, void* nut,
                     const DynSgsArgs& a, int write_aux, hipStream_t st) {
   switch (width) {
-    case 1: return go_dyn_step<GEQ, 1>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
-    case 2: return go_dyn_step<GEQ, 2>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
-    case 3: return go_dyn_step<GEQ, 3>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    case 1: return go_dyn_step<S, GEQ, 1>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    case 2: return go_dyn_step<S, GEQ, 2>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
+    case 3: return go_dyn_step<S, GEQ, 3>(geo, ctx, walls, src, dst, fy_in, fy_out, nut, a, write_aux, st);
     default: return hipErrorInvalidValue;
   }
 }
 
+template <int S>
+hipError_t by_geq(int geq, int width, const Geom& geo, const StepCtx& ctx, const Walls& walls,
+                  const void* src, void* dst, const void* fy_in, void* fy_out, void* nut,
+                  const DynSgsArgs& a, int write_aux, hipStream_t st) {
+  return geq == 1 ? by_width<S, 1>(width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                                   write_aux, st)
+                  : by_width<S, 2>(width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                                   write_aux, st);
+}
+
 }  // namespace
 
+#if CAVLBM_DYN_OBJECT == 64
 hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, bool coeffs,
                               hipStream_t st) {
   if (count < 1 || count > 2) return hipErrorInvalidValue;
@@ -766,14 +803,37 @@ hipError_t launch_dynamic_model(const Geom* geo, const DynSgsArgs* a, int count,
   }
 }
 
-hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,
-                                const void* src, void* dst, const void* fy_in, void* fy_out,
-                                void* nut, const DynSgsArgs& a, int write_aux, hipStream_t st) {
+hipError_t launch_dyn_step(int storage, int geq, const Geom& geo, const StepCtx& ctx,
+                           const Walls& walls, const void* src, void* dst, const void* fy_in,
+                           void* fy_out, void* nut, const DynSgsArgs& a, int write_aux,
+                           hipStream_t st) {
   if (!dyn_step_fits(a.width)) return hipErrorInvalidValue;
-  return geq == 1 ? by_width<1>(a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
-                                write_aux, st)
-                  : by_width<2>(a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+  if (storage != ST_FP64)
+    return launch_dyn_step_lowp(storage, geq, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
                                 write_aux, st);
+  return by_geq<ST_FP64>(geq, a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                         write_aux, st);
 }
+
+#else  // CAVLBM_DYN_OBJECT == 32
+
+hipError_t launch_dyn_step_lowp(int storage, int geq, const Geom& geo, const StepCtx& ctx,
+                                const Walls& walls, const void* src, void* dst,
+                                const void* fy_in, void* fy_out, void* nut, const DynSgsArgs& a,
+                                int write_aux, hipStream_t st) {
+  if (!dyn_step_fits(a.width)) return hipErrorInvalidValue;
+  switch (storage) {
+    case ST_FP32:
+      return by_geq<ST_FP32>(geq, a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut, a,
+                             write_aux, st);
+    case ST_FP16_OFFSET:
+      return by_geq<ST_FP16_OFFSET>(geq, a.width, geo, ctx, walls, src, dst, fy_in, fy_out, nut,
+                                    a, write_aux, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+#endif
 
 }  // namespace cavlbm

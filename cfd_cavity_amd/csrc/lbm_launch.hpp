@@ -121,11 +121,11 @@ constexpr int kDynPlanes = 20;  // planes of scratch + outputs above
 // independent lattices share each launch.
 hipError_t launch_dynamic_sgs(const Geom* geo, const DynSgsArgs* a, int count, bool coeffs,
                               hipStream_t st);
-// One launch per dynamic step from the FP64 C-form (src -> dst): streaming,
-// the model's passes, the coefficient stage and the collision, reading the
-// force plane fy_in and writing fy_out (tiles read their halo's fy).  With
-// write_aux the collision's nu_t and the coefficient planes are stored.
-// Test-filter widths 1..3.
+// One launch per dynamic step from the C-form of any storage (src -> dst):
+// streaming, the model's passes (FP64), the coefficient stage and the
+// collision, reading the force plane fy_in and writing fy_out (tiles read
+// their halo's fy).  With write_aux the collision's nu_t and the coefficient
+// planes are stored.  Test-filter widths 1..3.
 inline bool dyn_step_fits(int width) { return width >= 1 && width <= 3; }
 // The whole model (filter, terms and coefficient stage, writing C_nu, C_kappa,
 // nu_t_pre, alpha_t_pre and, with part, the per-workgroup sums) on count (1
@@ -134,9 +134,16 @@ inline bool dyn_step_fits(int width) { return width >= 1 && width <= 3; }
 // of part per sum, the model writes for a lattice of that geometry.
 hipError_t launch_dynamic_model(const Geom* geo, const DynSgsArgs* a, int count, hipStream_t st);
 int dyn_model_blocks(const Geom& geo, int width);
-hipError_t launch_dyn_step_fp64(int geq, const Geom& geo, const StepCtx& ctx, const Walls& walls,
-                                const void* src, void* dst, const void* fy_in, void* fy_out,
-                                void* nut, const DynSgsArgs& a, int write_aux, hipStream_t st);
+hipError_t launch_dyn_step(int storage, int geq, const Geom& geo, const StepCtx& ctx,
+                           const Walls& walls, const void* src, void* dst, const void* fy_in,
+                           void* fy_out, void* nut, const DynSgsArgs& a, int write_aux,
+                           hipStream_t st);
+// launch_dyn_step's FP32 / FP16-offset forms (the object built with
+// contraction, as the other FP32-arithmetic step kernels).
+hipError_t launch_dyn_step_lowp(int storage, int geq, const Geom& geo, const StepCtx& ctx,
+                                const Walls& walls, const void* src, void* dst,
+                                const void* fy_in, void* fy_out, void* nut, const DynSgsArgs& a,
+                                int write_aux, hipStream_t st);
 __host__ __device__ inline DynCoef dyn_coef_of(const DynSgsArgs& a) {
   DynCoef d;
   d.terms = a.terms;

@@ -427,10 +427,9 @@ class CavitySolver:
         return out.value
 
     def set_dynamic_fusion(self, one_launch):
-        """Fused dynamic-SGS steps (True, the default, where they apply: a
-        test filter of 1..3 nodes; one launch in FP64 storage, three in FP32 /
-        FP16-offset) or four launches (False); all give the same state bit
-        for bit."""
+        """One launch per dynamic-SGS step (True, the default, where it applies:
+        a test filter of 1..3 nodes, any storage) or four (False); both give
+        the same state bit for bit."""
         self._check(self._L.cavlbm_set_dynamic_fusion(self._h, 1 if one_launch else 0))
 
     @property

@@ -201,18 +201,17 @@ int cavlbm_time_step_kernel(cavlbm_solver *s, uint64_t launches, double *mean_ms
  * cavlbm_time_step_kernel times launches of that kind. */
 int cavlbm_set_time_blocking(cavlbm_solver *s, int32_t steps);
 int cavlbm_steps_per_launch(const cavlbm_solver *s, int32_t *out);
-/* Fused dynamic-SGS steps (no reference counterpart; a performance setting of
- * this implementation).  With a test filter of 1..3 nodes (eta_test < 3.5)
- * a dynamic step from the C-form is one kernel in FP64 storage (streaming,
- * the model's filter, similarity scales, Lilly contractions and coefficient
- * stage, and the collision, per tile through LDS) and three in FP32 /
- * FP16-offset storage (the post-stream f planes and fields, the whole model
- * in one tiled launch, the collision).  Otherwise, or with one_launch = 0,
- * it is four (the f planes and fields of the post-stream state, the filter,
- * the contractions, the collision with the coefficient stage).  All forms
- * give the same populations and per-node values bit for bit.  The default
- * is fused.  cavlbm_dynamic_launches reports the launches per dynamic step
- * for the handle's current settings. */
+/* One launch per dynamic-SGS step (no reference counterpart; a performance
+ * setting of this implementation).  With a test filter of 1..3 nodes
+ * (eta_test < 3.5) a dynamic step from the C-form is one kernel in every
+ * storage: streaming, the model's filter, similarity scales, Lilly
+ * contractions and coefficient stage (FP64), and the collision, per tile
+ * through LDS.  Otherwise, or with one_launch = 0, it is four (the f planes
+ * and fields of the post-stream state, the filter, the contractions, the
+ * collision with the coefficient stage).  Both give the same populations and
+ * per-node values bit for bit.  The default is one launch.
+ * cavlbm_dynamic_launches reports the launches per dynamic step for the
+ * handle's current settings. */
 int cavlbm_set_dynamic_fusion(cavlbm_solver *s, int32_t one_launch);
 int cavlbm_dynamic_launches(const cavlbm_solver *s, int32_t *out);
 

@@ -303,18 +303,19 @@ def test_wide_filter_keeps_four_launches():
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16_offset"])
 @pytest.mark.parametrize("model", [(0.5, -0.1, 2.0), (0.5, -0.1, 1.0), (0.3, -0.1, 3.0)])
-def test_low_precision_three_launches_equal_four(storage, model):
-    """FP32 / FP16-offset storage: the fused dynamic step (the whole model in
-    one tiled launch, then the collision with its nu_t / alpha_t) gives the
-    four-launch step's state bit for bit, through direct steps and graph
-    replays."""
+@pytest.mark.parametrize("nx,ny", [(50, 70), (3, 40), (33, 17)])
+def test_low_precision_one_launch_equals_four(storage, model, nx, ny):
+    """FP32 / FP16-offset storage: the one-launch dynamic step (the pull and
+    collision in the storage's arithmetic, the model in FP64 on the pulled
+    values) gives the four-launch step's state bit for bit, through direct
+    steps and graph replays."""
     runs = []
     for fuse in (True, False):
-        s = CavitySolver(50, 70, 1e6, 0.71, 0.1, 0.9, 0.02, 42, 1, storage=storage)
+        s = CavitySolver(nx, ny, 1e6, 0.71, 0.1, 0.9, 0.02, 42, 1, storage=storage)
         s.run(190)
         s.set_dynamic_fusion(fuse)
         s.set_dynamic_sgs(True, *model)
-        assert s.dynamic_launches == (3 if fuse else 4)
+        assert s.dynamic_launches == (1 if fuse else 4)
         runs.append(s)
     for chunk in (1, 2, 7, 60):
         for s in runs:
