@@ -181,3 +181,23 @@ def test_cli_octree_dynamic_smag(tmp_path):
     s.run(120)
     cs, _, kap, _, _ = s.dynamic_stats()
     assert f"cs_dyn = {cs:.4f} | kap_dyn = {kap:.4f}" in lines[-1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,tol", [("fp32", 1e-3), ("fp16_offset", 2e-2)])
+def test_cli_dynamic_smag_low_precision(tmp_path, storage, tol):
+    """--storage fp32 / fp16_offset with --dynamic-smag (the one-launch step in
+    that storage, the model in FP64): the run completes with the coefficient
+    suffix on every progress line, and its time-averaged Nusselt number stays
+    within the stated relative tolerance of the FP64 run."""
+    nus = {}
+    for st in ("fp64", storage):
+        out = tmp_path / st
+        r = run("--nx", 24, "--ny", 96, "--ra", 1e6, "--steps", 1200, "--sample-start", 600,
+                "--sample-every", 50, "--progress-every", 400, "--dynamic-smag",
+                "--storage", st, "--out", out)
+        assert r.returncode == 0, r.stderr
+        lines = (out / "progress.log").read_text().splitlines()
+        assert lines and all(" | cs_dyn_mean = " in ln for ln in lines)
+        nus[st] = json.loads((out / "summary.json").read_text())["nusselt_hot"]
+    assert abs(nus[storage] - nus["fp64"]) <= tol * abs(nus["fp64"]), nus
