@@ -33,11 +33,12 @@ namespace {
 constexpr double kCs2d = 1.0 / 3.0;
 constexpr double kLambda = 3.0 / 16.0;
 
+// geo carries the row_tiles grid.
 __device__ __forceinline__ bool node_of(const Geom& geo, int& ix, int& iy) {
   int bx, by;
   tile_of_block(geo, bx, by);
   ix = bx * kTileX + (int)(threadIdx.x % kTileX);
-  iy = by * kTileY + (int)(threadIdx.x / kTileX);
+  iy = by * kRowTileY + (int)(threadIdx.x / kTileX);
   return ix < geo.nx && iy < geo.ny;
 }
 
@@ -107,7 +108,7 @@ __device__ __forceinline__ void gather_lambda(const Geom& geo, const double* mu,
 }
 
 template <int GEQ, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_adj_node(Geom geo, StepCtx c, AdjArgs a) {
+__global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, AdjArgs a) {
   int ix, iy;
   if (!node_of(geo, ix, iy)) return;
   const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
@@ -352,17 +353,18 @@ hipError_t launch_nu_rows(const Geom& geo, int geq, const double* X, const doubl
 
 hipError_t launch_adj_step(const Geom& geo, int geq, bool first, const StepCtx& ctx,
                            const AdjArgs& a, hipStream_t st) {
-  const dim3 grid(nblocks_of(geo)), block(kBlock);
+  const Geom gr = row_tiles(geo);
+  const dim3 grid(nblocks_of(gr)), block(kRowBlock);
   if (geq == 1) {
     if (first)
-      hipLaunchKernelGGL((k_adj_node<1, true>), grid, block, 0, st, geo, ctx, a);
+      hipLaunchKernelGGL((k_adj_node<1, true>), grid, block, 0, st, gr, ctx, a);
     else
-      hipLaunchKernelGGL((k_adj_node<1, false>), grid, block, 0, st, geo, ctx, a);
+      hipLaunchKernelGGL((k_adj_node<1, false>), grid, block, 0, st, gr, ctx, a);
   } else {
     if (first)
-      hipLaunchKernelGGL((k_adj_node<2, true>), grid, block, 0, st, geo, ctx, a);
+      hipLaunchKernelGGL((k_adj_node<2, true>), grid, block, 0, st, gr, ctx, a);
     else
-      hipLaunchKernelGGL((k_adj_node<2, false>), grid, block, 0, st, geo, ctx, a);
+      hipLaunchKernelGGL((k_adj_node<2, false>), grid, block, 0, st, gr, ctx, a);
   }
   return hipGetLastError();
 }

@@ -75,7 +75,7 @@ __device__ __forceinline__ void nu_row_add(Dual dlt, double* acc, double* dacc, 
 // One tangent step; with acc (a step that pulls, m >= 1) it also accumulates
 // the Nusselt row terms of the state S_m it pulls, as the window step does.
 template <int GEQ, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_tangent_step(Geom geo, StepCtx ctx, double dcs,
+__global__ __launch_bounds__(kRowBlock) void k_tangent_step(Geom geo, StepCtx ctx, double dcs,
                                                           const double* theta,
                                                           const double* dtheta,
                                                           const double* X, const double* Xd,
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_tangent_step(Geom geo, StepCtx ctx, 
   int bx, by;
   tile_of_block(geo, bx, by);
   const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
-  const int iy = by * kTileY + (int)(threadIdx.x / kTileX);
+  const int iy = by * kRowTileY + (int)(threadIdx.x / kTileX);  // geo: the row_tiles grid
   const int nx = geo.nx, ny = geo.ny, pitch = geo.pitch;
   if (ix >= nx || iy >= ny) return;
   const long long P = geo.plane;
@@ -185,9 +185,10 @@ hipError_t launch_tangent_step(const Geom& geo, int geq, bool first, const StepC
                                double* Y, double* Yd, const double* mpops, const double* mrho,
                                const double* mux, const double* muy, const double* mtemp,
                                double* acc, double* dacc, hipStream_t st) {
-  const dim3 grid(nblocks_of(geo)), block(kBlock);
+  const Geom gr = row_tiles(geo);
+  const dim3 grid(nblocks_of(gr)), block(kRowBlock);
 #define CAVLBM_T(G, F)                                                                      \
-  hipLaunchKernelGGL((k_tangent_step<G, F>), grid, block, 0, st, geo, ctx, dcs, theta, dtheta, \
+  hipLaunchKernelGGL((k_tangent_step<G, F>), grid, block, 0, st, gr, ctx, dcs, theta, dtheta, \
                      X, Xd, fy, fyd, Y, Yd, mpops, mrho, mux, muy, mtemp, acc, dacc)
   if (geq == 1) {
     if (first) CAVLBM_T(1, true); else CAVLBM_T(1, false);
