@@ -16,7 +16,10 @@ namespace cavlbm {
 
 namespace {
 
-constexpr int kOctBlock = 256;
+// One wave per workgroup: the levels and interfaces are small, so the
+// launches' few thousand lanes spread over more CUs (64x256 base grid: two
+// levels 45.1 -> 43.9 us per coarse step, three 78.3 -> 75.0 against 256).
+constexpr int kOctBlock = 64;
 
 __device__ __forceinline__ int node_id(long long n) {
   const long long k = (long long)blockIdx.x * kOctBlock + threadIdx.x;
