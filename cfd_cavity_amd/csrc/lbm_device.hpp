@@ -188,6 +188,12 @@ struct StepBufs {
 constexpr int kTileX = 64;
 constexpr int kTileY = 4;
 constexpr int kBlock = kTileX * kTileY;
+// The objective window's single steps and its backward and tangent steps run
+// one 64-lane wave per workgroup (tiles of kTileX x kRowTileY nodes): their
+// lattices are small, so the waves spread over every CU instead of filling a
+// fraction of the CUs four waves each.
+constexpr int kRowTileY = 1;
+constexpr int kRowBlock = kTileX * kRowTileY;
 
 // ---------------------------------------------------------------------------
 // Arithmetic policy.
@@ -699,10 +705,11 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
 template <int S, int GEQ, int MODE>
 __global__ __launch_bounds__(kBlock) void k_step(Geom geo, StepCtx ctx, Walls walls,
                                                   StepBufs b, int write_aux) {
+  constexpr int kTY = MODE == MODE_WINDOW ? kRowTileY : kTileY;  // geo: its tile grid
   int bx, by;
   tile_of_block(geo, bx, by);
   const int ix = bx * kTileX + (int)(threadIdx.x % kTileX);
-  const int iy = by * kTileY + (int)(threadIdx.x / kTileX);
+  const int iy = by * kTY + (int)(threadIdx.x / kTileX);
   if (ix >= geo.nx || iy >= geo.ny) return;
   step_node<S, GEQ, MODE>(geo, ctx, walls, b, write_aux, ix, iy);
 }

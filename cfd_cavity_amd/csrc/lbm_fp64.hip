@@ -8,8 +8,10 @@ namespace cavlbm {
 template <int GEQ, int MODE>
 static hipError_t go(const Geom& geo, const StepCtx& ctx, const Walls& walls,
                      const StepBufs& b, int write_aux, hipStream_t st) {
-  hipLaunchKernelGGL((k_step<ST_FP64, GEQ, MODE>), dim3(nblocks_of(geo)), dim3(kBlock), 0,
-                     st, geo, ctx, walls, b, write_aux);
+  const Geom g = MODE == MODE_WINDOW ? row_tiles(geo) : geo;
+  hipLaunchKernelGGL((k_step<ST_FP64, GEQ, MODE>), dim3(nblocks_of(g)),
+                     dim3(MODE == MODE_WINDOW ? kRowBlock : kBlock), 0, st, g, ctx, walls, b,
+                     write_aux);
   return hipGetLastError();
 }
 

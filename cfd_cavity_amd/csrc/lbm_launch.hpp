@@ -81,12 +81,9 @@ hipError_t launch_field_convert(const Geom& geo, int storage, const void* src_r,
 
 inline int nblocks_of(const Geom& g) { return g.nbx * g.nby; }
 
-// The backward and tangent steps of the objective window run one 64-lane
-// wave per workgroup (tiles of kTileX x kRowTileY nodes): their lattices are
-// small, so the waves spread over every CU instead of filling a quarter of
-// the CUs four waves each.  row_tiles: geo with that tile grid.
-constexpr int kRowTileY = 1;
-constexpr int kRowBlock = kTileX * kRowTileY;
+// row_tiles: geo with the tile grid of kTileX x kRowTileY tiles (one wave
+// per workgroup), for the objective window's single steps, backward and
+// tangent steps.
 inline Geom row_tiles(const Geom& geo) {
   Geom g = geo;
   g.nby = (geo.ny + kRowTileY - 1) / kRowTileY;
