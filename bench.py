@@ -106,31 +106,13 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
     return res
 
 
-BATCH_SIZES = [(64, 256), (96, 384), (128, 512), (192, 768), (256, 1024)]
-
-
-def batch_specs(n=64, seed=42):
-    """64 cavities: sizes drawn (seeded) from BATCH_SIZES, Ra log-uniform in
-    [1e6, 1e9], Cs uniform in [0.08, 0.12], Pr 0.71, utarget 0.02."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    specs = []
-    for i in range(n):
-        nx, ny = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
-        ra = float(10 ** rng.uniform(6.0, 9.0))
-        cs = float(rng.uniform(0.08, 0.12))
-        specs.append(dict(nx=nx, ny=ny, ra=ra, cs=cs, pr=0.71, pr_t=0.9, utarget=0.02,
-                          seed=seed + i))
-    return specs
-
-
 def batched_form(steps: int, warmup: int, storage: str = "fp32"):
     """One launch per step for all cavities (graph replays), autotuned, against
     the same cavities stepped one after another with the single-cavity path."""
     from cfd_cavity_amd import CavityBatch, CavitySolver
+    from cfd_cavity_amd.batch import seeded_batch_specs
 
-    specs = batch_specs()
+    specs = seeded_batch_specs()
     b = CavityBatch(specs, storage=storage)
     nodes = b.nodes()
     b.run(warmup)

@@ -9,6 +9,28 @@ from . import _lib
 from ._lib import CavlbmError, lib
 from .solver import CavitySolver, make_params
 
+# Grid sizes of the batched workload, all of aspect ratio Az = 4 as the
+# reference's cavities (README.md:69 lists 64x256, 96x384, 128x512; its Ra=1e10
+# runs use 256x1024, README.md:167).
+BATCH_SIZES = [(64, 256), (96, 384), (128, 512), (192, 768), (256, 1024)]
+
+
+def seeded_batch_specs(n=64, seed=42):
+    """``n`` cavities: sizes drawn (seeded) from BATCH_SIZES, Ra log-uniform in
+    [1e6, 1e9], Cs uniform in [0.08, 0.12], Pr 0.71, Pr_t 0.9, utarget 0.02,
+    cavity i seeded with seed + i (numpy PCG64, stable across versions)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        nx, ny = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        ra = float(10 ** rng.uniform(6.0, 9.0))
+        cs = float(rng.uniform(0.08, 0.12))
+        specs.append(dict(nx=nx, ny=ny, ra=ra, cs=cs, pr=0.71, pr_t=0.9, utarget=0.02,
+                          seed=seed + i))
+    return specs
+
 
 class CavityBatch:
     """``specs``: list of dicts with CavitySolver::new arguments (nx, ny, ra,

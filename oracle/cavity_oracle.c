@@ -254,6 +254,11 @@ void orc_set_hot_wall(orc_solver *s, const double *theta_hot) {
   }
 }
 
+/* The Smagorinsky constant of later collisions (the `cs` field read by
+ * collide_node, solver.rs:1120); the window objective of the differentiation
+ * form runs its steps with a chosen Cs. */
+void orc_set_cs(orc_solver *s, double cs) { s->cs = cs; }
+
 /* ---- init_from_fields (src/uniform/solver.rs:232-262) --------------------- */
 int orc_init_from_fields(orc_solver *s, const double *rho, const double *ux,
                          const double *uy, const double *t, size_t n) {

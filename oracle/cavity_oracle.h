@@ -56,6 +56,7 @@ int orc_init_from_fields(orc_solver *s, const double *rho, const double *ux,
 /* Optional hot-wall temperature profile theta_hot[iy] (ny values) in place of
  * the constant T_HOT; NULL restores the constant. */
 void orc_set_hot_wall(orc_solver *s, const double *theta_hot);
+void orc_set_cs(orc_solver *s, double cs);
 
 /* Raw state access (numpy wraps these pointers). */
 double *orc_field(orc_solver *s, int which);

@@ -52,6 +52,7 @@ def lib():
         "orc_enforce_wall_velocity": (None, [vp]),
         "orc_init_from_fields": (i, [vp, dp, dp, dp, dp, sz]),
         "orc_set_hot_wall": (None, [vp, dp]),
+        "orc_set_cs": (None, [vp, d]),
         "orc_set_dynamic_sgs": (None, [vp, i, d, d, d]),
         # octree (octree_oracle.h)
         "orc_oct_new": (vp, [sz, sz, sz, d, d, d, u64, sz, i, i, d, i, d, i, C.c_char_p, sz]),
@@ -260,6 +261,9 @@ class OracleSolver:
             a = np.ascontiguousarray(theta, dtype=np.float64)
             assert a.size == self.ny
             self._L.orc_set_hot_wall(self._h, _dp(a))
+
+    def set_cs(self, cs):
+        self._L.orc_set_cs(self._h, float(cs))
 
     def set_dynamic_sgs(self, on, smag_alpha=0.5, nu_sm_min=-0.1, eta_test=2.0):
         self._L.orc_set_dynamic_sgs(self._h, 1 if on else 0, smag_alpha, nu_sm_min, eta_test)

@@ -1,7 +1,11 @@
-"""Hand-written adjoint of the fused FP64 step: gradients of the window-averaged
-hot-wall Nusselt number with respect to the hot-wall profile and Cs, checked
-against central finite differences through the torch.autograd.Function
-wrapper, and invariant under the checkpoint interval."""
+"""Hand-written adjoint of the fused FP64 step: the window objective J against
+the oracle (the mean of post.rs's nusselt_hot over the window's steps, with
+the same hot-wall profile and Cs), gradients of J with respect to the hot-wall
+profile and Cs checked against central finite differences through the
+torch.autograd.Function wrapper and against the FP64 tangent, and invariance
+under the checkpoint interval."""
+import os
+
 import numpy as np
 import pytest
 
@@ -57,6 +61,48 @@ def fd_check(s, theta, cs, K, C, dirs, h=1e-6, tol=1e-4):
     assert max(errs) <= tol, errs
     assert max(terrs) <= 1e-8, terrs
     return float(j), gth, gcs
+
+
+WINDOW_CASES = [
+    # nx, ny, Ra, steps before the window, K, checkpoint interval, Cs, profile
+    (16, 40, 1e5, 200, 61, 16, 0.10, False),
+    (24, 64, 1e6, 230, 45, 7, 0.13, True),
+    (17, 35, 2e5, 90, 38, 5, 0.08, True),       # window starts inside the spin-up
+    (96, 384, 1e7, 8000, 2000, 100, 0.10, True),  # the adjoint configuration
+]
+
+
+@pytest.mark.parametrize("nx,ny,ra,t0,K,C,cs,profile", WINDOW_CASES)
+def test_window_objective_matches_oracle(oracle, nx, ny, ra, t0, K, C, cs, profile):
+    """J = (1/K) sum_{m=1..K} nusselt_hot(S_m) (post.rs:68-78): the oracle
+    steps K times from the same state S_0 with the hot-wall profile and Cs set
+    and averages its nusselt_hot after each step.  The sums run in another
+    order (rows per step there, steps per row here), hence 1e-12 relative.
+    The window's J, the adjoint's and the tangent's are the same number.  The
+    mean over S_0..S_{K-1} (a window shifted by one step) must miss by more
+    than the tolerance, so an off-by-one in the window cannot pass."""
+    s = spun_up(nx, ny, ra, t0)
+    jobs = min(16, os.cpu_count() or 1)
+    o = oracle.OracleSolver(nx, ny, ra, 0.71, 0.10, 0.9, 0.02, 42, jobs)
+    o.run(t0)
+    theta = (0.5 + 0.01 * np.random.default_rng(nx * ny).uniform(-1, 1, ny)) if profile else None
+    o.set_hot_wall(theta)
+    o.set_cs(cs)
+    nus = [o.diagnostics()["nusselt_hot"]]
+    for _ in range(K):
+        o.step()
+        nus.append(o.diagnostics()["nusselt_hot"])
+    j_oracle = sum(nus[1:]) / K
+    j_shifted = sum(nus[:-1]) / K
+    j = nu_window(s, theta, cs, K)
+    print(f"\n[window] {nx}x{ny} t0={t0} K={K}: J={j:.15g} oracle={j_oracle:.15g} "
+          f"rel={abs(j - j_oracle) / abs(j_oracle):.2e} shifted rel="
+          f"{abs(j - j_shifted) / abs(j_oracle):.2e}")
+    assert abs(j - j_oracle) <= 1e-12 * abs(j_oracle)
+    assert abs(j - j_shifted) > 1e-12 * abs(j_oracle)
+    assert adjoint_nu_window(s, theta, cs, K, C)[0] == j
+    assert tangent_nu_window(s, theta, cs, None, 0.0, K)[0] == j
+    assert s.time_step == t0
 
 
 def test_adjoint_small_vs_finite_differences():

@@ -1,14 +1,24 @@
-"""Batched form: every cavity of a batch must equal, bit for bit, the same
-cavity stepped alone (same per-node code), across graph replays and after
-the autotuner (which must leave the state unchanged).  A batch advances
-several steps per launch (the N-step tiles) like a single cavity does; the
-comparisons run each side with its own choice, so they also check the
-batched N-step kernel against the single-cavity kernels."""
+"""Batched form.  In FP64 every cavity of a batch is bitwise the oracle's
+cavity: the seeded 64-cavity workload (sizes 64x256 .. 256x1024, Ra in
+[1e6, 1e9], Cs in [0.08, 0.12]) against the oracle's per-cavity digests after
+300 steps, and a small mixed batch against live oracle runs.  In every storage
+a batched cavity equals, bit for bit, the same cavity stepped alone (same
+per-node code), across graph replays and after the autotuner (which must
+leave the state unchanged); the FP32 workload stays finite and bounded for
+its 50 000 steps.  A batch advances several steps per launch (the N-step
+tiles) like a single cavity does; the comparisons run each side with its own
+choice, so they also check the batched N-step kernel against the
+single-cavity kernels."""
+import os
+
 import numpy as np
 import pytest
 
 from cfd_cavity_amd import CavityBatch, CavitySolver
+from cfd_cavity_amd.batch import seeded_batch_specs
+from tests.golden.make_golden import BATCH_KEY, BATCH_STEPS, FIELDS
 from tests.helpers import bits_equal
+from tests.test_golden import assert_diagnostics_close, device_digest, load_digests
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +43,60 @@ def same_state(a, b):
         if not bits_equal(a.get_pops(w), b.get_pops(w)):
             return w
     return None
+
+
+def test_seeded_batch64_fp64_matches_oracle_digests():
+    """The 64 seeded cavities in one FP64 batch, run in chunks that cover the
+    spin-up's direct launches, graph replays and a remainder (1 + 150 + 149
+    steps), are each bitwise the oracle's cavity after 300 steps."""
+    specs = seeded_batch_specs()
+    want = load_digests()[BATCH_KEY]
+    assert len(want) == len(specs) == 64
+    b = CavityBatch(specs, storage="fp64")
+    for chunk in (1, 150, BATCH_STEPS - 151):
+        b.run(chunk)
+    for i, (c, sp, dig) in enumerate(zip(b, specs, want)):
+        assert dig["args"][:2] == [sp["nx"], sp["ny"]] and dig["steps"] == BATCH_STEPS
+        assert c.time_step == BATCH_STEPS
+        assert device_digest(c) == dig["sha256"], f"cavity {i} ({sp['nx']}x{sp['ny']})"
+        assert_diagnostics_close(c.diagnostics(), dig["diagnostics"])
+
+
+def test_small_batch_fp64_matches_live_oracle(oracle):
+    """A mixed FP64 batch (one lattice narrower than a tile) against the
+    oracle run beside it, after the spin-up and after graph replays."""
+    batch = CavityBatch(SPECS, storage="fp64")
+    jobs = min(16, os.cpu_count() or 1)
+    ors = [oracle.OracleSolver(sp["nx"], sp["ny"], sp["ra"], 0.71, sp["cs"], 0.9, 0.02,
+                               sp.get("seed", 42), jobs) for sp in SPECS]
+    for chunk in (3, 250):
+        batch.run(chunk)
+        for o in ors:
+            o.run(chunk)
+        for i, (c, o) in enumerate(zip(batch, ors)):
+            assert bits_equal(c.get_pops("f"), o.f) and bits_equal(c.get_pops("g"), o.g), i
+            for n in FIELDS:
+                assert bits_equal(c.get_field(n), getattr(o, n)), (i, n)
+
+
+def test_seeded_batch64_fp32_50k_steps_stays_bounded():
+    """The batched workload as specified: 64 seeded cavities, FP32 storage,
+    50 000 steps.  Every cavity stays finite, under the reference's abort
+    threshold max|u| < 0.10 (src/main.rs:215-275), with its mass within 1e-4
+    of the initial nx*ny."""
+    specs = seeded_batch_specs()
+    b = CavityBatch(specs, storage="fp32")
+    b.run(50000)
+    worst = (0.0, 0.0)
+    for i, (c, sp) in enumerate(zip(b, specs)):
+        d = c.diagnostics()
+        n = sp["nx"] * sp["ny"]
+        drift = abs(d["total_mass"] - n) / n
+        worst = (max(worst[0], d["max_u"]), max(worst[1], drift))
+        assert d["all_finite"] and d["time_step"] == 50000, i
+        assert d["max_u"] < 0.10, (i, d["max_u"])
+        assert drift < 1e-4, (i, drift)
+    print(f"\n[batch64 fp32 50k] max |u| {worst[0]:.4f}, max mass drift {worst[1]:.2e}")
 
 
 @pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16_offset"])
