@@ -8,7 +8,10 @@ stream+BC+moments+collide pass over the whole lattice; inputs stay resident
 in HBM.  The run loop advances several steps per kernel launch (temporal
 blocking, bitwise equal to single steps: 4 in FP64, 3 in FP32 and FP16-offset); the
 "fp64_2step" and "fp64_1step" forms time two steps per launch and the
-single-step kernel beside it.  Prints ONE JSON line.
+single-step kernel beside it.  The timed region (exactly --steps steps after
+--warmup untimed ones) runs five times; ms_per_step and value come from HIP
+events on the solver stream (median of the five), the host wall clock around
+the synchronisations is reported beside them.  Prints ONE JSON line.
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
 rayon path) timed on this host's cores on a bounded sample of the same
@@ -53,10 +56,18 @@ def make_solver(storage: str, tblock=None):
     return s
 
 
-def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
+def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int = 5):
     """One storage form with the library's default steps per launch; a name
     ending in "_<n>step" runs the same storage with at most n steps per
-    launch (n = 1: the single-step kernel), for comparison."""
+    launch (n = 1: the single-step kernel), for comparison.
+
+    After `warmup` untimed steps the timed region -- exactly `steps` steps,
+    bracketed by stream and device synchronisations -- runs `reps` times in
+    a row.  Each repetition is timed by HIP events recorded on the solver's
+    stream around its launches and by the host clock around the
+    synchronisations; the figures are the medians over the repetitions."""
+    import statistics
+
     import torch
 
     storage, tblock = form, None
@@ -67,16 +78,20 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
     spl = s.steps_per_launch
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     s.run(warmup)
-    s.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev_ms = s.time_run(steps)
-    s.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    ev, walls = [], []
+    for _ in range(reps):
+        s.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev.append(s.time_run(steps))
+        s.synchronize()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    ev_ms = statistics.median(ev)
+    wall = statistics.median(walls)
     d = s.diagnostics()
     kern_ms = s.time_step_kernel(kernel_launches) if kernel_launches > 0 else float("nan")
-    mlups = nodes * steps / wall / 1e6
+    mlups = nodes * steps / (ev_ms * 1e-3) / 1e6
     b = BYTES_PER_LUP[storage]
     # average step-kernel launch duration over the timed region, from HIP
     # events recorded on the solver's stream (steps / spl launches), and the
@@ -86,8 +101,10 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int):
     res = {
         "steps_per_launch": spl,
         "mlups": mlups,
-        "ms_per_step": wall * 1e3 / steps,
-        "ms_per_step_events": ev_ms / steps,
+        "ms_per_step": ev_ms / steps,
+        "ms_per_step_wall": wall * 1e3 / steps,
+        "mlups_wall": nodes * steps / wall / 1e6,
+        "ms_per_step_events_reps": [round(v / steps, 6) for v in ev],
         "kernel_ms_isolated": kern_ms,
         "launch_ms": launch_ms,
         "bytes_per_lup": b,
@@ -133,7 +150,10 @@ def batched_form(steps: int, warmup: int, storage: str = "fp32"):
             "mlups": mlups, "ms_per_step": ms / steps, "sequential_mlups": seq_mlups,
             "speedup_vs_sequential": mlups / seq_mlups, "tuned": tune,
             "finite_cavities": finite,
-            "hbm_frac": mlups * 1e6 * BYTES_PER_LUP[storage] / (HBM_PEAK_GBS * 1e9)}
+            # MLUPS x the one-step algorithm's bytes per update over the HBM
+            # peak: an effective rate, which exceeds 1 once several steps
+            # run per launch (the state then crosses HBM once per N updates)
+            "effective_hbm_frac": mlups * 1e6 * BYTES_PER_LUP[storage] / (HBM_PEAK_GBS * 1e9)}
 
 
 def measured_traffic(steps_per_launch: int):
@@ -148,6 +168,31 @@ def measured_traffic(steps_per_launch: int):
         for line in open(os.path.join(ROOT, "profiles", name)):
             if line.startswith(key):
                 return int(line.split("=")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
+CLOCK_GHZ = 2.4          # MI355X peak engine clock
+SIMDS = 256 * 4          # 256 CUs x 4 SIMDs
+FP64_CYCLES_PER_WAVE_INST = 4   # 16 FP64 lanes per SIMD: a wave64 instruction in 4 cycles
+
+
+def fp64_issue_bound(steps_per_launch: int, launch_ms: float):
+    """The second bound of the multi-step kernel: its FP64 VALU issue time
+    (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per launch, from the committed
+    counter summary, at 4 cycles per wave instruction over the 1024 SIMDs at
+    the peak clock) as a fraction of the launch time; None when the summary
+    holds no figure for this kernel."""
+    key = f"FP64_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH"
+    try:
+        for line in open(os.path.join(ROOT, "profiles", "k_stepn_256x1024.md")):
+            if line.startswith(key):
+                n = float(line.split("=")[1].strip())
+                busy_ms = n * FP64_CYCLES_PER_WAVE_INST / SIMDS / (CLOCK_GHZ * 1e6)
+                return {"wave_instructions_per_launch": n, "issue_ms": round(busy_ms, 6),
+                        "issue_frac": round(busy_ms / launch_ms, 4),
+                        "clock_ghz": CLOCK_GHZ, "source": "profiles/k_stepn_256x1024.md"}
     except OSError:
         pass
     return None
@@ -302,10 +347,13 @@ def cpu_baseline(budget_s: float):
     """The oracle (row-band OpenMP restatement of the rayon path) on a bounded
     sample of the same workload, with all host threads (max 16), and on a
     shorter sample with one thread."""
-    threads = min(os.cpu_count() or 1, 16)
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    # 16 threads: the host CPU share of one GPU on the measurement boxes
+    threads = min(affinity or os.cpu_count() or 1, 16)
     v, steps, dt = cpu_rate(threads, budget_s)
     v1, steps1, dt1 = cpu_rate(1, max(budget_s / 4, 1.0))
     return {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "host_cpu_count": os.cpu_count(), "affinity_cpu_count": affinity,
             "sample": f"{steps} steps of 256x1024 Ra=1e10 from the reference init "
                       f"({dt:.1f} s, oracle/cavity_oracle.c, {threads} OpenMP row-band threads)",
             "value_1thread": v1,
@@ -339,37 +387,96 @@ def copy_bandwidth():
     return out
 
 
-def precision_run(steps: int):
-    """The precision comparison at 256x1024, Ra=1e10: FP64, FP32 and
-    FP16-offset storage from the same init for `steps` steps.  Reports per
-    form Nu (hot, cold) at the end, the mean Nu_hot over the second half
-    (sampled every 1000 steps), w_max on the mid-height row ny/2
-    (src/main.rs:325-330), and the relative mass drift; the reduced forms
-    also their relative errors against FP64."""
+GATE_CASES = {
+    # steady cavities (20k steps) and the 256x1024 Ra=1e10 configuration after
+    # 10k steps (plumes formed, before the flow turns chaotic)
+    "64x256_ra1e6": (dict(nx=64, ny=256, ra=1.0e6, utarget=0.02), 20000),
+    "96x384_ra1e7": (dict(nx=96, ny=384, ra=1.0e7, utarget=0.02), 20000),
+    "256x1024_ra1e10": (dict(nx=256, ny=1024, ra=1.0e10, utarget=0.005), 10000),
+}
+
+
+def precision_gate():
+    """The metric's "mixed vs FP64 at rel dNu <= 1e-4": per case, FP32 and
+    FP16-offset storage against FP64 from the same init after the same steps
+    (rel |dNu| over the hot and cold walls, rel |dw_max| at mid-height, mass
+    drift)."""
+    from cfd_cavity_amd import CavitySolver
+
+    out = {}
+    for name, (c, steps) in GATE_CASES.items():
+        sols = {st: CavitySolver(c["nx"], c["ny"], c["ra"], 0.71, 0.10, 0.9, c["utarget"], 42, 1,
+                                 storage=st) for st in ("fp64", "fp32", "fp16_offset")}
+        m0 = c["nx"] * c["ny"]
+        vals = {}
+        for st, s in sols.items():
+            s.run(steps)
+            d = s.diagnostics()
+            w, _ = s.max_uy_on_row(c["ny"] // 2)
+            vals[st] = (d["nusselt_hot"], d["nusselt_cold"], w, abs(d["total_mass"] - m0) / m0)
+            s.close()
+        r = vals["fp64"]
+        row = {"steps": steps, "nusselt_hot_fp64": r[0], "mass_drift_fp64": r[3]}
+        for st in ("fp32", "fp16_offset"):
+            v = vals[st]
+            row[st] = {"rel_dNu": max(abs(v[0] - r[0]) / abs(r[0]), abs(v[1] - r[1]) / abs(r[1])),
+                       "rel_dw_max_mid": abs(v[2] - r[2]) / abs(r[2]), "mass_drift": v[3]}
+            row[st]["meets_1e-4"] = row[st]["rel_dNu"] <= 1e-4
+        out[name] = row
+    return out
+
+
+def precision_run(steps: int, batches: int = 10):
+    """The precision comparison at 256x1024, Ra=1e10 over `steps` steps: FP64,
+    FP32 and FP16-offset storage from the same init, plus an FP64 run from a
+    different seed (44; seeds 42 and 43 give the same state, the generator
+    is seeded with seed | 1) as the control of the flow's own variability.
+    Nu_hot is sampled every 1000 steps over the second half; its mean comes
+    with the standard error from `batches` batch means.  Per form: Nu (hot,
+    cold) at the end, the second-half mean and its sigma, w_max on the
+    mid-height row ny/2 (src/main.rs:325-330), the relative mass drift; the
+    other runs also their differences to the FP64 run, with the difference of
+    the means in units of the combined sigma."""
+    import numpy as np
+
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
-    out = {"steps": steps}
-    for storage in ("fp64", "fp32", "fp16_offset"):
-        s = make_solver(storage)
+    out = {"steps": steps, "sample_every": 1000, "batches": batches}
+    runs = (("fp64", "fp64", 42), ("fp32", "fp32", 42), ("fp16_offset", "fp16_offset", 42),
+            ("fp64_seed44", "fp64", 44))
+    for name, storage, seed in runs:
+        w = WORKLOAD
+        from cfd_cavity_amd import CavitySolver
+
+        s = CavitySolver(w["nx"], w["ny"], w["ra"], w["pr"], w["cs"], w["pr_t"], w["utarget"],
+                         seed, 1, storage=storage)
         m0 = s.diagnostics()["total_mass"]
         s.run(steps // 2)
-        acc, n = 0.0, 0
+        samples = []
         for _ in range(steps // 2 // 1000):
             s.run(1000)
-            acc += s.diagnostics()["nusselt_hot"]
-            n += 1
-        s.run(steps - steps // 2 - 1000 * n)
+            samples.append(s.diagnostics()["nusselt_hot"])
+        s.run(steps - steps // 2 - 1000 * len(samples))
         d = s.diagnostics()
-        w, _ = s.max_uy_on_row(WORKLOAD["ny"] // 2)
-        out[storage] = {"nusselt_hot": d["nusselt_hot"], "nusselt_cold": d["nusselt_cold"],
-                        "mean_nusselt_hot_2nd_half": acc / max(n, 1), "w_max_mid": w,
-                        "mass_drift_rel": abs(d["total_mass"] - m0) / m0,
-                        "all_finite": d["all_finite"], "time_step": d["time_step"]}
+        wmax, _ = s.max_uy_on_row(WORKLOAD["ny"] // 2)
+        x = np.array(samples)
+        nb = min(batches, max(len(x), 1))
+        bm = np.array([v.mean() for v in np.array_split(x, nb)]) if len(x) else np.zeros(1)
+        sigma = float(bm.std(ddof=1) / np.sqrt(nb)) if nb > 1 else float("nan")
+        out[name] = {"nusselt_hot": d["nusselt_hot"], "nusselt_cold": d["nusselt_cold"],
+                     "mean_nusselt_hot_2nd_half": float(x.mean()) if len(x) else float("nan"),
+                     "sigma_mean_nusselt_hot": sigma, "w_max_mid": wmax,
+                     "mass_drift_rel": abs(d["total_mass"] - m0) / m0,
+                     "all_finite": d["all_finite"], "time_step": d["time_step"]}
         s.close()
     ref = out["fp64"]
-    for storage in ("fp32", "fp16_offset"):
-        v = out[storage]
+    for name in ("fp32", "fp16_offset", "fp64_seed44"):
+        v = out[name]
         for key in ("nusselt_hot", "mean_nusselt_hot_2nd_half", "w_max_mid"):
             v["rel_err_" + key] = abs(v[key] - ref[key]) / abs(ref[key])
+        comb = (v["sigma_mean_nusselt_hot"] ** 2 + ref["sigma_mean_nusselt_hot"] ** 2) ** 0.5
+        v["mean_diff_in_sigma"] = (abs(v["mean_nusselt_hot_2nd_half"]
+                                       - ref["mean_nusselt_hot_2nd_half"]) / comb
+                                   if comb > 0 else float("nan"))
     out["nodes"] = nodes
     return out
 
@@ -447,8 +554,10 @@ def main():
             print("[bench] adjoint form 96x384", file=sys.stderr, flush=True)
             extra["adjoint"] = adjoint_form()
 
-    precision = None
+    precision = gate = None
     if "uniform" in paths and a.precision_steps > 0:
+        print("[bench] precision gate (FP32 / FP16-offset vs FP64)", file=sys.stderr, flush=True)
+        gate = precision_gate()
         print(f"[bench] precision forms, {a.precision_steps} steps", file=sys.stderr, flush=True)
         precision = precision_run(a.precision_steps)
     copy = copy_bandwidth() if main_form is not None else None
@@ -474,6 +583,8 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": main_form["ms_per_step"],
+        "ms_per_step_wall": main_form["ms_per_step_wall"],
+        "value_wall": round(main_form["mlups_wall"], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(main_form["mlups"] / REFERENCE_CPU_MLUPS_256x1024, 2),
@@ -492,11 +603,14 @@ def main():
                      "traffic_bytes_per_lup": (round(traffic / (spl * nodes), 1)
                                                if traffic else None),
                      "launch_ms": main_form["launch_ms"], "steps_per_launch": spl,
-                     "timing": "HIP events around the timed steps on the solver stream"},
+                     "timing": "HIP events on the solver stream around each of 5 repetitions "
+                               "of the timed steps; median",
+                     "fp64_valu": fp64_issue_bound(spl, main_form["launch_ms"])},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
         "batched": batched,
+        "precision_gate": gate,
         "precision_forms": precision,
         "copy_bandwidth": copy,
         "other_paths": ({k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)

@@ -6,9 +6,9 @@ include/cavlbm.h); this package is its host-side mirror of the reference's
 ``CavitySolver`` API.  There is no CPU fallback.
 """
 from .solver import CavitySolver, DELTA_T, T_COLD, T_HOT  # noqa: F401
-from . import post, run, snapshot  # noqa: F401
+from . import run, snapshot  # noqa: F401
 from .batch import CavityBatch  # noqa: F401
 from .octree import OctreeCavitySolver  # noqa: F401
 
 __all__ = ["CavitySolver", "CavityBatch", "OctreeCavitySolver", "T_HOT", "T_COLD", "DELTA_T",
-           "post", "run", "snapshot"]
+           "run", "snapshot"]

@@ -644,7 +644,11 @@ int main(int argc, char** argv) {
                   (unsigned long long)snx, (unsigned long long)sny, a.nx, a.ny,
                   (snx == a.nx && sny == a.ny) ? "" : " (interpolation bilinéaire)");
       for (int q = 0; q < 4; ++q)
-        cavlbm_interpolate_field(f[q].data(), snx, sny, g[q].data(), a.nx, a.ny);
+        if (cavlbm_interpolate_field(f[q].data(), snx, sny, g[q].data(), a.nx, a.ny)) {
+          std::fprintf(stderr, "Erreur de lecture du snapshot %s : %s\n", a.restart.c_str(),
+                       cavlbm_snapshot_last_error());
+          return 6;
+        }
       sim.init_from_fields(g[0], g[1], g[2], g[3]);
     }
     const cavlbm_derived d = sim.derived();

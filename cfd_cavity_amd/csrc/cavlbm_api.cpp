@@ -66,6 +66,8 @@ struct cavlbm_solver {
   int fy_par = 0;  // swaps of fy / fy_alt so far, mod 2
   // dynamic-SGS steps, by starting buffer parity cur | (fy parity << 1)
   hipGraphExec_t dgraph[4] = {nullptr, nullptr, nullptr, nullptr};
+  // the same for steps with the dynamic model's kept values (model off)
+  hipGraphExec_t pgraph[4] = {nullptr, nullptr, nullptr, nullptr};
   bool graph_failed = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* avg = nullptr;  // TimeAverager sums (5 planes), allocated on first use
@@ -221,6 +223,11 @@ void drop_graphs(cavlbm_solver* s) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  for (auto& g : s->pgraph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 void swap_fy(cavlbm_solver* s) {
@@ -305,11 +312,19 @@ hipError_t dyn_fused_launches(cavlbm_solver* s, const StepCtx& ctx, const DynSgs
 // A step with the per-node SGS values (solver.rs:279-293 with
 // compute_dynamic_sgs first): the model reads the post-stream state and the
 // collision takes nu_t / alpha_t from it.  From the C-form the fused launches
-// above; from the M-form (after init or setters), or with the model off and
-// its last values kept, the model's three passes and the collision from the
-// M-form (MODE_FIRST_PRE).
+// above, or, with the model off and its last values kept, one fused step
+// whose collision reads them (MODE_FUSED_PRE); from the M-form (after init or
+// setters) the model's three passes and the collision from the M-form
+// (MODE_FIRST_PRE).
 int dyn_step(cavlbm_solver* s, int write_aux) {
   const StepCtx ctx = make_ctx(s, s->time_step);
+  if (!s->dyn_on && s->c_valid) {
+    HIP_TRY(s, launch(s, MODE_FUSED_PRE, ctx, write_aux));
+    s->cur ^= 1;
+    s->m_valid = false;
+    s->time_step += 1;
+    return 0;
+  }
   if (s->dyn_on && s->c_valid) {
     bool fy_swap;
     HIP_TRY(s, dyn_fused_launches(s, ctx, dyn_args(s, ctx), write_aux, fy_swap));
@@ -361,8 +376,11 @@ int block_step(cavlbm_solver* s, int n, int write_aux) {
 // Captures kGraphSteps fused steps from the current buffers (cur, fy): n
 // steps per launch (the run loop's steady choice), kGraphSteps / n launches,
 // an even count, so the replay ends on the buffers it started from.
+// With the dynamic model off and its values kept, the steps are single
+// MODE_FUSED_PRE launches.
 hipGraphExec_t capture(cavlbm_solver* s) {
   const StepCtx ctx = make_ctx(s, s->d.spin_up_steps);  // constant after spin-up
+  const int mode = s->dyn_pre ? MODE_FUSED_PRE : MODE_FUSED;
   const int saved = s->cur;
   void* const fy_saved = s->fy;
   void* const fy_alt_saved = s->fy_alt;
@@ -378,7 +396,7 @@ hipGraphExec_t capture(cavlbm_solver* s) {
       ok = launch_block(s, n, true, 0) == hipSuccess;
       swap_fy(s);
     } else {
-      ok = launch(s, MODE_FUSED, ctx, 0) == hipSuccess;
+      ok = launch(s, mode, ctx, 0) == hipSuccess;
     }
     s->cur ^= 1;
   }
@@ -455,10 +473,10 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
     }
     // Steady part: whole graphs of kGraphSteps fused steps (constant per-step
     // scalars after spin-up), keeping the final step direct so that it writes
-    // nu_t.
-    if (s->c_valid && !s->graph_failed && !s->dyn_on && !s->dyn_pre &&
+    // nu_t; with the dynamic model's kept values, graphs of their own.
+    if (s->c_valid && !s->graph_failed && !s->dyn_on &&
         s->time_step >= s->d.spin_up_steps && left > (uint64_t)kGraphSteps) {
-      hipGraphExec_t& g = s->graph[s->cur | (s->fy_par << 1)];
+      hipGraphExec_t& g = (s->dyn_pre ? s->pgraph : s->graph)[s->cur | (s->fy_par << 1)];
       if (!g) g = capture(s);
       if (!g) {
         s->graph_failed = true;

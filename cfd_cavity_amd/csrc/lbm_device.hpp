@@ -51,7 +51,9 @@ enum Mode {
   MODE_FIRST_PRE = 4,    // MODE_FIRST with per-node nu_t / alpha_t given (dynamic SGS)
   MODE_MATERIALIZE_F = 5,  // MODE_MATERIALIZE writing the f planes and fields only
   MODE_FUSED_DYN = 6,    // MODE_FUSED with nu_t / alpha_t from the model's coefficient stage
-  MODE_WINDOW = 7        // MODE_FUSED reading fy_in, with the Nusselt row term of the pulled state
+  MODE_WINDOW = 7,       // MODE_FUSED reading fy_in, with the Nusselt row term of the pulled state
+  MODE_FUSED_PRE = 8     // MODE_FUSED with per-node nu_t / alpha_t given (the dynamic model's
+                         // last values, kept after it is switched off)
 };
 
 struct Geom {
@@ -174,8 +176,8 @@ struct StepBufs {
   double* ux;
   double* uy;
   double* temp;
-  const double* nut_pre;  // MODE_FIRST_PRE: nu_t per node (FP64), else unused
-  const double* at_pre;   // MODE_FIRST_PRE: alpha_t per node (FP64), else unused
+  const double* nut_pre;  // MODE_FIRST_PRE / MODE_FUSED_PRE: nu_t per node (FP64), else unused
+  const double* at_pre;   // MODE_FIRST_PRE / MODE_FUSED_PRE: alpha_t per node (FP64), else unused
   DynCoef dc;             // MODE_FUSED_DYN: the model's coefficient stage
   // MODE_WINDOW (FP64): the previous force plane (may equal fy), and per row
   // iy the accumulated |T(1,iy) - T(2,iy)| and / or its sign, of the state
@@ -620,7 +622,7 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   R rho, ux, uy, temp;
 
   if (MODE == MODE_FUSED || MODE == MODE_MATERIALIZE || MODE == MODE_MATERIALIZE_F ||
-      MODE == MODE_FUSED_DYN || MODE == MODE_WINDOW) {
+      MODE == MODE_FUSED_DYN || MODE == MODE_WINDOW || MODE == MODE_FUSED_PRE) {
     const R fyp = static_cast<const R*>(MODE == MODE_WINDOW ? b.fy_in : b.fy)[k];
     pull_node<S, GEQ>(geo, walls, static_cast<const T*>(b.src), fyp, ix, iy, f, g, rho, ux, uy,
                       temp);
@@ -669,7 +671,7 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
   }
 
   R fy_new, nut_new;
-  if (MODE == MODE_FIRST_PRE) {
+  if (MODE == MODE_FIRST_PRE || MODE == MODE_FUSED_PRE) {
     collide_node<R, GEQ, true>(f, g, rho, ux, uy, temp, ctx, R(ctx.cs), fy_new, nut_new,
                                R(b.nut_pre[k]), R(b.at_pre[k]));
   } else if (MODE == MODE_FUSED_DYN) {

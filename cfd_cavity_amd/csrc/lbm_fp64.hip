@@ -27,6 +27,7 @@ hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& c
   CAVLBM_CASE(2, MODE_FIRST_PRE)
   CAVLBM_CASE(2, MODE_MATERIALIZE_F)
   CAVLBM_CASE(2, MODE_FUSED_DYN)
+  CAVLBM_CASE(2, MODE_FUSED_PRE)
   CAVLBM_CASE(2, MODE_WINDOW)
   CAVLBM_CASE(1, MODE_FUSED)
   CAVLBM_CASE(1, MODE_FIRST)
@@ -35,6 +36,7 @@ hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& c
   CAVLBM_CASE(1, MODE_FIRST_PRE)
   CAVLBM_CASE(1, MODE_MATERIALIZE_F)
   CAVLBM_CASE(1, MODE_FUSED_DYN)
+  CAVLBM_CASE(1, MODE_FUSED_PRE)
   CAVLBM_CASE(1, MODE_WINDOW)
 #undef CAVLBM_CASE
   return hipErrorInvalidValue;

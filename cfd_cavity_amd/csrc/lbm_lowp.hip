@@ -28,6 +28,7 @@ static hipError_t dispatch(int geq, int mode, const Geom& geo, const StepCtx& ct
   CAVLBM_CASE(2, MODE_FIRST_PRE)
   CAVLBM_CASE(2, MODE_MATERIALIZE_F)
   CAVLBM_CASE(2, MODE_FUSED_DYN)
+  CAVLBM_CASE(2, MODE_FUSED_PRE)
   CAVLBM_CASE(1, MODE_FUSED)
   CAVLBM_CASE(1, MODE_FIRST)
   CAVLBM_CASE(1, MODE_MATERIALIZE)
@@ -35,6 +36,7 @@ static hipError_t dispatch(int geq, int mode, const Geom& geo, const StepCtx& ct
   CAVLBM_CASE(1, MODE_FIRST_PRE)
   CAVLBM_CASE(1, MODE_MATERIALIZE_F)
   CAVLBM_CASE(1, MODE_FUSED_DYN)
+  CAVLBM_CASE(1, MODE_FUSED_PRE)
 #undef CAVLBM_CASE
   return hipErrorInvalidValue;
 }

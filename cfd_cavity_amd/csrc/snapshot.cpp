@@ -46,6 +46,18 @@ double get_f64(const unsigned char* p) {
   return x;
 }
 
+// Nodes of a grid read from a header and the bytes of its four f64 fields,
+// with `room` bytes left in a uint64 size; false for an empty grid or when
+// nx*ny or 32*nx*ny does not fit (a corrupt header).
+bool grid_payload(uint64_t nx, uint64_t ny, uint64_t room, uint64_t* n, uint64_t* bytes) {
+  if (nx == 0 || ny == 0) return false;
+  const uint64_t nn = nx * ny;
+  if (nn / nx != ny || nn > room / 32) return false;
+  *n = nn;
+  *bytes = 32 * nn;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -103,8 +115,10 @@ int cavlbm_snapshot_read(const char* path, uint64_t* nx, uint64_t* ny, double* r
   if (data.size() < 24 || std::memcmp(data.data(), kMagic, 8) != 0)
     return io_fail("snapshot: bad magic or file too short");
   const uint64_t sx = get_u64(data.data() + 8), sy = get_u64(data.data() + 16);
-  const uint64_t n = sx * sy;
-  const uint64_t expect = 24 + 4 * n * 8;
+  uint64_t n = 0, payload = 0;
+  if (!grid_payload(sx, sy, UINT64_MAX - 24, &n, &payload))
+    return io_fail("snapshot: invalid grid " + std::to_string(sx) + "x" + std::to_string(sy));
+  const uint64_t expect = 24 + payload;
   if (data.size() < expect)
     return io_fail("snapshot: expected " + std::to_string(expect) + " bytes, read " +
                    std::to_string(data.size()));
@@ -249,8 +263,11 @@ int cavlbm_octsnap_read(const char* path, uint32_t* levels, uint64_t* step, uint
   for (uint32_t g = 0; g < ng; ++g) {
     d[2 * g] = get_u64(data.data() + 28 + 16 * g);
     d[2 * g + 1] = get_u64(data.data() + 36 + 16 * g);
-    const uint64_t n = d[2 * g] * d[2 * g + 1];
-    expect += 4 * n * 8;
+    uint64_t n = 0, payload = 0;
+    if (!grid_payload(d[2 * g], d[2 * g + 1], UINT64_MAX - expect, &n, &payload))
+      return io_fail("snapshot octree : grille " + std::to_string(g) + " invalide " +
+                     std::to_string(d[2 * g]) + "x" + std::to_string(d[2 * g + 1]));
+    expect += payload;
     total += n;
   }
   if (data.size() != expect)

@@ -46,7 +46,9 @@ enum {
 enum {
   CAVLBM_STORAGE_FP64 = 0,
   CAVLBM_STORAGE_FP32 = 1,
-  CAVLBM_STORAGE_FP16_OFFSET = 2 /* f stored as half(f - w_i), g as half(g) */
+  CAVLBM_STORAGE_FP16_OFFSET = 2 /* f stored as half(f - w_i), g as half(g - w_i T_j) with
+                                    T_j the writing node's temperature, kept in one FP32
+                                    plane after the 18 half planes */
 };
 
 enum { CAVLBM_FIELD_RHO = 0, CAVLBM_FIELD_UX = 1, CAVLBM_FIELD_UY = 2,

@@ -150,11 +150,35 @@ def test_values_persist_after_switching_off(oracle):
     pre, stats = g.nu_t_pre, g.dynamic_stats()
     for s in (g, o):
         s.set_dynamic_sgs(False, 0.5, -0.1, 2.0)
-    g.run(70)   # long enough for the graph path, which must not be taken
-    o.run(70)
+    g.run(150)  # past the spin-up (200): fused steps, then graph replays of them
+    o.run(150)
     assert bits_equal(g.nu_t_pre, pre) and g.dynamic_stats() == stats
     assert bits_equal(g.get_field("nu_t"), pre)
     assert_state_close(g, o, 1e-10, "after switching off")
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+def test_kept_values_fused_steps_equal_mform_steps(storage):
+    """With the model switched off and its values kept, steps from the C-form
+    collide with them in the fused step (MODE_FUSED_PRE, graph-replayed after
+    the spin-up); a solver made to start every step from the M-form (a field
+    written back after each step) collides with them there (MODE_FIRST_PRE).
+    Both give bitwise the same state."""
+    a, b = (CavitySolver(*args_of(SMALL), 1, storage=storage) for _ in range(2))
+    for s in (a, b):
+        s.run(120)
+        s.set_dynamic_sgs(True, 0.5, -0.1, 2.0)
+        s.run(4)
+        s.set_dynamic_sgs(False, 0.5, -0.1, 2.0)
+    a.run(130)
+    for _ in range(130):
+        b.set_field("rho", b.get_field("rho"))
+        b.run(1)
+    assert a.time_step == b.time_step == 254
+    for name in FIELDS:
+        assert bits_equal(a.get_field(name), b.get_field(name)), name
+    for w in ("f", "g"):
+        assert bits_equal(a.get_pops(w), b.get_pops(w)), w
 
 
 def test_fp32_storage_follows_fp64():

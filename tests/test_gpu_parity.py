@@ -8,7 +8,7 @@ The reference tests of src/uniform/tests.rs are restated on the GPU path.
 import numpy as np
 import pytest
 
-from cfd_cavity_amd import CavitySolver, post
+from cfd_cavity_amd import CavitySolver
 from cfd_cavity_amd._lib import CavlbmError, E_SIZE, E_UNSUPPORTED
 from tests.helpers import SMALL, args_of, bits_equal, first_mismatch, rel
 
@@ -290,7 +290,7 @@ def test_nusselt_conduction_limit_near_az():       # tests.rs:481-506
     t = np.array([0.5 + (-0.5 - 0.5) * (ix / (nx - 1)) for iy in range(ny) for ix in range(nx)])
     s.set_field("temperature", t)
     az = (ny - 1) / (nx - 1)
-    assert abs(post.nusselt_hot(s) - az) < 0.05
+    assert abs(s.diagnostics()["nusselt_hot"] - az) < 0.05   # post.rs:68-78
 
 
 def test_thermal_wall_link_conductance(oracle):    # tests.rs:602-643

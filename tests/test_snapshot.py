@@ -81,3 +81,18 @@ def test_interpolate_bilinear_in_both_directions():
     x1, y1 = np.meshgrid(np.arange(nx1) / (nx1 - 1), np.arange(ny1) / (ny1 - 1))
     f1 = snapshot.interpolate_field(f0.ravel(), nx0, ny0, nx1, ny1).reshape(ny1, nx1)
     assert np.max(np.abs(f1 - (0.3 + 0.7 * x1 - 0.2 * y1 + 0.9 * x1 * y1))) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(0, 16), (16, 0), (1 << 32, 1 << 32), ((1 << 62) + 1, 4),
+                                   (1 << 60, 1)])
+def test_snapshot_corrupt_header_is_rejected(tmp_path, nx, ny):
+    """A header whose grid is empty, or whose nx*ny or 32*nx*ny bytes do not
+    fit in 64 bits (nx = ny = 2^32 wraps nx*ny to 0), is an I/O error, not an
+    empty or truncated read."""
+    import struct
+
+    path = tmp_path / "bad.bin"
+    path.write_bytes(b"CAVSNAP1" + struct.pack("<QQ", nx, ny) + b"\0" * 64)
+    with pytest.raises(CavlbmError) as e:
+        snapshot.read_snapshot(path)
+    assert e.value.code == E_IO
