@@ -124,6 +124,7 @@ SIGNATURES = {
     "cavlbm_interpolate_field": (C.c_int, [_dp, _u64, _u64, _dp, _u64, _u64]),
     "cavlbm_snapshot_last_error": (C.c_char_p, []),
     "cavlbm_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _dp]),
+    "cavlbm_nu_window_static": (C.c_int, [_vp, _dp, _u64, _d, _u64, _dp]),
     "cavlbm_adjoint_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _u64, _u64, _dp, _dp, _dp]),
     "cavlbm_tangent_nu_window": (C.c_int, [_vp, _dp, _u64, _d, _dp, _d, _u64, _dp, _dp]),
     "cavlbm_batch_create": (C.c_int, [C.POINTER(Params), _u64, C.POINTER(_vp)]),

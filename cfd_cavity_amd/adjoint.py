@@ -41,6 +41,18 @@ def nu_window(solver, theta, cs, window_steps):
     return j.value
 
 
+def nu_window_static(solver, theta, cs, window_steps):
+    """J as ``nu_window`` computes it, with the window's buffers in the
+    library's static device array instead of device allocations (lattices of
+    at most 65536 row-padded nodes and 1024 rows)."""
+    th = _theta(solver, theta)
+    j = C.c_double()
+    solver._check(solver._L.cavlbm_nu_window_static(
+        solver._h, None if th is None else _dp(th), solver.ny(), float(cs), int(window_steps),
+        C.byref(j)))
+    return j.value
+
+
 def adjoint_nu_window(solver, theta, cs, window_steps, checkpoint_every=100):
     """(J, dJ/dtheta (ny,), dJ/dCs) by the adjoint with checkpoint-and-recompute."""
     th = _theta(solver, theta)

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1669,6 +1670,19 @@ const char* cavlbm_batch_last_error(const cavlbm_batch* b) {
 // ===========================================================================
 // Window objective and its adjoint (FP64)
 // ===========================================================================
+
+// Device memory of the allocation-free window objective
+// (cavlbm_nu_window_static): the window's two states (18 planes each), two
+// force planes, the nu_t plane, the per-row Nusselt sums and the hot-wall
+// profile, for lattices whose pitched plane holds at most kStaticPlane nodes
+// and at most kStaticRows rows (20.5 MB per GPU, held until the process ends).
+// Host code never uses the name `scratch` as an address: each device's copy
+// is found with hipGetSymbolAddress (static_scratch below).
+constexpr size_t kStaticPlane = 65536;  // e.g. 128x512, 96x384, 256x256
+constexpr size_t kStaticRows = 1024;
+constexpr size_t kStaticDoubles = 39 * kStaticPlane + 2 * kStaticRows;
+static __device__ __align__(256) double scratch[kStaticDoubles];
+
 namespace {
 
 // Device work area of one window evaluation; freed on destruction.
@@ -1753,8 +1767,8 @@ struct AuxStreamJoin {
   ~AuxStreamJoin() { (void)hipStreamSynchronize(st); }
 };
 
-int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K,
-                 WindowWork& w, double*& theta) {
+// The checks every window entry point makes before touching the device.
+int window_check(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K) {
   if (s->storage != ST_FP64)
     return fail(s, CAVLBM_E_UNSUPPORTED, "window objective and adjoint need FP64 storage");
   if (s->dyn_on || s->dyn_pre)
@@ -1762,8 +1776,15 @@ int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_
                 "window objective and adjoint are for the fixed-Cs Smagorinsky model");
   if (K == 0) return fail(s, CAVLBM_E_PARAM, "window_steps must be >= 1");
   if (theta_hot && ny != s->p.ny) return fail(s, CAVLBM_E_SIZE, "theta_hot must have ny values");
+  return 0;
+}
+
+int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_t K,
+                 WindowWork& w, double*& theta) {
+  int rc = window_check(s, theta_hot, ny, K);
+  if (rc) return rc;
   HIP_TRY(s, hipSetDevice(s->p.device));
-  int rc = materialize(s);  // S_0 in M-form (the logical state is unchanged)
+  rc = materialize(s);  // S_0 in M-form (the logical state is unchanged)
   if (rc) return rc;
   theta = w.alloc<double>(s->p.ny);
   if (!theta) return fail(s, CAVLBM_E_HIP, "window allocation failed");
@@ -1774,17 +1795,27 @@ int window_setup(cavlbm_solver* s, const double* theta_hot, uint64_t ny, uint64_
   return 0;
 }
 
-// Forward window: J, and optionally the checkpoints (X_{qC}, fy_{qC}).
+// The device buffers of a forward window: X_m in ping-pong, the force plane
+// of X_m (the N-step launches read one plane and write the other), the nu_t
+// plane the steps write, and the per-row Nusselt sums (zero on entry).
+struct WindowBufs {
+  double* X[2];
+  double* fy[2];
+  double* nut;
+  double* acc;
+};
+
+// Forward window: J, and optionally the checkpoints (X_{qC}, fy_{qC}), whose
+// slots come from w (required when ckpt is given).  hrows: ny host doubles
+// receiving the row sums.
 int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K, uint64_t C,
-                   WindowWork& w, std::vector<double*>* ckpt, double* j_out) {
+                   const WindowBufs& wb, WindowWork* w, std::vector<double*>* ckpt,
+                   double* hrows, double* j_out) {
   const size_t P = (size_t)s->geo.plane;
-  double* X[2] = {w.alloc<double>(18 * P), w.alloc<double>(18 * P)};
-  // fy_m; the N-step launches read one plane and write the other
-  double* fy[2] = {w.alloc<double>(P), w.alloc<double>(P)};
-  double* nut = w.alloc<double>(P);
-  double* acc = w.zeroed<double>(s->p.ny);
-  if (!X[0] || !X[1] || !fy[0] || !fy[1] || !nut || !acc)
-    return fail(s, CAVLBM_E_HIP, "window allocation failed");
+  double* const* X = wb.X;
+  double* const* fy = wb.fy;
+  double* nut = wb.nut;
+  double* acc = wb.acc;
   const uint64_t t0 = s->time_step;
   const Walls walls{theta};
   int cur = 0, fc = 0;
@@ -1818,7 +1849,7 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
     }
     m += (uint64_t)n;
     if (ckpt && (m - 1) % C == 0) {  // X_{m-1}, the last state of this launch
-      double* slot = w.alloc<double>(19 * P);
+      double* slot = w->alloc<double>(19 * P);
       if (!slot) return fail(s, CAVLBM_E_HIP, "checkpoint allocation failed");
       HIP_TRY(s, hipMemcpyAsync(slot, X[cur], 18 * P * sizeof(double), hipMemcpyDeviceToDevice,
                                 s->st));
@@ -1829,14 +1860,51 @@ int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K,
   }
   // Nu(S_K) from X_{K-1}
   HIP_TRY(s, launch_nu_rows(s->geo, s->geq, X[cur], theta, acc, nullptr, s->st));
-  std::vector<double> h(s->p.ny);
-  HIP_TRY(s, hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            s->st));
+  HIP_TRY(s, hipMemcpyAsync(hrows, acc, s->p.ny * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(s, hipStreamSynchronize(s->st));
   double sum = 0.0;
-  for (double v : h) sum += v;
+  for (uint64_t iy = 0; iy < s->p.ny; ++iy) sum += hrows[iy];
   *j_out = sum / (double)s->p.ny * s->d.lz / 1.0 / (double)K;
   return 0;
+}
+
+// The address of `scratch` on the solver's device (current), looked up once
+// per device and kept.
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+int static_scratch(cavlbm_solver* s, double** out) {
+  const int d = s->p.device;
+  if (d < 0 || d >= kScratchDevices) return fail(s, CAVLBM_E_PARAM, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[d]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(s, CAVLBM_E_HIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[d] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[d];
+  return 0;
+}
+
+// The forward window on buffers from the solver's window arena.
+int window_forward(cavlbm_solver* s, const double* theta, double cs, uint64_t K, uint64_t C,
+                   WindowWork& w, std::vector<double*>* ckpt, double* j_out) {
+  const size_t P = (size_t)s->geo.plane;
+  WindowBufs wb{};
+  wb.X[0] = w.alloc<double>(18 * P);
+  wb.X[1] = w.alloc<double>(18 * P);
+  wb.fy[0] = w.alloc<double>(P);
+  wb.fy[1] = w.alloc<double>(P);
+  wb.nut = w.alloc<double>(P);
+  wb.acc = w.zeroed<double>(s->p.ny);
+  if (!wb.X[0] || !wb.X[1] || !wb.fy[0] || !wb.fy[1] || !wb.nut || !wb.acc)
+    return fail(s, CAVLBM_E_HIP, "window allocation failed");
+  std::vector<double> h(s->p.ny);
+  return window_forward(s, theta, cs, K, C, wb, &w, ckpt, h.data(), j_out);
 }
 
 }  // namespace
@@ -1851,6 +1919,46 @@ int cavlbm_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny, dou
   int rc = window_setup(s, theta_hot, ny, window_steps, w, theta);
   if (rc) return rc;
   return window_forward(s, theta, cs, window_steps, 1, w, nullptr, j_out);
+}
+
+int cavlbm_nu_window_static(cavlbm_solver* s, const double* theta_hot, uint64_t ny, double cs,
+                            uint64_t window_steps, double* j_out) {
+  if (!s || !j_out) return CAVLBM_E_PARAM;
+  int rc = window_check(s, theta_hot, ny, window_steps);
+  if (rc) return rc;
+  const size_t P = (size_t)s->geo.plane;
+  const size_t rows = s->p.ny;
+  if (P > kStaticPlane || rows > kStaticRows)
+    return fail(s, CAVLBM_E_SIZE,
+                "cavlbm_nu_window_static: the lattice's pitched plane (" + std::to_string(P) +
+                    " nodes, " + std::to_string(rows) + " rows) exceeds the static scratch (" +
+                    std::to_string(kStaticPlane) + " nodes, " + std::to_string(kStaticRows) +
+                    " rows); use cavlbm_nu_window");
+  HIP_TRY(s, hipSetDevice(s->p.device));
+  double* base = nullptr;
+  rc = static_scratch(s, &base);
+  if (rc) return rc;
+  rc = materialize(s);  // S_0 in M-form (the logical state is unchanged)
+  if (rc) return rc;
+  // P is a multiple of 64 (rows padded to 64 nodes), so every part stays
+  // 256-byte aligned
+  WindowBufs wb{};
+  wb.X[0] = base;
+  wb.X[1] = base + 18 * P;
+  wb.fy[0] = base + 36 * P;
+  wb.fy[1] = base + 37 * P;
+  wb.nut = base + 38 * P;
+  wb.acc = base + 39 * P;
+  double* theta = wb.acc + (rows + 31) / 32 * 32;
+  // the array keeps the previous call's contents: the row sums start at zero
+  HIP_TRY(s, launch_fill_f64(wb.acc, 0.0, rows, s->st));
+  if (theta_hot)
+    HIP_TRY(s, hipMemcpyAsync(theta, theta_hot, rows * sizeof(double), hipMemcpyHostToDevice,
+                              s->st));
+  else
+    HIP_TRY(s, launch_fill_f64(theta, 0.5, rows, s->st));
+  double hrows[kStaticRows];
+  return window_forward(s, theta, cs, window_steps, 1, wb, nullptr, nullptr, hrows, j_out);
 }
 
 int cavlbm_adjoint_nu_window(cavlbm_solver* s, const double* theta_hot, uint64_t ny, double cs,

@@ -272,9 +272,24 @@ __global__ __launch_bounds__(kBlock) void k_avg_acc(Geom geo, const double* ux, 
   sums[4 * P + k] += u * u + v * v;
 }
 
+// p[0..n) = v: clears or sets a small device buffer with a kernel, which
+// (unlike a memset node) stays correct inside captured graphs.
+__global__ __launch_bounds__(256) void k_fill_f64(double* p, double v, unsigned n) {
+  const unsigned i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 int diag_blocks(const Geom& geo) { return nblocks_of(geo); }
+
+hipError_t launch_fill_f64(double* p, double v, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (n > 0xffffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v,
+                     (unsigned)n);
+  return hipGetLastError();
+}
 
 hipError_t launch_avg_accumulate(const Geom& geo, int storage, const double* ux, const double* uy,
                                  const double* temp, const void* nut, double* sums,

@@ -56,6 +56,8 @@ hipError_t launch_phase_macro(const Geom& geo, int storage, const double* mpops,
                               const void* fy, double* rho, double* ux, double* uy,
                               double* temp, hipStream_t st);
 hipError_t launch_phase_walls(const Geom& geo, double* ux, double* uy, hipStream_t st);
+// p[0..n) = v (FP64), by a kernel
+hipError_t launch_fill_f64(double* p, double v, size_t n, hipStream_t st);
 hipError_t launch_phase_bounce_back(const Geom& geo, double* mpops, hipStream_t st);
 
 // Diagnostics: FP64 reductions over the M-form fields.  `partials` holds

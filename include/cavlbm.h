@@ -262,6 +262,19 @@ const char *cavlbm_snapshot_last_error(void);
  * the solver until cavlbm_destroy, so a repeated call allocates nothing. */
 int cavlbm_nu_window(cavlbm_solver *s, const double *theta_hot, uint64_t ny,
                      double cs, uint64_t window_steps, double *j_out);
+/* J exactly as cavlbm_nu_window computes it (bitwise the same value), with
+ * no device allocation: the window's buffers live in a static device array of
+ * the library (20.5 MB on each GPU it runs on, held until the process ends).
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (tight evaluation loops of an optimiser, harnesses that forbid
+ * hipMalloc).  Limit: the lattice's row-padded plane (ny * nx rounded up to
+ * a multiple of 64 per row) must hold at most 65536 nodes and ny <= 1024
+ * (e.g. 128x512, 96x384), else CAVLBM_E_SIZE.  Every call on a GPU uses that
+ * GPU's one array: calls that overlap on one GPU (from several threads or
+ * solvers at once) race and must not be made. */
+int cavlbm_nu_window_static(cavlbm_solver *s, const double *theta_hot, uint64_t ny,
+                            double cs, uint64_t window_steps, double *j_out);
+
 /* J and its gradient by the hand-written adjoint of the fused step, with
  * checkpoints every checkpoint_every steps and recomputation in between.
  * grad_theta has ny entries (rows 0 and ny-1 are exactly 0: the adiabatic
