@@ -208,6 +208,10 @@ constexpr int kRowBlock = kTileX * kRowTileY;
 // operands (a zero dividend may return +0 where division gives -0; every use
 // below adds that value to a nonzero term or to +0).  Five FMA-class
 // instructions replace the generic division sequence.
+// divc: for the constant divisors cs2, 2 cs2^2, 2 cs2 and cs2^2, whose
+// rounded reciprocals (3, 4.5, 1.5, 9) satisfy y*b = 1 - 2^-54 exactly, a*y
+// is within a relative 2^-54 of a/b and its rounding is already faithful, so
+// one correction gives the correctly rounded quotient (three instructions).
 //
 // FP32 (precision forms): reciprocal multiplication and fast division.
 template <class R> inline constexpr bool kLowp = sizeof(R) == 4;  // FP32 arithmetic
@@ -221,11 +225,17 @@ template <> struct Arith<double> {
     r = __builtin_fma(-b, q, a);
     return __builtin_fma(r, y, q);
   }
+  __device__ static __forceinline__ double divc(double a, double b, double y) {
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+  }
   __host__ __device__ static __forceinline__ double div(double a, double b) { return a / b; }
   __host__ __device__ static __forceinline__ double rcp(double b) { return 1.0 / b; }
 };
 template <> struct Arith<float> {
   __device__ static __forceinline__ float divr(float a, float, float y) { return a * y; }
+  __device__ static __forceinline__ float divc(float a, float, float y) { return a * y; }
   __device__ static __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
   __device__ static __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
 };
@@ -306,14 +316,14 @@ template <class R>
 __device__ __forceinline__ void eq_terms(R ux, R uy, EqTerms<R>& e) {
   using A = Arith<R>;
   const R u2 = ux * ux + uy * uy;
-  e.t = A::divr(u2, R(kDenU2), R(kInvDenU2));
+  e.t = A::divc(u2, R(kDenU2), R(kInvDenU2));
   const R eu[5] = {R(0.0), ux, ux + uy, uy, -ux + uy};
   e.q1[0] = R(0.0);
   e.q2[0] = R(0.0);
 #pragma unroll
   for (int i = 1; i < 5; ++i) {
-    const R q1 = A::divr(eu[i], R(kCs2), R(kInvCs2));
-    const R q2 = A::divr(eu[i] * eu[i], R(kDenEu2), R(kInvDenEu2));
+    const R q1 = A::divc(eu[i], R(kCs2), R(kInvCs2));
+    const R q2 = A::divc(eu[i] * eu[i], R(kDenEu2), R(kInvDenEu2));
     e.q1[i] = q1;
     e.q1[i + 4] = -q1;
     e.q2[i] = q2;
@@ -418,12 +428,12 @@ __device__ __forceinline__ void collide_relax(R (&f)[9], R (&g)[9], R ux, R uy, 
   // (cy - uy)/cs2 takes three values; (c.u) cy is +-eu of the directions
   // with cy = 1 (exact negation for cy = -1), and a signed zero for cy = 0,
   // where (0 - uy)/cs2 is never -0, so that term adds nothing.
-  const R a_p = A::divr(R(1.0) - uy, R(kCs2), R(kInvCs2));
-  const R a_0 = A::divr(R(0.0) - uy, R(kCs2), R(kInvCs2));
-  const R a_m = A::divr(R(-1.0) - uy, R(kCs2), R(kInvCs2));
-  const R b2 = A::divr(ux + uy, R(kDenCs4), R(kInvDenCs4));
-  const R b3 = A::divr(uy, R(kDenCs4), R(kInvDenCs4));
-  const R b4 = A::divr(-ux + uy, R(kDenCs4), R(kInvDenCs4));
+  const R a_p = A::divc(R(1.0) - uy, R(kCs2), R(kInvCs2));
+  const R a_0 = A::divc(R(0.0) - uy, R(kCs2), R(kInvCs2));
+  const R a_m = A::divc(R(-1.0) - uy, R(kCs2), R(kInvCs2));
+  const R b2 = A::divc(ux + uy, R(kDenCs4), R(kInvDenCs4));
+  const R b3 = A::divc(uy, R(kDenCs4), R(kInvDenCs4));
+  const R b4 = A::divc(-ux + uy, R(kDenCs4), R(kInvDenCs4));
   const R bracket[9] = {a_0, a_0, a_p + b2, a_p + b3, a_p + b4, a_0, a_m + b2, a_m + b3, a_m + b4};
   R s_i[9];
 #pragma unroll

@@ -37,6 +37,7 @@ __device__ __forceinline__ Dual fmax(Dual a, Dual b) { return a.v >= b.v ? a : b
 
 template <> struct Arith<Dual> {
   __device__ static __forceinline__ Dual divr(Dual a, Dual b, Dual) { return a / b; }
+  __device__ static __forceinline__ Dual divc(Dual a, Dual b, Dual) { return a / b; }
   __device__ static __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
   __device__ static __forceinline__ Dual rcp(Dual b) { return Dual(1.0) / b; }
 };
