@@ -213,6 +213,21 @@ constexpr int kRowBlock = kTileX * kRowTileY;
 // is within a relative 2^-54 of a/b and its rounding is already faithful, so
 // one correction gives the correctly rounded quotient (three instructions).
 //
+// div, rcp, sqrtp (device): the generic FP64 division and square root
+// sequences without their operand scaling.  The division is v_rcp_f64, two
+// Newton steps, the quotient and one correction: the generic sequence with
+// the v_div_scale operands left unscaled, v_div_fmas as a plain fma and no
+// v_div_fixup; for a numerator that is +0 or of magnitude in [2^-900, 2^900]
+// and a denominator in that range (no scaling condition holds and no
+// intermediate leaves the normal range) it returns the same correctly
+// rounded quotient.  The square root is v_rsq_f64 and the generic Goldschmidt
+// / Newton refinement without the 2^256 scaling of inputs below 2^-767, with
+// the +0 / infinity select kept.  Every operand they receive lies in these
+// ranges: reciprocals of relaxation times (> 0.5) and of densities above
+// f64::EPSILON, 3/16 over tau+ - 0.5 > 0, and |S|'s square root of a sum of
+// squares (+0 or above 1e-40) over a positive denominator.  The host keeps
+// IEEE division (relax_factors is evaluated on both sides).
+//
 // FP32 (precision forms): reciprocal multiplication and fast division.
 template <class R> inline constexpr bool kLowp = sizeof(R) == 4;  // FP32 arithmetic
 
@@ -230,14 +245,53 @@ template <> struct Arith<double> {
     const double r = __builtin_fma(-b, q, a);
     return __builtin_fma(r, y, q);
   }
-  __host__ __device__ static __forceinline__ double div(double a, double b) { return a / b; }
-  __host__ __device__ static __forceinline__ double rcp(double b) { return 1.0 / b; }
+  __host__ __device__ static __forceinline__ double div(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+#else
+    return a / b;
+#endif
+  }
+  __host__ __device__ static __forceinline__ double rcp(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double r = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(r, y, y);
+#else
+    return 1.0 / b;
+#endif
+  }
+  __device__ static __forceinline__ double sqrtp(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return (x == 0.0 || __builtin_isinf(x)) ? x : g;
+  }
 };
 template <> struct Arith<float> {
   __device__ static __forceinline__ float divr(float a, float, float y) { return a * y; }
   __device__ static __forceinline__ float divc(float a, float, float y) { return a * y; }
   __device__ static __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
   __device__ static __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+  __device__ static __forceinline__ float sqrtp(float x) { return sqrtf(x); }
 };
 
 // ---------------------------------------------------------------------------
@@ -477,7 +531,7 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R pi_xy = R(0.0) + d_f[2] - d_f[4] + d_f[6] - d_f[8];
   const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
   const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
-  const R s_mag = A::div(sqrt(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
+  const R s_mag = A::div(A::sqrtp(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
   const R nu_t = PRE ? nut_pre : cs * cs * s_mag;
   nut_out = nu_t;
 

@@ -40,6 +40,7 @@ template <> struct Arith<Dual> {
   __device__ static __forceinline__ Dual divc(Dual a, Dual b, Dual) { return a / b; }
   __device__ static __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
   __device__ static __forceinline__ Dual rcp(Dual b) { return Dual(1.0) / b; }
+  __device__ static __forceinline__ Dual sqrtp(Dual x) { return sqrt(x); }
 };
 
 namespace {
