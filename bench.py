@@ -9,9 +9,10 @@ in HBM.  The run loop advances several steps per kernel launch (temporal
 blocking, bitwise equal to single steps: 4 in FP64, 3 in FP32 and FP16-offset); the
 "fp64_2step" and "fp64_1step" forms time two steps per launch and the
 single-step kernel beside it.  The timed region (exactly --steps steps after
---warmup untimed ones) runs five times; ms_per_step and value come from HIP
-events on the solver stream (median of the five), the host wall clock around
-the synchronisations is reported beside them.  Prints ONE JSON line.
+--warmup untimed ones) runs eleven times; ms_per_step and value come from HIP
+events on the solver stream (median of the eleven: a 20-step region lasts
+about 0.2 ms and single regions vary by a few percent), the host wall clock
+around the synchronisations is reported beside them.  Prints ONE JSON line.
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
 rayon path) timed on this host's cores on a bounded sample of the same
@@ -56,7 +57,7 @@ def make_solver(storage: str, tblock=None):
     return s
 
 
-def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int = 5):
+def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int = 11):
     """One storage form with the library's default steps per launch; a name
     ending in "_<n>step" runs the same storage with at most n steps per
     launch (n = 1: the single-step kernel), for comparison.
@@ -603,7 +604,7 @@ def main():
                      "traffic_bytes_per_lup": (round(traffic / (spl * nodes), 1)
                                                if traffic else None),
                      "launch_ms": main_form["launch_ms"], "steps_per_launch": spl,
-                     "timing": "HIP events on the solver stream around each of 5 repetitions "
+                     "timing": "HIP events on the solver stream around each of 11 repetitions "
                                "of the timed steps; median",
                      "fp64_valu": fp64_issue_bound(spl, main_form["launch_ms"])},
         "cpu_baseline": cpu,
