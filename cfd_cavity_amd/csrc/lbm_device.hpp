@@ -95,10 +95,36 @@ struct DynCoef {
   double nu_t_min, alpha_t_min;
 };
 
-// Window [i - half_left, i + width - half_left) clamped to [0, n).
+// RN(1/b) for 2^-900 <= |b| <= 2^900: the generic FP64 reciprocal sequence
+// (v_rcp_f64, two Newton steps, one correction) without its operand scaling,
+// which no such b needs (see Arith below).
+__device__ __forceinline__ double rcp_f64(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  const double r = __builtin_fma(-b, y, 1.0);
+  return __builtin_fma(r, y, y);
+}
+
+// a / b correctly rounded from y = RN(1/b): q = a*y and two Markstein
+// corrections (see Arith below).
+__device__ __forceinline__ double quot_f64(double a, double b, double y) {
+  double q = a * y;
+  double r = __builtin_fma(-b, q, a);
+  q = __builtin_fma(r, y, q);
+  r = __builtin_fma(-b, q, a);
+  return __builtin_fma(r, y, q);
+}
+
+// Window [i - half_left, i + width - half_left) clamped to [0, n).  mean(s):
+// s / cnt, the correctly rounded quotient from the window's reciprocal (the
+// box sums start at +0, so no dividend is -0).
 struct DynWin {
   int x0, x1, y0, y1;
-  double cnt;
+  double cnt, rcnt;
+  __device__ __forceinline__ double mean(double s) const { return quot_f64(s, cnt, rcnt); }
 };
 
 __device__ __forceinline__ DynWin dyn_window(const Geom& geo, int width, int half_left, int ix,
@@ -109,6 +135,7 @@ __device__ __forceinline__ DynWin dyn_window(const Geom& geo, int width, int hal
   w.y0 = iy >= half_left ? iy - half_left : 0;
   w.y1 = min(iy + width - half_left, geo.ny);
   w.cnt = (double)((w.y1 - w.y0) * (w.x1 - w.x0));
+  w.rcnt = rcp_f64(w.cnt);
   return w;
 }
 
@@ -117,7 +144,7 @@ __device__ __forceinline__ double dyn_box(const Geom& geo, const DynWin& w, cons
   double s = 0.0;
   for (int yy = w.y0; yy < w.y1; ++yy)
     for (int xx = w.x0; xx < w.x1; ++xx) s += f[yy * geo.pitch + xx];
-  return s / w.cnt;
+  return w.mean(s);
 }
 
 // lilly_coefficient (solver.rs:1251-1261).
@@ -234,11 +261,7 @@ template <class R> inline constexpr bool kLowp = sizeof(R) == 4;  // FP32 arithm
 template <class R> struct Arith;
 template <> struct Arith<double> {
   __device__ static __forceinline__ double divr(double a, double b, double y) {
-    double q = a * y;
-    double r = __builtin_fma(-b, q, a);
-    q = __builtin_fma(r, y, q);
-    r = __builtin_fma(-b, q, a);
-    return __builtin_fma(r, y, q);
+    return quot_f64(a, b, y);
   }
   __device__ static __forceinline__ double divc(double a, double b, double y) {
     const double q = a * y;
@@ -261,13 +284,7 @@ template <> struct Arith<double> {
   }
   __host__ __device__ static __forceinline__ double rcp(double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    double y = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    const double r = __builtin_fma(-b, y, 1.0);
-    return __builtin_fma(r, y, y);
+    return rcp_f64(b);
 #else
     return 1.0 / b;
 #endif

@@ -72,7 +72,7 @@ __device__ __forceinline__ double box_mean(const DynWin& w, const Acc& f) {
   double s = 0.0;
   for (int yy = w.y0; yy < w.y1; ++yy)
     for (int xx = w.x0; xx < w.x1; ++xx) s += f(xx, yy);
-  return s / w.cnt;
+  return w.mean(s);
 }
 
 // Centred differences with one-sided clamping at the walls (solver.rs:1229-1242).
@@ -181,7 +181,9 @@ __device__ __forceinline__ DynTerms dyn_terms_node(const DynStrain& sh, const Dy
   const double lxx = q.lb[0] - ubx * ubx;
   const double lxy = q.lb[1] - ubx * uby;
   const double lyy = q.lb[2] - uby * uby;
-  const double lkk3 = (lxx + lyy) / 3.0;
+  // (lxx + lyy) / 3: 3 RN(1/3) = 1 - 2^-54, so one correction is exact; the
+  // dividend is never -0 (each term is a +0-started box mean minus a square)
+  const double lkk3 = Arith<double>::divc(lxx + lyy, 3.0, 1.0 / 3.0);
   const double ldxx = lxx - lkk3;
   const double ldyy = lyy - lkk3;
   const double ljx = q.lb[3] - ubx * tb;
@@ -282,15 +284,15 @@ __global__ __launch_bounds__(kBlock) void k_dyn_filter(DynSgsPair p) {
       s_ut += u * t;
       s_vt += v * t;
     }
-  const double ubx = s_u / w.cnt, uby = s_v / w.cnt, tb = s_t / w.cnt;
+  const double ubx = w.mean(s_u), uby = w.mean(s_v), tb = w.mean(s_t);
   a.ubx[k] = ubx;
   a.uby[k] = uby;
   a.tb[k] = tb;
-  a.lb[0 * geo.plane + k] = s_uu / w.cnt;
-  a.lb[1 * geo.plane + k] = s_uv / w.cnt;
-  a.lb[2 * geo.plane + k] = s_vv / w.cnt;
-  a.lb[3 * geo.plane + k] = s_ut / w.cnt;
-  a.lb[4 * geo.plane + k] = s_vt / w.cnt;
+  a.lb[0 * geo.plane + k] = w.mean(s_uu);
+  a.lb[1 * geo.plane + k] = w.mean(s_uv);
+  a.lb[2 * geo.plane + k] = w.mean(s_vv);
+  a.lb[3 * geo.plane + k] = w.mean(s_ut);
+  a.lb[4 * geo.plane + k] = w.mean(s_vt);
   dyn_similarity(a.ux[k], a.uy[k], a.temp[k], ubx, uby, tb, a.qsm[k], a.phi[k]);
 }
 
@@ -440,7 +442,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
         s_v += rv(xx, yy);
         s_t += rt(xx, yy);
       }
-    const double ubx = s_u / w.cnt, uby = s_v / w.cnt, tb = s_t / w.cnt;
+    const double ubx = w.mean(s_u), uby = w.mean(s_v), tb = w.mean(s_t);
     sh[DT::kOffF + 0 * DT::kF + idx] = ubx;
     sh[DT::kOffF + 1 * DT::kF + idx] = uby;
     sh[DT::kOffF + 2 * DT::kF + idx] = tb;
@@ -477,11 +479,11 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
         s_ut += u * tt;
         s_vt += v * tt;
       }
-    q.lb[0] = s_uu / w.cnt;
-    q.lb[1] = s_uv / w.cnt;
-    q.lb[2] = s_vv / w.cnt;
-    q.lb[3] = s_ut / w.cnt;
-    q.lb[4] = s_vt / w.cnt;
+    q.lb[0] = w.mean(s_uu);
+    q.lb[1] = w.mean(s_uv);
+    q.lb[2] = w.mean(s_vv);
+    q.lb[3] = w.mean(s_ut);
+    q.lb[4] = w.mean(s_vt);
     DynStrain s;
     s.xx = sh[DT::kOffS + 0 * DT::kT + idx];
     s.xy = sh[DT::kOffS + 1 * DT::kT + idx];

@@ -2,7 +2,8 @@
 (r = fma(-b, q, a); q = fma(r, y, q)) with y = RN(1/b).  This checks, on the
 CPU with the same algorithm (C, fma from libm), that the result is bitwise
 equal to IEEE division for the kernels' constant divisors, for divisors near
-the densities and relaxation times they meet, and for random normal operands.
+the densities and relaxation times they meet, for the window counts 1..64 of
+the dynamic model's box means, and for random normal operands.
 Plain multiplication by the reciprocal is shown to differ, so the test has
 teeth."""
 import math
@@ -30,11 +31,12 @@ int main(int argc, char** argv){
   for (long i = 0; i < n; ++i) {
     double a = (u01()*2.0 - 1.0) * ldexp(1.0, (int)(nx() % 80) - 60);
     double b;
-    int m = (int)(i % 7);
+    int m = (int)(i % 8);
     if (m < 4) b = dens[m];
     else if (m == 4) b = 0.9 + 0.2*u01();
     else if (m == 5) b = 0.5 + 40.0*u01();
-    else b = (0.5 + u01()) * ldexp(1.0, (int)(nx() % 200) - 100);
+    else if (m == 6) b = (0.5 + u01()) * ldexp(1.0, (int)(nx() % 200) - 100);
+    else b = (double)(1 + nx() % 64);  /* box-filter window counts */
     double y = 1.0 / b;
     double q = a / b;
     if (!same(div2(a,b,y), q)) ++bad2;
@@ -64,9 +66,10 @@ def test_two_step_fma_division_is_ieee(prog):
 
 
 # ---------------------------------------------------------------------------
-# One correction for the constant divisors cs2, 2 cs2^2, 2 cs2, cs2^2.
+# One correction for the constant divisors cs2, 2 cs2^2, 2 cs2, cs2^2 and 3
+# (the dynamic model's trace).
 #
-# For these b the rounded reciprocal y = RN(1/b) is 3, 4.5, 1.5, 9 and
+# For these b the rounded reciprocal y = RN(1/b) is 3, 4.5, 1.5, 9, RN(1/3) and
 # y*b = 1 - 2^-54 exactly, so a*y = (a/b)(1 - 2^-54) lies within a relative
 # 2^-54 below a/b.  Then q0 = RN(a*y) is a faithful quotient: either a*y is
 # already within the two floating-point neighbours of a/b, or it lies below
@@ -92,9 +95,9 @@ static int same(double x, double y){ uint64_t a,b; memcpy(&a,&x,8); memcpy(&b,&y
 int main(int argc, char** argv){
   long n = atol(argv[1]);
   const double cs2 = 1.0/3.0;
-  const double dens[4] = {cs2, 2.0*cs2*cs2, 2.0*cs2, cs2*cs2};
+  const double dens[5] = {cs2, 2.0*cs2*cs2, 2.0*cs2, cs2*cs2, 3.0};
   long bad = 0, bad0 = 0, cases = 0;
-  for (int d = 0; d < 4; ++d) {
+  for (int d = 0; d < 5; ++d) {
     const double b = dens[d], y = 1.0 / b;
     for (long i = 0; i < n; ++i) {
       double a;
@@ -130,7 +133,7 @@ def test_constant_divisors_reciprocal_error_is_2_pow_minus_54():
     from fractions import Fraction as Fr
 
     cs2 = 1.0 / 3.0
-    for b in (cs2, 2.0 * cs2 * cs2, 2.0 * cs2, cs2 * cs2):
+    for b in (cs2, 2.0 * cs2 * cs2, 2.0 * cs2, cs2 * cs2, 3.0):
         y = 1.0 / b
         # y = RN(1/b): within half an ulp of 1/b (ulp of y, y not a power of 2)
         half_ulp = Fr(abs(math.nextafter(y, math.inf) - y)) / 2
@@ -145,6 +148,6 @@ def test_one_correction_division_is_ieee_for_the_constant_divisors(tmp_path):
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", str(c), "-o", str(exe), "-lm"], check=True)
     out = subprocess.run([str(exe), "5000000"], capture_output=True, text=True, check=True).stdout
     bad, bad0, cases = map(int, out.split())
-    assert cases > 19_000_000
+    assert cases > 24_000_000
     assert bad == 0
     assert bad0 > 100_000  # a*y alone is not IEEE division
