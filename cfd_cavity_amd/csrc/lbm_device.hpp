@@ -255,7 +255,10 @@ constexpr int kRowBlock = kTileX * kRowTileY;
 // squares (+0 or above 1e-40) over a positive denominator.  The host keeps
 // IEEE division (relax_factors is evaluated on both sides).
 //
-// FP32 (precision forms): reciprocal multiplication and fast division.
+// FP32 (precision forms): reciprocal multiplication, division through the
+// hardware reciprocal and the hardware square root (about 1 ulp each); every
+// FP32-arithmetic kernel shares these, so blocked, single-step and batched
+// steps still round alike.
 template <class R> inline constexpr bool kLowp = sizeof(R) == 4;  // FP32 arithmetic
 
 template <class R> struct Arith;
@@ -306,9 +309,11 @@ template <> struct Arith<double> {
 template <> struct Arith<float> {
   __device__ static __forceinline__ float divr(float a, float, float y) { return a * y; }
   __device__ static __forceinline__ float divc(float a, float, float y) { return a * y; }
-  __device__ static __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
+  __device__ static __forceinline__ float div(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
   __device__ static __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
-  __device__ static __forceinline__ float sqrtp(float x) { return sqrtf(x); }
+  __device__ static __forceinline__ float sqrtp(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
 
 // ---------------------------------------------------------------------------
