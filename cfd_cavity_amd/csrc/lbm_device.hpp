@@ -244,16 +244,24 @@ constexpr int kRowBlock = kTileX * kRowTileY;
 // sequences without their operand scaling.  The division is v_rcp_f64, two
 // Newton steps, the quotient and one correction: the generic sequence with
 // the v_div_scale operands left unscaled, v_div_fmas as a plain fma and no
-// v_div_fixup; for a numerator that is +0 or of magnitude in [2^-900, 2^900]
-// and a denominator in that range (no scaling condition holds and no
-// intermediate leaves the normal range) it returns the same correctly
-// rounded quotient.  The square root is v_rsq_f64 and the generic Goldschmidt
-// / Newton refinement without the 2^256 scaling of inputs below 2^-767, with
-// the +0 / infinity select kept.  Every operand they receive lies in these
-// ranges: reciprocals of relaxation times (> 0.5) and of densities above
-// f64::EPSILON, 3/16 over tau+ - 0.5 > 0, and |S|'s square root of a sum of
-// squares (+0 or above 1e-40) over a positive denominator.  The host keeps
-// IEEE division (relax_factors is evaluated on both sides).
+// v_div_fixup; when the numerator is +0 or of magnitude in [2^-900, 2^900]
+// and the denominator and the quotient are in that range, no scaling
+// condition holds and no intermediate leaves the normal range, so it returns
+// what the generic sequence returns.  The square root is v_rsq_f64 and the
+// generic Goldschmidt / Newton refinement without the 2^256 scaling of inputs
+// below 2^-767, with the +0 / infinity select kept.  Every operand they
+// receive lies in these ranges: reciprocals of relaxation times (> 0.5) and of
+// densities above f64::EPSILON, 3/16 over tau+ - 0.5 > 0, and |S|'s square
+// root of a sum of squares (+0 or above 1e-40) over a positive denominator.
+// Refined from any first approximation within 2^-20 these equal IEEE
+// division, reciprocal and sqrt on random operands and on quotients next to a
+// rounding midpoint; two cases depend on the hardware's first approximation
+// instead: divisors with an all-ones significand, 2^k (2 - 2^-52), need it to
+// be RN(1/b), and square roots within about 2^-100 relative of a midpoint
+// need it within about 2^-24.  v_rcp_f64 returns RN(1/b) for those divisors
+// and both approximations are within 2^-24.2, and the helpers themselves
+// equal IEEE results on such operands on the GPU.  The host keeps IEEE
+// division (relax_factors is evaluated on both sides).
 //
 // FP32 (precision forms): reciprocal multiplication, division through the
 // hardware reciprocal and the hardware square root (about 1 ulp each); every
