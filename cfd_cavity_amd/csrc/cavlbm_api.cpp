@@ -748,6 +748,7 @@ int cavlbm_destroy(cavlbm_solver* s) {
   if (!s) return 0;
   (void)hipSetDevice(s->p.device);
   if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->st2) (void)hipStreamSynchronize(s->st2);
   free_all(s);
   delete s;
   return 0;
