@@ -124,9 +124,12 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int
     return res
 
 
-def batched_form(steps: int, warmup: int, storage: str = "fp32"):
+def batched_form(steps: int, warmup: int, storage: str = "fp32", total_steps: int = 50000):
     """One launch per step for all cavities (graph replays), autotuned, against
-    the same cavities stepped one after another with the single-cavity path."""
+    the same cavities stepped one after another with the single-cavity path.
+    After the timed steps the batch runs on to total_steps (the workload's
+    specified length) and reports how many cavities stay finite, the largest
+    max |u| and the largest relative mass drift."""
     from cfd_cavity_amd import CavityBatch, CavitySolver
     from cfd_cavity_amd.batch import seeded_batch_specs
 
@@ -138,6 +141,21 @@ def batched_form(steps: int, warmup: int, storage: str = "fp32"):
     ms = b.time_run(steps)
     mlups = nodes * steps / (ms * 1e-3) / 1e6
     finite = sum(1 for c in b if c.diagnostics()["all_finite"])
+    run_on = {}
+    if total_steps > 0:
+        done = b[0].time_step
+        if total_steps > done:
+            t0 = time.perf_counter()
+            b.run(total_steps - done)
+            wall = time.perf_counter() - t0
+        else:
+            wall = 0.0
+        diags = [(c.diagnostics(), sp["nx"] * sp["ny"]) for c, sp in zip(b, specs)]
+        run_on = {"total_steps": diags[0][0]["time_step"],
+                  "finite_cavities": sum(1 for d, _ in diags if d["all_finite"]),
+                  "max_u": max(d["max_u"] for d, _ in diags),
+                  "max_rel_mass_drift": max(abs(d["total_mass"] - n) / n for d, n in diags),
+                  "run_on_wall_s": round(wall, 3)}
     b.close()
     seq_ms = 0.0
     for sp in specs:
@@ -150,7 +168,7 @@ def batched_form(steps: int, warmup: int, storage: str = "fp32"):
     return {"cavities": len(specs), "nodes": nodes, "storage": storage, "steps": steps,
             "mlups": mlups, "ms_per_step": ms / steps, "sequential_mlups": seq_mlups,
             "speedup_vs_sequential": mlups / seq_mlups, "tuned": tune,
-            "finite_cavities": finite,
+            "finite_cavities": finite, "run_on": run_on,
             # MLUPS x the one-step algorithm's bytes per update over the HBM
             # peak: an effective rate, which exceeds 1 once several steps
             # run per launch (the state then crosses HBM once per N updates)
@@ -492,6 +510,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-total-steps", type=int, default=50000,
+                    help="run the batch on to this many steps after timing (0: no)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the dynamic-SGS and octree measurements")
@@ -533,7 +553,7 @@ def main():
     batched = None
     if not a.no_batch:
         print(f"[bench] batched: 64 cavities, {a.batch_steps} steps", file=sys.stderr, flush=True)
-        batched = batched_form(a.batch_steps, a.warmup)
+        batched = batched_form(a.batch_steps, a.warmup, total_steps=a.batch_total_steps)
 
     extra = None
     if not a.no_extra:
