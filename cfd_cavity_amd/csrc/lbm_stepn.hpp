@@ -36,7 +36,9 @@ struct StepCtxN {
 // first sub-step on (kTX + 2(N-1)) x (kTY + 2(N-1)) nodes, kNT threads.
 // Measured at 256x1024 (shapes compared within one run):
 //   FP64: N=2 32x12 (against 32x16, 32x10, 24x16, 64x4, 32x6, 16x14, 32x4),
-//         N=3 16x16 (against 16x12, 16x20, 32x8, 32x10, 32x12, 64x4),
+//         N=3 16x16 (against 16x12, 16x20, 32x8, 32x10, 32x12, 64x4, and
+//         16x18, 16x19, 16x21, whose regions fill whole waves better but
+//         which measured 5-12 % slower),
 //         N=4 16x16 (against 16x12, 32x8) -- the fastest, 16 % below N=2;
 //   FP32: N=2 64x8 (against 32x12..32x28, 64x12), N=3 32x16 (against 32x12,
 //         64x8, 16x16) -- the fastest, 15 % below N=2 -- and N=4 16x16
