@@ -19,6 +19,11 @@
  *   - collide_node: Guo split pinned by the transient-Poiseuille closed form;
  *     moments of the source term and a one-step equilibrium collision pinned
  *     in closed form; full step pinned by an independent numpy textbook step.
+ *   - xorshift64* and the initial noise: the state update pinned as a GF(2)
+ *     matrix of full period 2^64 - 1 equal to the published shift triple,
+ *     the multiplier recovered from outputs, the noise rebuilt by a matrix
+ *     generator; the nu / alpha / beta spin-up ramps pinned by their closed
+ *     forms through orc_step_scalars (tests/test_oracle_rng_ramp.py).
  *   - operand order inside boussinesq_force / compute_turbulent_viscosity /
  *     y_plus and the D2Q9 direction order: PARITY UNPINNED against the crate
  *     (values pinned, round-off order chosen here).
