@@ -8,6 +8,9 @@ pin the oracle against regressions and give the GPU tests fixed vectors:
   digests.json          sha256 of the FP64 state bytes after N steps, plus the
                         diagnostics at that step, for
                         - 64x256 Ra=1e6 and 128x512 Ra=1e8 after 1000 steps;
+                        - 64x256 Ra=1e6 after 20 000 steps with the
+                          diagnostics logged every 1000 steps (the
+                          reference's own CPU configuration);
                         - 128x512 Ra=1e8 after 100 000 steps (the GPU drop-in
                           configuration's last checkpoint);
                         - 256x1024 Ra=1e10, utarget 0.005, after 10 000 steps
@@ -42,6 +45,10 @@ LONG_CASES = {
     "128x512_ra1e8_100000": ((128, 512, 1.0e8, 0.71, 0.1, 0.9, 0.02, 42), 100000),
     "256x1024_ra1e10_10000": ((256, 1024, 1.0e10, 0.71, 0.1, 0.9, 0.005, 42), 10000),
 }
+# name: (args, steps, log interval)
+LOG_CASES = {
+    "64x256_ra1e6_20000_log1000": ((64, 256, 1.0e6, 0.71, 0.1, 0.9, 0.02, 42), 20000, 1000),
+}
 BATCH_STEPS = 300
 BATCH_KEY = "batch64_fp64_300"
 JOBS = int(os.environ.get("GOLDEN_JOBS", os.cpu_count() or 8))
@@ -69,6 +76,19 @@ def run_case(args, steps):
             "diagnostics": {k: v for k, v in d.items()}}
 
 
+def run_log_case(args, steps, every):
+    t0 = time.time()
+    o = pyoracle.OracleSolver(*args, JOBS)
+    log = []
+    while o.time_step < steps:
+        o.run(every)
+        log.append({"step": o.time_step, **o.diagnostics()})
+    print(f"  {args[0]}x{args[1]} x {steps}, log every {every}: {time.time() - t0:.1f} s",
+          flush=True)
+    return {"args": list(args), "steps": steps, "every": every, "sha256": state_digest(o),
+            "log": log}
+
+
 def main(argv):
     path = os.path.join(HERE, "digests.json")
     digests = {}
@@ -84,6 +104,9 @@ def main(argv):
         todo.update(LONG_CASES)
     for name, (args, steps) in todo.items():
         digests[name] = run_case(args, steps)
+    if "--long" in argv:
+        for name, (args, steps, every) in LOG_CASES.items():
+            digests[name] = run_log_case(args, steps, every)
     if "--batch" in argv:
         from cfd_cavity_amd.batch import seeded_batch_specs
 

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.golden.make_golden import CASES, FIELDS, LONG_CASES, state_digest
+from tests.golden.make_golden import CASES, FIELDS, LOG_CASES, LONG_CASES, state_digest
 from tests.helpers import bits_equal
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -38,6 +38,24 @@ def test_oracle_reproduces_digests(oracle):
     o = oracle.OracleSolver(*args, 8)
     o.run(steps)
     assert state_digest(o) == dig[name]["sha256"]
+
+
+LOG_KEYS = ("nusselt_hot", "nusselt_cold", "total_mass", "total_f_mass", "y_plus_hot", "max_u")
+
+
+def test_oracle_reproduces_log_start(oracle):
+    """The logged 64x256 Ra=1e6 run: the oracle reproduces its first three
+    log points exactly (the whole run is checked on the GPU)."""
+    name = "64x256_ra1e6_20000_log1000"
+    rec = load_digests()[name]
+    args, steps, every = LOG_CASES[name]
+    o = oracle.OracleSolver(*args, 8)
+    for ref in rec["log"][:3]:
+        o.run(every)
+        d = o.diagnostics()
+        assert d["time_step"] == ref["step"]
+        for k in LOG_KEYS:
+            assert d[k] == ref[k], (ref["step"], k)
 
 
 @pytest.mark.gpu
@@ -87,3 +105,25 @@ def test_gpu_matches_digests(name):
     assert s.time_step == steps == dig["steps"]
     assert device_digest(s) == dig["sha256"]
     assert_diagnostics_close(s.diagnostics(), dig["diagnostics"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(LOG_CASES))
+def test_gpu_matches_oracle_log(name):
+    """The reference's own CPU configuration, 64x256 Ra=1e6 for 20 000 steps:
+    the diagnostics logged every 1000 steps (Nusselt numbers, both masses,
+    y+, max |u|) agree with the oracle's to 1e-12 relative at every point, and
+    the final state is bitwise the oracle's."""
+    from cfd_cavity_amd import CavitySolver
+
+    rec = load_digests()[name]
+    args, steps, every = LOG_CASES[name]
+    s = CavitySolver(*args, 1)
+    for ref in rec["log"]:
+        s.run(every)
+        d = s.diagnostics()
+        assert d["time_step"] == ref["step"] and d["all_finite"]
+        for k in LOG_KEYS:
+            assert abs(d[k] - ref[k]) <= 1e-12 * abs(ref[k]), (ref["step"], k, d[k], ref[k])
+    assert s.time_step == steps
+    assert device_digest(s) == rec["sha256"]
