@@ -116,6 +116,9 @@ __global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, Adj
   const long long k = (long long)iy * pitch + ix;
   const bool wall = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
   const double cs2 = kCs2d;
+  // divisions through the step's FP64 helpers: IEEE quotients of these
+  // positive, normal divisors, with shorter sequences than the compiler's
+  using A = Arith<double>;
 
   // ---- forward: pre-collision state of node k -----------------------------
   double f[9], g[9], rho, ux, uy, temp, fyp = 0.0, r = 0.0, mx = 0.0, my = 0.0;
@@ -150,8 +153,8 @@ __global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, Adj
     ux = mx;
     uy = my;
     if (r > kF64Eps) {
-      ux = mx / r;
-      uy = my / r + 0.5 * fyp / r;
+      ux = A::div(mx, r);
+      uy = A::div(my, r) + A::div(0.5 * fyp, r);
       divided = true;
     }
     if (wall) {
@@ -184,18 +187,18 @@ __global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, Adj
   const double sq = sqrt(p2);
   const double rho_m = fmax(rho, 1e-12);
   const double den = 2.0 * cs2 * c.tau * rho_m;
-  const double smag = sq / den;
+  const double smag = A::div(sq, den);
   const double nu_t = c.cs * c.cs * smag;
   const double tp = 0.5 + 3.0 * (c.nu + nu_t);
-  const double tm = 0.5 + kLambda / (tp - 0.5);
-  const double tt = 0.5 + 3.0 * (c.alpha + nu_t / c.pr_t);
-  const double itp = 1.0 / tp, itm = 1.0 / tm, itt = 1.0 / tt;
+  const double tm = 0.5 + A::div(kLambda, tp - 0.5);
+  const double tt = 0.5 + 3.0 * (c.alpha + A::div(nu_t, c.pr_t));
+  const double itp = A::rcp(tp), itm = A::rcp(tm), itt = A::rcp(tt);
   const double af = 0.5 * (itp + itm), bf = 0.5 * (itp - itm), ag = itt;
   const double km = 1.0 - 0.5 * itm, kp = 1.0 - 0.5 * itp;
   double B[9], sm[9], sp[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    B[i] = (kCy[i] - uy) / cs2 + eu[i] * kCy[i] / (cs2 * cs2);
+    B[i] = A::divc(kCy[i] - uy, cs2, 3.0) + A::divc(eu[i] * kCy[i], cs2 * cs2, 9.0);
     const double s = kW[i] * B[i] * fy;
     sm[i] = kW[i] * (kCy[i] / cs2) * fy;
     sp[i] = s - sm[i];
@@ -233,20 +236,20 @@ __global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, Adj
   double tmb = 0.5 * kmb * itm * itm - 0.5 * itm * itm * (ab - bb);
   double tpb = 0.5 * kpb * itp * itp - 0.5 * itp * itp * (ab + bb);
   const double ttb = -agb * itt * itt;
-  double nutb = 3.0 * ttb / c.pr_t;
-  tpb += -kLambda / ((tp - 0.5) * (tp - 0.5)) * tmb;
+  double nutb = A::div(3.0 * ttb, c.pr_t);
+  tpb += A::div(-kLambda, (tp - 0.5) * (tp - 0.5)) * tmb;
   nutb += 3.0 * tpb;
   const double smagb = c.cs * c.cs * nutb;
   const double csb = 2.0 * c.cs * smag * nutb;
   double rhob = 0.0;
   if (p2 > 0.0) {
-    const double p2b = smagb * 0.5 / (sq * den);
+    const double p2b = A::div(smagb * 0.5, sq * den);
     const double pxxb = 2.0 * pxx * p2b, pxyb = 4.0 * pxy * p2b, pyyb = 2.0 * pyy * p2b;
 #pragma unroll
     for (int i = 0; i < 9; ++i)
       db[i] += kCx[i] * kCx[i] * pxxb + kCx[i] * kCy[i] * pxyb + kCy[i] * kCy[i] * pyyb;
   }
-  if (rho > 1e-12) rhob -= smag / rho_m * smagb;
+  if (rho > 1e-12) rhob -= A::div(smag, rho_m) * smagb;
   double tempb = 0.0, usqb = 0.0;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -289,10 +292,10 @@ __global__ __launch_bounds__(kRowBlock) void k_adj_node(Geom geo, StepCtx c, Adj
   }
   double rb = rhob, mxb = uxb, myb = uyb, fypb = 0.0;
   if (divided) {
-    mxb = uxb / r;
-    myb = uyb / r;
-    fypb = 0.5 * uyb / r;
-    rb += -(mx * uxb + (my + 0.5 * fyp) * uyb) / (r * r);
+    mxb = A::div(uxb, r);
+    myb = A::div(uyb, r);
+    fypb = A::div(0.5 * uyb, r);
+    rb += A::div(-(mx * uxb + (my + 0.5 * fyp) * uyb), r * r);
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) {

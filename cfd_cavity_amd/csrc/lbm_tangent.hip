@@ -23,21 +23,29 @@ __device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) {
   return {a.v * b.v, a.d * b.v + a.v * b.d};
 }
+// Division, square root and the constant-divisor quotient through the step's
+// FP64 helpers (one reciprocal, two corrected quotients): the value parts
+// are the IEEE results the forward step computes, with shorter sequences
+// than the compiler's division.
 __device__ __forceinline__ Dual operator/(Dual a, Dual b) {
-  const double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
+  const double y = Arith<double>::rcp(b.v);
+  const double q = quot_f64(a.v, b.v, y);
+  return {q, quot_f64(a.d - q * b.d, b.v, y)};
 }
 __device__ __forceinline__ Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
 __device__ __forceinline__ bool operator>(Dual a, Dual b) { return a.v > b.v; }
 __device__ __forceinline__ Dual sqrt(Dual a) {
-  const double s = ::sqrt(a.v);
-  return {s, a.v > 0.0 ? a.d / (2.0 * s) : 0.0};
+  const double s = Arith<double>::sqrtp(a.v);
+  return {s, a.v > 0.0 ? Arith<double>::div(a.d, 2.0 * s) : 0.0};
 }
 __device__ __forceinline__ Dual fmax(Dual a, Dual b) { return a.v >= b.v ? a : b; }
 
 template <> struct Arith<Dual> {
   __device__ static __forceinline__ Dual divr(Dual a, Dual b, Dual) { return a / b; }
-  __device__ static __forceinline__ Dual divc(Dual a, Dual b, Dual) { return a / b; }
+  __device__ static __forceinline__ Dual divc(Dual a, Dual b, Dual y) {
+    const double q = Arith<double>::divc(a.v, b.v, y.v);
+    return {q, Arith<double>::divc(a.d - q * b.d, b.v, y.v)};
+  }
   __device__ static __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
   __device__ static __forceinline__ Dual rcp(Dual b) { return Dual(1.0) / b; }
   __device__ static __forceinline__ Dual sqrtp(Dual x) { return sqrt(x); }

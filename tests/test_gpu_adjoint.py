@@ -37,8 +37,9 @@ def fd_check(s, theta, cs, K, C, dirs, h=1e-6, tol=1e-4):
 
     th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
     c = torch.tensor(cs, dtype=torch.float64, requires_grad=True)
-    j = NuWindow().apply(th, c, s, K, C)
-    j.backward()
+    j_t = NuWindow().apply(th, c, s, K, C)
+    j_t.backward()
+    j = float(j_t.detach())
     gth, gcs = th.grad.numpy(), float(c.grad)
     assert gth[0] == 0.0 and gth[-1] == 0.0
     errs = []
