@@ -201,20 +201,35 @@ def fp64_issue_bound(steps_per_launch: int, launch_ms: float):
     """The second bound of the multi-step kernel: its FP64 VALU issue time
     (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per launch, from the committed
     counter summary, at 4 cycles per wave instruction over the 1024 SIMDs at
-    the peak clock) as a fraction of the launch time; None when the summary
-    holds no figure for this kernel."""
-    key = f"FP64_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH"
+    the peak clock) as a fraction of the launch time, and the same for all of
+    its VALU instructions (SQ_INSTS_VALU); None when the summary holds no
+    figure for this kernel."""
+    keys = {"fp64": f"FP64_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH",
+            "valu": f"VALU_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH"}
+    found = {}
     try:
         for line in open(os.path.join(ROOT, "profiles", "k_stepn_256x1024.md")):
-            if line.startswith(key):
-                n = float(line.split("=")[1].strip())
-                busy_ms = n * FP64_CYCLES_PER_WAVE_INST / SIMDS / (CLOCK_GHZ * 1e6)
-                return {"wave_instructions_per_launch": n, "issue_ms": round(busy_ms, 6),
-                        "issue_frac": round(busy_ms / launch_ms, 4),
-                        "clock_ghz": CLOCK_GHZ, "source": "profiles/k_stepn_256x1024.md"}
+            for name, key in keys.items():
+                if line.startswith(key):
+                    found[name] = float(line.split("=")[1].strip())
     except OSError:
-        pass
-    return None
+        return None
+    if "fp64" not in found:
+        return None
+
+    def busy(n):
+        return n * FP64_CYCLES_PER_WAVE_INST / SIMDS / (CLOCK_GHZ * 1e6)
+
+    n = found["fp64"]
+    out = {"wave_instructions_per_launch": n, "issue_ms": round(busy(n), 6),
+           "issue_frac": round(busy(n) / launch_ms, 4),
+           "clock_ghz": CLOCK_GHZ, "source": "profiles/k_stepn_256x1024.md"}
+    if "valu" in found:
+        v = found["valu"]
+        out.update({"all_valu_wave_instructions_per_launch": v,
+                    "all_valu_issue_ms": round(busy(v), 6),
+                    "all_valu_issue_frac": round(busy(v) / launch_ms, 4)})
+    return out
 
 
 def copy_probe(mib: int, reps: int = 5):

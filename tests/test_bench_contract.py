@@ -39,4 +39,7 @@ def test_fp64_issue_bound_arithmetic():
     # 4 cycles per wave64 FP64 instruction over 1024 SIMDs at the peak clock
     assert b["issue_ms"] == pytest.approx(n * 4 / 1024 / 2.4e6, rel=1e-4)
     assert 0.3 < b["issue_frac"] < 0.6
+    # all VALU instructions (FP64, integer, FP32) include the FP64 ones
+    assert b["all_valu_wave_instructions_per_launch"] > n
+    assert b["issue_frac"] < b["all_valu_issue_frac"] < 0.7
     assert bench.fp64_issue_bound(3, 0.03) is None  # no figure for that kernel
