@@ -129,3 +129,17 @@ def test_no_cpu_fallback_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower().replace("the oracle is", ""), f
+
+
+def test_missing_library_fails_loudly():
+    """Without the built library the package raises on first use: there is no
+    CPU fallback for the product path."""
+    import sys
+
+    code = ("import cfd_cavity_amd._lib as L; L.LIB_PATH = '/nonexistent/libcavlbm.so'\n"
+            "try:\n    L.lib()\nexcept RuntimeError as e:\n    print('raised:', e)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=root, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "raised:" in p.stdout and "no CPU fallback" in p.stdout
