@@ -57,28 +57,13 @@ def make_solver(storage: str, tblock=None):
     return s
 
 
-def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int = 11):
-    """One storage form with the library's default steps per launch; a name
-    ending in "_<n>step" runs the same storage with at most n steps per
-    launch (n = 1: the single-step kernel), for comparison.
-
-    After `warmup` untimed steps the timed region -- exactly `steps` steps,
-    bracketed by stream and device synchronisations -- runs `reps` times in
-    a row.  Each repetition is timed by HIP events recorded on the solver's
-    stream around its launches and by the host clock around the
-    synchronisations; the figures are the medians over the repetitions."""
-    import statistics
-
+def timed_regions(s, steps: int, reps: int):
+    """`reps` repetitions of exactly `steps` steps, each bracketed by stream
+    and device synchronisations and timed by HIP events recorded on the
+    solver's stream around its launches and by the host clock; returns the
+    per-repetition event and wall times (ms)."""
     import torch
 
-    storage, tblock = form, None
-    if form.endswith("step") and "_" in form:
-        storage, suffix = form.rsplit("_", 1)
-        tblock = int(suffix[:-len("step")])
-    s = make_solver(storage, tblock)
-    spl = s.steps_per_launch
-    nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
-    s.run(warmup)
     ev, walls = [], []
     for _ in range(reps):
         s.synchronize()
@@ -87,9 +72,64 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int
         ev.append(s.time_run(steps))
         s.synchronize()
         torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
+        walls.append((time.perf_counter() - t0) * 1e3)
+    return ev, walls
+
+
+def load_gpu(storage: str, tblock, ms: float):
+    """Keeps the GPU busy for about `ms` milliseconds with the same workload
+    on a throwaway solver.  The GPU's clocks settle only under sustained load:
+    the same 4-step launch, at the same cycle count, takes 39-42 us after an
+    idle gap and 36.2 us after about 100 ms of load, whatever the flow state
+    (profiles/k_stepn_clock_state_256x1024.md).  Returns the solver, which the
+    caller destroys after its measurement (a destruction in between would
+    leave the GPU idle again), and the steps it ran."""
+    if ms <= 0:
+        return None, 0
+    w = make_solver(storage, tblock)
+    t0 = time.perf_counter()
+    n = 0
+    while (time.perf_counter() - t0) * 1e3 < ms:
+        w.run(400)
+        w.synchronize()
+        n += 400
+    return w, n
+
+
+def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int = 11,
+             gpu_load_ms: float = 300.0):
+    """One storage form with the library's default steps per launch; a name
+    ending in "_<n>step" runs the same storage with at most n steps per
+    launch (n = 1: the single-step kernel), for comparison.
+
+    After `warmup` untimed steps the timed region -- exactly `steps` steps --
+    runs `reps` times in a row (timed_regions); the figures are the medians
+    over the repetitions.  This is done twice on fresh solvers of the
+    workload: right after the solver is set up (the GPU coming out of idle,
+    reported as *_after_idle), and then, for the reported figures, right
+    after `gpu_load_ms` of sustained load on a throwaway solver
+    (load_gpu), which brings the GPU to the clocks a long run sees; the
+    measured solver's own warmup and timed steps are unchanged."""
+    import statistics
+
+    storage, tblock = form, None
+    if form.endswith("step") and "_" in form:
+        storage, suffix = form.rsplit("_", 1)
+        tblock = int(suffix[:-len("step")])
+    nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
+    cold = make_solver(storage, tblock)
+    cold.run(warmup)
+    ev_cold, _ = timed_regions(cold, steps, reps)
+    cold.close()
+    s = make_solver(storage, tblock)
+    spl = s.steps_per_launch
+    load, load_steps = load_gpu(storage, tblock, gpu_load_ms)
+    s.run(warmup)
+    ev, walls = timed_regions(s, steps, reps)
+    if load is not None:
+        load.close()
     ev_ms = statistics.median(ev)
-    wall = statistics.median(walls)
+    wall = statistics.median(walls) * 1e-3
     d = s.diagnostics()
     kern_ms = s.time_step_kernel(kernel_launches) if kernel_launches > 0 else float("nan")
     mlups = nodes * steps / (ev_ms * 1e-3) / 1e6
@@ -106,6 +146,10 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int
         "ms_per_step_wall": wall * 1e3 / steps,
         "mlups_wall": nodes * steps / wall / 1e6,
         "ms_per_step_events_reps": [round(v / steps, 6) for v in ev],
+        "ms_per_step_after_idle": statistics.median(ev_cold) / steps,
+        "mlups_after_idle": nodes * steps / (statistics.median(ev_cold) * 1e-3) / 1e6,
+        "gpu_load_ms": gpu_load_ms,
+        "gpu_load_steps": load_steps,
         "kernel_ms_isolated": kern_ms,
         "launch_ms": launch_ms,
         "bytes_per_lup": b,
@@ -230,6 +274,32 @@ def fp64_issue_bound(steps_per_launch: int, launch_ms: float):
                     "all_valu_issue_ms": round(busy(v), 6),
                     "all_valu_issue_frac": round(busy(v) / launch_ms, 4)})
     return out
+
+
+def clock_probe():
+    """The 4-step kernel's launch time against the GPU's recent activity, for
+    a kernel trace (rocprofv3 --kernel-trace): a fresh solver's first 100
+    steps, 8000 steps of sustained load, a second fresh solver (early flow
+    state) right after that load, the loaded solver again, and again after
+    one second of idle.  Phases are printed as they start."""
+    s = make_solver("fp64")
+    s2 = make_solver("fp64")
+    s2.run(5)
+    s2.synchronize()
+    phases = [("fresh solver, 100 steps", s, 100), ("sustained load, 8000 steps", s, 8000),
+              ("early-state solver right after the load, 100 steps", s2, 100),
+              ("loaded solver, 400 steps", s, 400)]
+    for i, (name, solver, n) in enumerate(phases, 1):
+        print(f"[clock-probe] phase {i}: {name}", file=sys.stderr, flush=True)
+        solver.run(n)
+        solver.synchronize()
+    time.sleep(1.0)
+    print("[clock-probe] phase 5: the same solver after 1 s idle, 400 steps", file=sys.stderr,
+          flush=True)
+    s.run(400)
+    s.synchronize()
+    s.close()
+    s2.close()
 
 
 def copy_probe(mib: int, reps: int = 5):
@@ -524,6 +594,9 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32,fp16_offset,fp64_2step,fp64_1step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gpu-load-ms", type=float, default=300.0,
+                    help="sustained load on a throwaway solver before each form's timed "
+                         "steps (0: none); the form's after-idle figures are measured first")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-total-steps", type=int, default=50000,
                     help="run the batch on to this many steps after timing (0: no)")
@@ -537,16 +610,25 @@ def main():
                     help="steps of the FP64/FP32/FP16-offset comparison (0: skip)")
     ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree,adjoint",
                     help="paths to measure (a subset prints a partial line, for profiling)")
+    ap.add_argument("--clock-probe", action="store_true",
+                    help="only run the launch-time-against-GPU-activity phases (for a trace)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
+    if a.clock_probe:
+        clock_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the multi-GPU "
               "layout is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    # torch (device synchronisation, copies) brings its own HIP runtime; it is
+    # loaded before the first solver loads the library's, as torch's device
+    # initialisation fails when the other runtime was initialised first
+    import torch  # noqa: F401
 
     paths = set(a.paths.split(","))
     if "uniform" not in paths:
@@ -556,7 +638,8 @@ def main():
     forms = {}
     for storage in filter(None, a.forms.split(",")):
         print(f"[bench] {storage}: warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
-        forms[storage] = gpu_form(storage, a.steps, a.warmup, a.kernel_launches)
+        forms[storage] = gpu_form(storage, a.steps, a.warmup, a.kernel_launches,
+                                  gpu_load_ms=a.gpu_load_ms)
     main_form = forms.get("fp64") or next(iter(forms.values()), None)
     ref_nu = forms["fp64"]["nusselt_hot"] if "fp64" in forms else None
     for k, v in forms.items():
@@ -621,6 +704,11 @@ def main():
         "ms_per_step": main_form["ms_per_step"],
         "ms_per_step_wall": main_form["ms_per_step_wall"],
         "value_wall": round(main_form["mlups_wall"], 2),
+        "ms_per_step_after_idle": main_form["ms_per_step_after_idle"],
+        "value_after_idle": round(main_form["mlups_after_idle"], 2),
+        "clock_state": (f"timed right after {a.gpu_load_ms:g} ms of sustained load on a "
+                        "throwaway solver of the same workload (clocks settled as in a long "
+                        "run); *_after_idle: the same measurement right after set-up"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(main_form["mlups"] / REFERENCE_CPU_MLUPS_256x1024, 2),
@@ -640,7 +728,7 @@ def main():
                                                if traffic else None),
                      "launch_ms": main_form["launch_ms"], "steps_per_launch": spl,
                      "timing": "HIP events on the solver stream around each of 11 repetitions "
-                               "of the timed steps; median",
+                               "of the timed steps; median; settled clocks (clock_state)",
                      "fp64_valu": fp64_issue_bound(spl, main_form["launch_ms"])},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
