@@ -12,7 +12,12 @@ single-step kernel beside it.  The timed region (exactly --steps steps after
 --warmup untimed ones) runs eleven times; ms_per_step and value come from HIP
 events on the solver stream (median of the eleven: a 20-step region lasts
 about 0.2 ms and single regions vary by a few percent), the host wall clock
-around the synchronisations is reported beside them.  Prints ONE JSON line.
+around the synchronisations is reported beside them.  The GPU's clock settles
+only under sustained load (the same launch takes the same cycles but up to
+15 % longer right after an idle gap), so each form is measured twice on fresh
+solvers: right after set-up (reported as *_after_idle) and right after
+--gpu-load-ms of sustained load on a throwaway solver of the same workload
+(the reported figures, the clocks of a long run).  Prints ONE JSON line.
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
 rayon path) timed on this host's cores on a bounded sample of the same
