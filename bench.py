@@ -51,6 +51,13 @@ METRIC = "MLUPS & %HBM peak at 256x1024; mixed vs FP64 at rel dNu<=1e-4"
 WORKLOAD = dict(nx=256, ny=1024, ra=1.0e10, pr=0.71, cs=0.10, pr_t=0.9, utarget=0.005, seed=42)
 
 
+def storage_of(form: str) -> str:
+    """The storage of a form name ("fp32", "fp64_2step" -> "fp64", ...)."""
+    if form.endswith("step") and "_" in form:
+        return form.rsplit("_", 1)[0]
+    return form
+
+
 def make_solver(storage: str, tblock=None):
     from cfd_cavity_amd import CavitySolver
 
@@ -117,10 +124,9 @@ def gpu_form(form: str, steps: int, warmup: int, kernel_launches: int, reps: int
     measured solver's own warmup and timed steps are unchanged."""
     import statistics
 
-    storage, tblock = form, None
-    if form.endswith("step") and "_" in form:
-        storage, suffix = form.rsplit("_", 1)
-        tblock = int(suffix[:-len("step")])
+    storage, tblock = storage_of(form), None
+    if storage != form:
+        tblock = int(form.rsplit("_", 1)[1][:-len("step")])
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     cold = make_solver(storage, tblock)
     cold.run(warmup)
@@ -717,7 +723,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(main_form["mlups"] / REFERENCE_CPU_MLUPS_256x1024, 2),
-        "dtype": DTYPE["fp64" if "fp64" in forms else next(iter(forms))],
+        "dtype": DTYPE["fp64" if "fp64" in forms else storage_of(next(iter(forms)))],
         "data": "synthetic (reference init: linear T + seeded xorshift64* noise)",
         "config": {"workload": "thermal LBM D2Q9 f+g DDF cavity 256x1024, Ra=1e10, Pr=0.71, "
                                "Pr_t=0.9, Cs=0.10, utarget=0.005, seed 42; TRT+Smagorinsky+Guo, "

@@ -43,3 +43,9 @@ def test_fp64_issue_bound_arithmetic():
     assert b["all_valu_wave_instructions_per_launch"] > n
     assert b["issue_frac"] < b["all_valu_issue_frac"] < 0.7
     assert bench.fp64_issue_bound(3, 0.03) is None  # no figure for that kernel
+
+
+def test_form_names_map_to_storages():
+    assert [bench.storage_of(f) for f in ("fp64", "fp32", "fp16_offset", "fp64_2step",
+                                          "fp64_1step", "fp32_1step", "fp16_offset_1step")] == \
+        ["fp64", "fp32", "fp16_offset", "fp64", "fp64", "fp32", "fp16_offset"]
