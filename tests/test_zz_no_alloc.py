@@ -43,3 +43,20 @@ def test_static_window_rejects_a_lattice_beyond_the_array():
     with pytest.raises(CavlbmError) as e:
         nu_window_static(s, None, 0.10, 10)
     assert e.value.code == E_SIZE
+
+
+def test_static_window_at_the_array_limits():
+    """64x1024 fills the static array exactly (a pitched plane of 65 536 nodes
+    and 1024 rows): J equals the allocating entry point's.  One column or one
+    row more is a size error."""
+    s = spun_up(64, 1024, 1e6, 60)
+    theta = 0.5 + 0.01 * np.random.default_rng(5).uniform(-1, 1, 1024)
+    j_static = nu_window_static(s, theta, 0.10, 24)
+    j_alloc = nu_window(s, theta, 0.10, 24)
+    print(f"\n[no-alloc] 64x1024 K=24: static {j_static!r} allocating {j_alloc!r}")
+    assert j_static == j_alloc
+    for nx, ny in ((65, 1024), (64, 1025)):
+        t = CavitySolver(nx, ny, 1e6, 0.71, 0.10, 0.9, 0.02, 42, 1)
+        with pytest.raises(CavlbmError) as e:
+            nu_window_static(t, None, 0.10, 10)
+        assert e.value.code == E_SIZE, (nx, ny)
