@@ -62,7 +62,7 @@ def test_cli_run_averaging_snapshot_and_log(tmp_path):
     summ = json.loads((out / "summary.json").read_text())
     assert summ["time_samples"] == 15 and summ["steps"] == 2500
     nx, ny, rho, ux, uy, t = snapshot.read_snapshot(out / "snapshot.bin")
-    # the driver's final state equals the library stepped the same number of steps
+    # cavity_les's final state equals the library stepped the same number of steps
     s = CavitySolver(32, 128, 1e6, 0.71, 0.1, 0.9, 0.02, 42, 1)
     s.run(2500)
     assert (nx, ny) == (32, 128) and np.array_equal(t, s.get_field("temperature"))
