@@ -450,10 +450,20 @@ __device__ __forceinline__ R geq(int i, R t, R ux, R uy) {
 //   a_g = (1/tau_t + 1/tau_t)/2, b_g = (1/tau_t - 1/tau_t)/2,
 //   k_minus = 1 - 1/(2 tau-), k_plus = 1 - 1/(2 tau+)
 // (solver.rs:1140-1185).
+//
+// Terms that are always a zero are left out: b_g = +0 and the odd force
+// part s_minus = w_i (0/cs2) fy of the directions with cy = 0, and sums start
+// at their first term instead of +0.  A zero term (of either sign) leaves a
+// rounded sum or difference with the value it has without that term, and
+// changes at most the sign of a result that is zero; adding such a value to
+// a nonzero population gives the same population.  So every collision below
+// returns the reference's bits for every state whose populations are
+// nonzero (a population is w_i rho or w_i T plus O(u) terms), and equal
+// values in any case.  Sums whose terms are all zeros, the only ones whose
+// sign could change, occur only there.
 template <class R, int GEQ>
 __device__ __forceinline__ void collide_neq(const R (&f)[9], const R (&g)[9], R rho, R ux, R uy,
-                                            R temp, R (&d_f)[9], R (&d_g)[9]) {
-  EqTerms<R> e;
+                                            R temp, R (&d_f)[9], R (&d_g)[9], EqTerms<R>& e) {
   eq_terms<R>(ux, uy, e);
   R feq_[9], geq_[9];
 #pragma unroll
@@ -477,9 +487,10 @@ __device__ __forceinline__ void collide_neq(const R (&f)[9], const R (&g)[9], R 
 
 // Relaxation factors of one collision (the per-node part of collide_node, or
 // constants of a level with fixed relaxation times).
+// b_g = (1/tau_t - 1/tau_t)/2 is +0 for every finite tau_t and is not kept.
 template <class R>
 struct RelaxF {
-  R a_f, b_f, a_g, b_g, k_minus, k_plus;
+  R a_f, b_f, a_g, k_minus, k_plus;
 };
 
 // From tau+ and tau_t+ (tau- from the magic parameter Lambda = 3/16), in the
@@ -495,8 +506,7 @@ __host__ __device__ __forceinline__ RelaxF<R> relax_factors(R tau_plus, R tau_t_
   RelaxF<R> r;
   r.a_f = R(0.5) * (itp + itm);
   r.b_f = R(0.5) * (itp - itm);
-  r.a_g = R(0.5) * (itt + itt);
-  r.b_g = R(0.5) * (itt - itt);
+  r.a_g = itt;  // (itt + itt)/2: doubling and halving are exact
   r.k_minus = R(1.0) - R(0.5) * itm;
   r.k_plus = R(1.0) - R(0.5) * itp;
   return r;
@@ -505,15 +515,17 @@ __host__ __device__ __forceinline__ RelaxF<R> relax_factors(R tau_plus, R tau_t_
 template <class R>
 __device__ __forceinline__ void collide_relax(R (&f)[9], R (&g)[9], R ux, R uy, R fy,
                                               const R (&d_f)[9], const R (&d_g)[9],
-                                              const RelaxF<R>& r) {
+                                              const RelaxF<R>& r, const EqTerms<R>& e) {
   using A = Arith<R>;
   // Guo source S_i = w_i [(cy - uy)/cs2 + (c.u) cy / cs2^2] fy; its odd part
   // s_minus = w_i (cy/cs2) fy carries the momentum (solver.rs:1168-1179).
   // (cy - uy)/cs2 takes three values; (c.u) cy is +-eu of the directions
   // with cy = 1 (exact negation for cy = -1), and a signed zero for cy = 0,
-  // where (0 - uy)/cs2 is never -0, so that term adds nothing.
+  // which leaves the bracket's value unchanged.  (0 - uy)/cs2 is the negated
+  // quotient uy/cs2 of the equilibria (RN is sign-symmetric; for uy = 0 the
+  // two are zeros of possibly different sign, and the term S_i is a zero).
   const R a_p = A::divc(R(1.0) - uy, R(kCs2), R(kInvCs2));
-  const R a_0 = A::divc(R(0.0) - uy, R(kCs2), R(kInvCs2));
+  const R a_0 = -e.q1[3];
   const R a_m = A::divc(R(-1.0) - uy, R(kCs2), R(kInvCs2));
   const R b2 = A::divc(ux + uy, R(kDenCs4), R(kInvDenCs4));
   const R b3 = A::divc(uy, R(kDenCs4), R(kInvDenCs4));
@@ -527,16 +539,19 @@ __device__ __forceinline__ void collide_relax(R (&f)[9], R (&g)[9], R ux, R uy, 
   R fo[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    const R cy = R(kCy[i]);
-    const R s_minus = R(kW[i]) * (cy / R(kCs2)) * fy;
-    const R s_plus = s_i[i] - s_minus;
-    fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + r.k_minus * s_minus +
-                    r.k_plus * s_plus);
+    if (kCy[i] == 0) {  // s_minus is a zero, s_plus = s_i
+      fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + r.k_plus * s_i[i]);
+    } else {
+      const R s_minus = R(kW[i]) * (R(kCy[i]) / R(kCs2)) * fy;
+      const R s_plus = s_i[i] - s_minus;
+      fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + r.k_minus * s_minus +
+                      r.k_plus * s_plus);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     f[i] = fo[i];
-    g[i] = g[i] + (-r.a_g * d_g[i] - r.b_g * d_g[kOpp[i]]);
+    g[i] = g[i] + -r.a_g * d_g[i];  // b_g d_g[opp] is a zero
   }
 }
 
@@ -553,13 +568,15 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R fy = rho * R(c.beta) * (temp - R(c.t_ref));
   fy_out = fy;
   R d_f[9], d_g[9];
-  collide_neq<R, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g);
+  EqTerms<R> e;
+  collide_neq<R, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g, e);
   // pi_neq = sum c c (f - feq) (solver.rs:1096-1106).  With c in {-1,0,1} the
-  // products are exact; zero products are dropped because the accumulators
-  // start at +0 and never hold -0, so adding a signed zero leaves them as is.
-  const R pi_xx = R(0.0) + d_f[1] + d_f[2] + d_f[4] + d_f[5] + d_f[6] + d_f[8];
-  const R pi_xy = R(0.0) + d_f[2] - d_f[4] + d_f[6] - d_f[8];
-  const R pi_yy = R(0.0) + d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
+  // products are exact and the zero ones are dropped; the sums start at their
+  // first term rather than +0, which changes at most the sign of a zero
+  // component, and the components are only squared.
+  const R pi_xx = d_f[1] + d_f[2] + d_f[4] + d_f[5] + d_f[6] + d_f[8];
+  const R pi_xy = d_f[2] - d_f[4] + d_f[6] - d_f[8];
+  const R pi_yy = d_f[2] + d_f[3] + d_f[4] + d_f[6] + d_f[7] + d_f[8];
   const R pi2 = pi_xx * pi_xx + R(2.0) * pi_xy * pi_xy + pi_yy * pi_yy;
   const R s_mag = A::div(A::sqrtp(pi2), R(2.0) * R(kCs2) * R(c.tau) * fmax(rho, R(1e-12)));
   const R nu_t = PRE ? nut_pre : cs * cs * s_mag;
@@ -568,7 +585,7 @@ __device__ __forceinline__ void collide_node(R (&f)[9], R (&g)[9], R rho, R ux, 
   const R tau_plus = R(0.5) + R(3.0) * (R(c.nu) + nu_t);
   const R alpha_t = PRE ? at_pre : A::divr(nu_t, R(c.pr_t), R(c.inv_pr_t));
   const R tau_t_plus = R(0.5) + R(3.0) * (R(c.alpha) + alpha_t);
-  collide_relax<R>(f, g, ux, uy, fy, d_f, d_g, relax_factors<R>(tau_plus, tau_t_plus));
+  collide_relax<R>(f, g, ux, uy, fy, d_f, d_g, relax_factors<R>(tau_plus, tau_t_plus), e);
 }
 
 // A collision with relaxation factors fixed in advance (the octree levels'
@@ -582,8 +599,9 @@ __device__ __forceinline__ void collide_node_fixed(double (&f)[9], double (&g)[9
   const double fy = rho * beta * (temp - t_ref);
   fy_out = fy;
   double d_f[9], d_g[9];
-  collide_neq<double, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g);
-  collide_relax<double>(f, g, ux, uy, fy, d_f, d_g, r);
+  EqTerms<double> e;
+  collide_neq<double, GEQ>(f, g, rho, ux, uy, temp, d_f, d_g, e);
+  collide_relax<double>(f, g, ux, uy, fy, d_f, d_g, r, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,15 +641,17 @@ __device__ __forceinline__ void isothermal_g(const Walls& walls, int ix, int iy,
 // update_macroscopic with the Guo correction (solver.rs:319-345) and
 // enforce_wall_velocity (solver.rs:992-1009) on the pulled populations.
 // Sums in the reference's order j = 0..8; zero products dropped and products
-// by -1 turned into subtractions (exact, accumulators from +0).
+// by -1 turned into subtractions (exact).  The reference's accumulators start
+// at +0; these start at the first term, which is the same unless every term
+// is a zero (then at most the sign of the zero sum differs).
 template <class R>
 __device__ __forceinline__ void node_moments(const R (&f)[9], const R (&g)[9], R fyp,
                                              bool wall, R& rho, R& ux, R& uy, R& temp) {
   using A = Arith<R>;
-  const R r = R(0.0) + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-  R un = R(0.0) + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
-  R vn = R(0.0) + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
-  const R tn = R(0.0) + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+  const R r = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  R un = f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
+  R vn = f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
+  const R tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
   if (r > R(kF64Eps)) {
     const R ir = A::rcp(r);
     un = A::divr(un, r, ir);

@@ -27,10 +27,24 @@ namespace cavlbm {
 
 constexpr int kMaxBlockSteps = 4;
 
-// Per-step scalars of the N collisions of one launch.
+// Per-step scalars of the N collisions of one launch.  t_ref, cs, pr_t and
+// inv_pr_t are constants of the solver, the same in every entry.
 struct StepCtxN {
   StepCtx c[kMaxBlockSteps];
 };
+
+// The collision scalars of sub-step s (0-based): beta, nu, alpha and tau of
+// that step, the solver's constants from entry 0.  Reading the constants
+// once keeps the kernel's scalar registers for one copy of them (four copies
+// spill to vector lanes).
+__device__ __forceinline__ StepCtx sub_ctx(const StepCtxN& cn, int s) {
+  StepCtx c = cn.c[0];
+  c.beta = cn.c[s].beta;
+  c.nu = cn.c[s].nu;
+  c.alpha = cn.c[s].alpha;
+  c.tau = cn.c[s].tau;
+  return c;
+}
 
 // Tile shapes per storage type and steps per launch: output tile kTX x kTY,
 // first sub-step on (kTX + 2(N-1)) x (kTY + 2(N-1)) nodes, kNT threads.
@@ -212,7 +226,8 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
           nu_acc[iy] += fabs((double)(tc - t2));
         }
       }
-      collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, cn.c[s - 1], R(cn.c[s - 1].cs), fyc, nutc);
+      collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, sub_ctx(cn, s - 1), R(cn.c[0].cs), fyc,
+                           nutc);
       if (s < N) storage_round<S>(f2, g2, tc);
 #pragma unroll
       for (int i = 0; i < 9; ++i) {

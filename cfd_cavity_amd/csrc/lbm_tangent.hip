@@ -127,10 +127,10 @@ __global__ __launch_bounds__(kRowBlock) void k_tangent_step(Geom geo, StepCtx ct
     }
     pulled_g_dual(geo, X, Xd, theta, dtheta, ix, iy, g);
     // update_macroscopic with the Guo correction, same sums as the step kernel
-    const Dual r = Dual(0.0) + f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-    Dual un = Dual(0.0) + f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
-    Dual vn = Dual(0.0) + f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
-    const Dual tn = Dual(0.0) + g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+    const Dual r = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    Dual un = f[1] + f[2] - f[4] - f[5] - f[6] + f[8];
+    Dual vn = f[2] + f[3] + f[4] - f[6] - f[7] - f[8];
+    const Dual tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
     const Dual fyp(fy[k], fyd[k]);
     if (r > Dual(kF64Eps)) {
       const Dual ir = A::rcp(r);
