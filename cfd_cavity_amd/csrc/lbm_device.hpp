@@ -536,11 +536,25 @@ __device__ __forceinline__ void collide_relax(R (&f)[9], R (&g)[9], R ux, R uy, 
   for (int i = 0; i < 9; ++i) s_i[i] = R(kW[i]) * bracket[i] * fy;
   if (kLowp<R>)  // FP32 arithmetic only: sum_i S_i = 0 exactly
     s_i[0] = -(s_i[1] + s_i[2] + s_i[3] + s_i[4] + s_i[5] + s_i[6] + s_i[7] + s_i[8]);
+  // odd force parts of the directions (0, 1) and (1, 1) and their k_minus terms
+  const R sm_a = R(kW[3]) * (R(1.0) / R(kCs2)) * fy;
+  const R sm_d = R(kW[2]) * (R(1.0) / R(kCs2)) * fy;
+  const R km_a = r.k_minus * sm_a;
+  const R km_d = r.k_minus * sm_d;
   R fo[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     if (kCy[i] == 0) {  // s_minus is a zero, s_plus = s_i
       fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + r.k_plus * s_i[i]);
+    } else if (!kLowp<R>) {
+      // w_i (cy/cs2) fy and k_minus times it for cy = -1 are the exact
+      // negations of those for cy = +1 (RN is sign-symmetric)
+      const R sm = kCx[i] == 0 ? sm_a : sm_d;
+      const R km = kCx[i] == 0 ? km_a : km_d;
+      const R s_minus = kCy[i] > 0 ? sm : -sm;
+      const R s_plus = s_i[i] - s_minus;
+      fo[i] = f[i] + (-r.a_f * d_f[i] - r.b_f * d_f[kOpp[i]] + (kCy[i] > 0 ? km : -km) +
+                      r.k_plus * s_plus);
     } else {
       const R s_minus = R(kW[i]) * (R(kCy[i]) / R(kCs2)) * fy;
       const R s_plus = s_i[i] - s_minus;

@@ -734,10 +734,13 @@ hipError_t go_dyn_step(const Geom& geo, const StepCtx& ctx, const Walls& walls, 
   using ST = StorageT<S>;
   using T = typename ST::T;
   using R = typename ST::R;
-  // 16x16 tiles; FP32-arithmetic storages with filters of 1..2 nodes take
-  // 32x16 (61 KB of LDS, two workgroups per CU).  Measured at 256x1024:
-  // FP64 29.8 us (16x16) against 30.8 (32x16), FP32 24.5 against 22.6.
-  constexpr int TX = (S != ST_FP64 && W <= 2) ? 32 : kDynTX, TY = kDynTY;
+  // 32x32 tiles: one workgroup of 1024 threads per CU (114 KB of LDS at
+  // W = 2, 123 KB at W = 3).  The grown regions the tile pulls and models
+  // are 1.27x its nodes at W = 2 against 1.56x for 16x16, so the step moves
+  // and recomputes less.  Measured at 256x1024, Ra=1e10, W = 2 (same box,
+  // one process per variant, three each): FP64 27.3 us against 28.3 (16x16),
+  // 29.3 (32x16) and 29.0 (16x32); FP32 storage 20.3 against 21.3 (32x16).
+  constexpr int TX = 32, TY = 32;
   Geom tiles = geo;
   tiles.nbx = (geo.nx + TX - 1) / TX;
   tiles.nby = (geo.ny + TY - 1) / TY;
