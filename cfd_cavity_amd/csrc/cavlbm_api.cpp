@@ -457,7 +457,8 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
   uint64_t left = n;
   while (left > 0) {
     // dynamic-SGS steps after the spin-up: graphs of kGraphSteps steps
-    if (s->dyn_on && s->c_valid && !s->graph_failed && s->time_step >= s->d.spin_up_steps &&
+    if (s->dyn_on && s->c_valid && !s->graph_failed && graph_replay_enabled() &&
+        s->time_step >= s->d.spin_up_steps &&
         left > (uint64_t)kGraphSteps) {
       hipGraphExec_t& g = s->dgraph[s->cur | (s->fy_par << 1)];
       if (!g) g = capture_dyn(s);
@@ -475,7 +476,7 @@ int run_steps(cavlbm_solver* s, uint64_t n) {
     // Steady part: whole graphs of kGraphSteps fused steps (constant per-step
     // scalars after spin-up), keeping the final step direct so that it writes
     // nu_t; with the dynamic model's kept values, graphs of their own.
-    if (s->c_valid && !s->graph_failed && !s->dyn_on &&
+    if (s->c_valid && !s->graph_failed && graph_replay_enabled() && !s->dyn_on &&
         s->time_step >= s->d.spin_up_steps && left > (uint64_t)kGraphSteps) {
       hipGraphExec_t& g = (s->dyn_pre ? s->pgraph : s->graph)[s->cur | (s->fy_par << 1)];
       if (!g) g = capture(s);
@@ -1392,7 +1393,8 @@ int batch_run_steps(cavlbm_batch* b, uint64_t n) {
   uint64_t left = n - consumed;
   cavlbm_solver* s0 = b->cav[0];
   while (left > 0) {
-    if (!b->graph_failed && s0->time_step >= b->max_spin && left > (uint64_t)kGraphSteps) {
+    if (!b->graph_failed && graph_replay_enabled() && s0->time_step >= b->max_spin &&
+        left > (uint64_t)kGraphSteps) {
       hipGraphExec_t& g = b->graph[s0->cur | (s0->fy_par << 1)];
       if (!g) g = batch_capture(b, s0->cur, s0->fy_par);
       if (!g) {

@@ -555,7 +555,7 @@ hipGraphExec_t capture_step(cavlbm_octree* o, int* next) {
 int run_steps(cavlbm_octree* o, uint64_t n) {
   for (uint64_t s = 0; s < n; ++s) {
     // constant per-step scalars without the viscosity ramp: replay one graph
-    if (o->nu_start_mult == 1.0 && !o->graph_failed) {
+    if (o->nu_start_mult == 1.0 && !o->graph_failed && graph_replay_enabled()) {
       const int key = mkey(o);
       if (!o->graph[key]) o->graph[key] = capture_step(o, &o->graph_next[key]);
       if (!o->graph[key]) {

@@ -4,10 +4,24 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "lbm_device.hpp"
 #include "lbm_stepn.hpp"
 
 namespace cavlbm {
+
+// Whether the steady run loops replay captured graphs (the default).  With
+// CAVLBM_GRAPHS=0 in the environment every step block is launched directly:
+// the same kernels and the same results, for tools that intercept graph
+// launches and to compare the two launch paths.  Read once per process.
+inline bool graph_replay_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("CAVLBM_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 
 // One step-kernel launch: storage S, g-equilibrium order geq (1 or 2), mode.
 hipError_t launch_step_fp64(int geq, int mode, const Geom& geo, const StepCtx& ctx,

@@ -123,6 +123,46 @@ def test_graph_replay_equals_single_steps(oracle):
     assert bits_equal(a.get_pops("g"), b.get_pops("g"))
 
 
+_DIRECT_CHILD = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from cfd_cavity_amd import CavitySolver
+from tests.helpers import SMALL, args_of
+s = CavitySolver(*args_of(SMALL))
+if sys.argv[3] == "dyn":
+    s.set_dynamic_sgs(True)
+s.run(600)
+np.savez(sys.argv[2], f=s.get_pops("f"), g=s.get_pops("g"), temperature=s.get_field("temperature"),
+         nu_t=s.get_field("nu_t"))
+"""
+
+
+@pytest.mark.parametrize("mode", ["plain", "dyn"])
+def test_direct_launches_equal_graph_replay(tmp_path, mode):
+    """CAVLBM_GRAPHS=0 (read once per process, so in a child process) launches
+    every step block directly; the run after the spin-up, which otherwise
+    replays captured graphs, gives the same bits."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "direct.npz")
+    env = dict(os.environ, CAVLBM_GRAPHS="0")
+    subprocess.run([sys.executable, "-c", _DIRECT_CHILD, root, out, mode], env=env, check=True,
+                   timeout=300)
+    d = np.load(out)
+    a = CavitySolver(*args_of(SMALL))
+    if mode == "dyn":
+        a.set_dynamic_sgs(True)
+    assert a.spin_up_steps < 600 - 48
+    a.run(600)
+    for name, got in (("f", a.get_pops("f")), ("g", a.get_pops("g")),
+                      ("temperature", a.get_field("temperature")), ("nu_t", a.get_field("nu_t"))):
+        assert bits_equal(got, d[name]), f"{mode} {name}: {first_mismatch(got, d[name])}"
+
+
 def test_interleaved_reads_do_not_perturb(oracle):
     g, o = pair(oracle)
     for chunk in (3, 1, 70, 200, 5):
