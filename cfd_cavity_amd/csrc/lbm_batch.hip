@@ -72,12 +72,14 @@ __global__ __launch_bounds__((TileN<S, N>::kNT)) void k_stepn_batch(
   R* nut = static_cast<R*>(b.nut);
   const bool interior = bx * kTX - kH >= 1 && (bx + 1) * kTX + kH - 1 <= d.geon.nx - 2 &&
                         by * kTY - kH >= 1 && (by + 1) * kTY + kH - 1 <= d.geon.ny - 2;
+  // without the sub-step priority hint (stepn_prio): the batch's tiles
+  // measured 0.098 against 0.106 ms per batch step with it
   if (interior)
-    stepn_tile<S, GEQ, N, true>(d.geon, cn, d.walls, src, dst, fy_in, fy_out, nut, write_aux, sh,
-                                bx, by);
+    stepn_tile<S, GEQ, N, true, false, false>(d.geon, cn, d.walls, src, dst, fy_in, fy_out, nut,
+                                              write_aux, sh, bx, by);
   else
-    stepn_tile<S, GEQ, N, false>(d.geon, cn, d.walls, src, dst, fy_in, fy_out, nut, write_aux,
-                                 sh, bx, by);
+    stepn_tile<S, GEQ, N, false, false, false>(d.geon, cn, d.walls, src, dst, fy_in, fy_out, nut,
+                                               write_aux, sh, bx, by);
 }
 
 template <int S, int GEQ, int N>
