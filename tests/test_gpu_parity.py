@@ -163,6 +163,64 @@ def test_direct_launches_equal_graph_replay(tmp_path, mode):
         assert bits_equal(got, d[name]), f"{mode} {name}: {first_mismatch(got, d[name])}"
 
 
+def test_collision_zero_terms_on_exact_zero_states(oracle):
+    """The collision leaves out terms that are always a zero (b_g, the odd
+    force part of the cy = 0 directions, the +0 starts of the sums) and takes
+    (0 - uy)/cs2 as -(uy/cs2).  On states full of exact zeros -- u = 0 at
+    equilibrium (zero non-equilibrium parts), uy = -0.0, T = 0 (f_y = 0),
+    populations exactly 0 -- one collision must equal the oracle's: bit for
+    bit where the populations are nonzero, as values everywhere."""
+    g, o = pair(oracle)
+    n = SMALL["nx"] * SMALL["ny"]
+    rng = np.random.default_rng(11)
+    rho = 1.0 + 1e-3 * rng.standard_normal(n)
+    ux = 1e-3 * rng.standard_normal(n)
+    uy = 1e-3 * rng.standard_normal(n)
+    t = 0.5 * rng.uniform(-1, 1, n)
+    kind = np.arange(n) % 5
+    ux[kind == 0] = 0.0          # at rest: f = feq, g = geq, every d = 0
+    uy[kind == 0] = 0.0
+    uy[kind == 1] = -0.0         # negative zero velocity component
+    t[kind == 2] = 0.0           # T = t_ref: f_y = 0
+    ux[kind == 3] = 0.0
+    uy[kind == 3] = 0.0
+    t[kind == 3] = 0.0           # at rest and f_y = 0
+    for s in (g, o):
+        s.init_from_fields(rho, ux, uy, t)
+    f = g.get_pops("f")
+    gp = g.get_pops("g")
+    assert bits_equal(f, o.f) and bits_equal(gp, o.g)
+    # kind 4: perturbed populations, a few of them exactly zero
+    f4 = f.copy()
+    g4 = gp.copy()
+    k4 = np.nonzero(kind == 4)[0]
+    f4[k4] += 1e-4 * rng.standard_normal((k4.size, 9))
+    g4[k4] += 1e-4 * rng.standard_normal((k4.size, 9))
+    f4[k4[::7], 3] = 0.0
+    g4[k4[::5], 6] = 0.0
+    g4[k4[1::9], 0] = -0.0
+    g.set_pops("f", f4)
+    g.set_pops("g", g4)
+    o.f[:] = f4
+    o.g[:] = g4
+    for name, v in (("rho", rho), ("ux", ux), ("uy", uy), ("temperature", t)):
+        g.set_field(name, v)
+        getattr(o, name)[:] = v
+    assert bits_equal(g.get_field("uy"), o.uy)  # the -0.0 entries reached both
+    g.collide(500)
+    o.collide(500)
+    gf, gg = g.get_pops("f"), g.get_pops("g")
+    # population by population: a nonzero input keeps the reference's bits
+    # (p + a zero of either sign is p); T = 0 makes every g of those nodes 0
+    for got, ref, before in ((gf, o.f, f4), (gg, o.g, g4)):
+        nz = before != 0.0
+        assert nz.sum() > 0.5 * nz.size
+        assert bits_equal(got[nz], ref[nz]), first_mismatch(got[nz], ref[nz])
+        assert np.array_equal(got, ref)  # values everywhere (+0 == -0)
+    for name in ("nu_t", "fy_node"):
+        assert np.array_equal(g.get_field(name), getattr(o, name)), name
+
+
 def test_interleaved_reads_do_not_perturb(oracle):
     g, o = pair(oracle)
     for chunk in (3, 1, 70, 200, 5):
