@@ -119,7 +119,11 @@ int cavlbm_destroy(cavlbm_solver *s);
 
 /* CavitySolver::step (solver.rs:279-293). */
 int cavlbm_step(cavlbm_solver *s);
-/* n consecutive step() calls (the loop of main.rs:164-165 without I/O). */
+/* n consecutive step() calls (the loop of main.rs:164-165 without I/O).
+ * After the buoyancy spin-up the steady steps are replayed from captured HIP
+ * graphs; with CAVLBM_GRAPHS=0 in the environment (read once per process)
+ * every solver, batch and octree handle launches them directly instead, with
+ * the same results bit for bit. */
 int cavlbm_run(cavlbm_solver *s, uint64_t nsteps);
 /* Blocks until queued work on the handle's stream has finished. */
 int cavlbm_synchronize(cavlbm_solver *s);
