@@ -327,23 +327,43 @@ template <> struct Arith<float> {
 // ---------------------------------------------------------------------------
 // storage traits; offsets are element offsets within one plane.
 template <int S> struct StorageT;
+// The random words of a node's stochastically rounded stores (FP16-offset
+// storage; unused by the other storages).
+struct SrKey {
+  uint32_t f, g;
+};
+
 template <> struct StorageT<ST_FP64> {
   using T = double;
   using R = double;
   static constexpr bool kTPlane = false;
+  __device__ static __forceinline__ SrKey key(unsigned, const R (&)[9], const R (&)[9]) {
+    return SrKey{0u, 0u};
+  }
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
   __device__ static __forceinline__ R load_g(const T* p, unsigned o, int, R) { return p[o]; }
-  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R) { p[o] = v; }
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v, const SrKey&) {
+    p[o] = v;
+  }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R, const SrKey&) {
+    p[o] = v;
+  }
 };
 template <> struct StorageT<ST_FP32> {
   using T = float;
   using R = float;
   static constexpr bool kTPlane = false;
+  __device__ static __forceinline__ SrKey key(unsigned, const R (&)[9], const R (&)[9]) {
+    return SrKey{0u, 0u};
+  }
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int) { return p[o]; }
   __device__ static __forceinline__ R load_g(const T* p, unsigned o, int, R) { return p[o]; }
-  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v) { p[o] = v; }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R) { p[o] = v; }
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int, R v, const SrKey&) {
+    p[o] = v;
+  }
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int, R v, R, const SrKey&) {
+    p[o] = v;
+  }
 };
 // The FP32 value x as an opaque register value: keeps the compiler from
 // fusing the FP32 operation that produced x with a following FP32 -> FP16
@@ -355,26 +375,83 @@ __device__ __forceinline__ float opaque_f32(float x) {
   return x;
 }
 
+// A 32-bit mixing function (xor-shift-multiply rounds): every output bit
+// depends on every input bit.
+__device__ __forceinline__ uint32_t sr_mix(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
+// FP32 -> FP16 with stochastic rounding (v_cvt_sr_f16_f32): rounds away from
+// zero when the top 13 bits of r plus the 13 dropped significand bits carry,
+// toward zero otherwise, so the result is one of the two neighbours of x and
+// its mean over uniform r is x (normal and subnormal FP16 results).
+__device__ __forceinline__ __half sr_half(float x, uint32_t r) {
+  typedef short short2v __attribute__((ext_vector_type(2)));
+  const short2v z = {0, 0};
+  const short2v y = __builtin_amdgcn_cvt_sr_f16_f32(z, x, r, false);
+  return __builtin_bit_cast(__half, (unsigned short)y[0]);
+}
+
 // FP16 storage of the deviations from each population's local rest state:
-// f_i - w_i, and g_i - w_i*T_j with T_j the temperature of the node j that
-// wrote the value (kept in one FP32 plane after the 18 FP16 planes).  The
-// stored deviations are O(1e-2) of the populations, so the ~11 significant
-// bits of FP16 act on them rather than on the populations themselves.
+// f_i - o_i, with o_i within 2.2e-7 of the weight w_i, and g_i - w_i*T_j with
+// T_j the temperature of the node j that wrote the value (kept in one FP32
+// plane after the 18 FP16 planes).  The stored deviations are O(1e-2) of the
+// populations, so the ~11 significant bits of FP16 act on them rather than on
+// the populations themselves.  The density is 1 + sum of the f deviations
+// (storage_moments): summing the populations in FP32 would add the offsets'
+// constant low bits to every node's density and round it with the same bias
+// at every node and step.
+//
+// The deviations are rounded stochastically.  Rounding to nearest gives a
+// slowly varying deviation the same error step after step, so the errors of
+// mass, heat and momentum add up over a run; stochastic rounding makes each
+// rounding error zero on average.  The random words are a function of the
+// node index and of the FP32 bits of the 18 values the node stores (key()),
+// and each population takes a rotation of its node's word: identical states
+// round identically, so the single-step, multi-step, batched and dynamic
+// kernels still give the same bits, and no step counter is needed.
 template <> struct StorageT<ST_FP16_OFFSET> {
   using T = __half;
   using R = float;
   static constexpr bool kTPlane = true;
+  // Rest-state offsets of f: the weights rounded to multiples of 2^-22
+  // (within 2.2e-7 of 4/9, 1/9, 1/36), chosen so that they sum to exactly 1.
+  // A stored deviation h is a multiple of 2^-24, so a loaded population
+  // h + o is exact in FP32 and f - o gives h back exactly.
+  __device__ static __forceinline__ float off(int i) {
+    return i == 0 ? 233017.0f / 524288.0f
+                  : ((i & 1) ? 233017.0f / 2097152.0f : 29127.0f / 1048576.0f);
+  }
+  __device__ static __forceinline__ SrKey key(unsigned k, const R (&f)[9], const R (&g)[9]) {
+    uint32_t hf = 0u, hg = 0u;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      hf = (hf ^ __float_as_uint(f[i])) * 0x01000193u;  // FNV-style fold
+      hg = (hg ^ __float_as_uint(g[i])) * 0x01000193u;
+    }
+    const uint32_t a = sr_mix(hf ^ sr_mix(k ^ 0x9E3779B9u));
+    return SrKey{a, sr_mix(hg ^ a)};
+  }
+  __device__ static __forceinline__ uint32_t word(uint32_t key, int i) {
+    return __builtin_amdgcn_alignbit(key, key, (unsigned)(7 * i) & 31u);
+  }
   __device__ static __forceinline__ R load_f(const T* p, unsigned o, int i) {
-    return __half2float(p[o]) + (float)kW[i];
+    return __half2float(p[o]) + off(i);
   }
   __device__ static __forceinline__ R load_g(const T* p, unsigned o, int i, R t_src) {
     return __half2float(p[o]) + (float)kW[i] * t_src;
   }
-  __device__ static __forceinline__ void store_f(T* p, unsigned o, int i, R v) {
-    p[o] = __float2half_rn(opaque_f32(v - (float)kW[i]));
+  __device__ static __forceinline__ void store_f(T* p, unsigned o, int i, R v, const SrKey& rk) {
+    p[o] = sr_half(opaque_f32(v - off(i)), word(rk.f, i));
   }
-  __device__ static __forceinline__ void store_g(T* p, unsigned o, int i, R v, R t_node) {
-    p[o] = __float2half_rn(opaque_f32(v - (float)kW[i] * t_node));
+  __device__ static __forceinline__ void store_g(T* p, unsigned o, int i, R v, R t_node,
+                                                 const SrKey& rk) {
+    p[o] = sr_half(opaque_f32(v - (float)kW[i] * t_node), word(rk.g, i));
   }
 };
 
@@ -682,6 +759,50 @@ __device__ __forceinline__ void node_moments(const R (&f)[9], const R (&g)[9], R
   temp = tn;
 }
 
+// The moments of a pulled node for storage S.  FP16-offset storage forms the
+// density and the momentum from the deviations h_i = f_i - o_i (exact) and
+// rho = 1 + sum h (the offsets sum to 1, and sum c_i o_i = 0): summing the
+// populations themselves in FP32 rounds the same way at every node and
+// step, because their low bits are the offsets' constant tails, and the
+// collision turns that rounding of rho into a mass change of one sign
+// (a relative drift of 9e-4 over 20 000 steps at 64x256).
+template <int S>
+__device__ __forceinline__ void storage_moments(const typename StorageT<S>::R (&f)[9],
+                                                const typename StorageT<S>::R (&g)[9],
+                                                typename StorageT<S>::R fyp, bool wall,
+                                                typename StorageT<S>::R& rho,
+                                                typename StorageT<S>::R& ux,
+                                                typename StorageT<S>::R& uy,
+                                                typename StorageT<S>::R& temp) {
+  using ST = StorageT<S>;
+  using R = typename ST::R;
+  if constexpr (S == ST_FP16_OFFSET) {
+    R h[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) h[i] = f[i] - ST::off(i);
+    const R r = R(1.0) + (h[0] + h[1] + h[2] + h[3] + h[4] + h[5] + h[6] + h[7] + h[8]);
+    R un = h[1] + h[2] - h[4] - h[5] - h[6] + h[8];
+    R vn = h[2] + h[3] + h[4] - h[6] - h[7] - h[8];
+    const R tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+    if (r > R(kF64Eps)) {
+      const R ir = Arith<R>::rcp(r);
+      un = Arith<R>::divr(un, r, ir);
+      vn = Arith<R>::divr(vn, r, ir);
+      vn += Arith<R>::divr(R(0.5) * fyp, r, ir);
+    }
+    if (wall) {
+      un = R(0.0);
+      vn = R(0.0);
+    }
+    rho = r;
+    ux = un;
+    uy = vn;
+    temp = tn;
+  } else {
+    node_moments<R>(f, g, fyp, wall, rho, ux, uy, temp);
+  }
+}
+
 // Streaming of one node from a C-form population set in HBM: pull with
 // full-way bounce-back for f (solver.rs:890-912), pull for g followed by the
 // thermal boundary conditions (solver.rs:951-988: isothermal columns, then
@@ -728,7 +849,7 @@ __device__ __forceinline__ void pull_node(const Geom& geo, const Walls& walls,
     }
   }
   const bool wall = !INTERIOR && (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1);
-  node_moments<R>(f, g, fyp, wall, rho, ux, uy, temp);
+  storage_moments<S>(f, g, fyp, wall, rho, ux, uy, temp);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,10 +944,11 @@ __device__ __forceinline__ void step_node(const Geom& geo, const StepCtx& ctx,
     }
   } else {
     T* dst = static_cast<T*>(b.dst);
+    const SrKey rk = ST::key((unsigned)k, f, g);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
-      ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp);
+      ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i], rk);
+      ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp, rk);
     }
     if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)temp;
   }

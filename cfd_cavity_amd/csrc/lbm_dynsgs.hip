@@ -600,10 +600,11 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
     dc.nut_pre[k] = nt;
     dc.at_pre[k] = at;
   }
+  const SrKey rk = ST::key((unsigned)k, f, g);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
-    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp);
+    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i], rk);
+    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], temp, rk);
   }
   if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)temp;
 }

@@ -76,19 +76,21 @@ constexpr int kDefaultBlockFp16 = 3;
 
 // The value a population takes through a store to and a load from the
 // storage (identity for FP64 and FP32; FP16-offset rounds the deviation from
-// the rest state, the node's own temperature t for g).
+// the rest state, the node's own temperature t for g, with the random words
+// of node k and the values it stores).
 template <int S>
 __device__ __forceinline__ void storage_round(typename StorageT<S>::R (&f)[9],
                                               typename StorageT<S>::R (&g)[9],
-                                              typename StorageT<S>::R t) {
+                                              typename StorageT<S>::R t, unsigned k) {
   using ST = StorageT<S>;
   if (!ST::kTPlane) return;
+  const SrKey rk = ST::key(k, f, g);
   typename ST::T q;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    ST::store_f(&q, 0u, i, f[i]);
+    ST::store_f(&q, 0u, i, f[i], rk);
     f[i] = ST::load_f(&q, 0u, i);
-    ST::store_g(&q, 0u, i, g[i], t);
+    ST::store_g(&q, 0u, i, g[i], t, rk);
     g[i] = ST::load_g(&q, 0u, i, t);
   }
 }
@@ -182,7 +184,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
         }
       }
       collide_node<R, GEQ>(f, g, rho, ux, uy, tc, cn.c[0], R(cn.c[0].cs), fyc, nutc);
-      if (N > 1) storage_round<S>(f, g, tc);
+      if (N > 1) storage_round<S>(f, g, tc, (unsigned)(iy * geo.pitch + ix));
     }
   }
 
@@ -227,7 +229,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       }
       const bool wall = !INTERIOR && (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1);
       R rho, ux, uy;
-      node_moments<R>(f2, g2, fyp, wall, rho, ux, uy, tc);
+      storage_moments<S>(f2, g2, fyp, wall, rho, ux, uy, tc);
       if constexpr (NU) {
         if (nu_node(ix, iy)) {
           // node (2, iy) pulls from the previous region: its sources lie
@@ -247,7 +249,7 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
       }
       collide_node<R, GEQ>(f2, g2, rho, ux, uy, tc, sub_ctx(cn, s - 1), R(cn.c[0].cs), fyc,
                            nutc);
-      if (s < N) storage_round<S>(f2, g2, tc);
+      if (s < N) storage_round<S>(f2, g2, tc, (unsigned)(iy * geo.pitch + ix));
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         f[i] = f2[i];
@@ -265,10 +267,11 @@ __device__ __forceinline__ void stepn_tile(const Geom& geo, const StepCtxN& cn,
   const unsigned P32 = (unsigned)geo.plane;
   fy_out[k] = fyc;
   if (write_aux) nut[k] = nutc;
+  const SrKey rk = ST::key((unsigned)k, f, g);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i]);
-    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], tc);
+    ST::store_f(dst, (unsigned)i * P32 + (unsigned)k, i, f[i], rk);
+    ST::store_g(dst, (unsigned)(9 + i) * P32 + (unsigned)k, i, g[i], tc, rk);
   }
   if (ST::kTPlane) reinterpret_cast<float*>(dst + (unsigned)18 * P32)[k] = (float)tc;
 }

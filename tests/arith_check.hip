@@ -1,7 +1,8 @@
 // arith_check.hip — test library: runs the FP64 division, reciprocal and
 // square-root helpers of the step kernels (cfd_cavity_amd/csrc/lbm_device.hpp)
 // on given operands, so that a GPU test can compare them with IEEE results
-// computed on the host.  Built with the step kernels' FP64 flags
+// computed on the host, and the FP16-offset storage's stochastic rounding
+// (sr_half) on given values and random words.  Built with the step kernels' FP64 flags
 // (-ffp-contract=off).  Not part of the product library.
 #include "../cfd_cavity_amd/csrc/lbm_device.hpp"
 
@@ -67,3 +68,43 @@ extern "C" int arith_check(const double* a, const double* b, const double* x, co
 }
 
 extern "C" int arith_check_planes() { return kPlanes; }
+
+namespace {
+
+// out[j * nx + i] = the FP16 bits of sr_half(x[i], r[j]).
+__global__ void k_sr(const float* x, const unsigned* r, unsigned short* out, int nx, int nr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx || j >= nr) return;
+  const __half h = cavlbm::sr_half(x[i], r[j]);
+  out[(size_t)j * nx + i] = __builtin_bit_cast(unsigned short, h);
+}
+
+}  // namespace
+
+// The FP16-offset storage's stochastic rounding: x: nx host floats, r: nr
+// host random words, out: nx * nr host FP16 bit patterns (row j for r[j]).
+// Returns 0, or 1 for a bad size, or the first failing HIP call's error code.
+extern "C" int sr_check(const float* x, const unsigned* r, unsigned short* out, int nx, int nr) {
+  if (nx <= 0 || nr <= 0 || nx > (1 << 20) || nr > 65535 || (size_t)nx * nr > ((size_t)1 << 28))
+    return 1;
+  const size_t xb = (size_t)nx * sizeof(float), rb = (size_t)nr * sizeof(unsigned);
+  const size_t ob = (size_t)nx * nr * sizeof(unsigned short);
+  void* d[3] = {nullptr, nullptr, nullptr};
+  hipError_t e = hipSetDevice(0);
+  if (e == hipSuccess) e = hipMalloc(&d[0], xb);
+  if (e == hipSuccess) e = hipMalloc(&d[1], rb);
+  if (e == hipSuccess) e = hipMalloc(&d[2], ob);
+  if (e == hipSuccess) e = hipMemcpy(d[0], x, xb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d[1], r, rb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sr, dim3((nx + 255) / 256, nr), dim3(256), 0, 0, (const float*)d[0],
+                       (const unsigned*)d[1], (unsigned short*)d[2], nx, nr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, d[2], ob, hipMemcpyDeviceToHost);
+  for (int k = 0; k < 3; ++k)
+    if (d[k] != nullptr) (void)hipFree(d[k]);
+  return (int)e;
+}

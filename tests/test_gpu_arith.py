@@ -188,3 +188,89 @@ def test_operands_distinguish_the_corrections():
     for name in ("constant divisors", "near-midpoint quotients"):
         a, b, _ = g[name]
         assert np.count_nonzero(a * (1.0 / b) != a / b) > len(a) // 10, name
+
+
+# ---------------------------------------------------------------------------
+# Stochastic rounding of the FP16-offset storage (sr_half: FP32 -> FP16 with
+# v_cvt_sr_f16_f32).  The storage relies on three properties: the result is
+# one of the two FP16 neighbours of x; it is the neighbour away from zero
+# exactly when the top 13 bits of the random word plus the 13 significand
+# bits that the conversion drops carry out (FP16 normal results); and so its
+# mean over the 8192 values of those 13 bits is x (exactly for normal
+# results, within 2^-37 where the result is subnormal).
+
+FP16_MIN_NORMAL = 2.0 ** -14
+
+
+def sr_values():
+    rng = np.random.default_rng(20261204)
+    sign = lambda n: rng.choice([-1.0, 1.0], n)  # noqa: E731
+    parts = [
+        np.exp2(rng.uniform(-14, 15, 1536)) * sign(1536),     # FP16 normal range
+        np.exp2(rng.uniform(-16, -2, 1536)) * sign(1536),     # the stored deviations' range
+        np.exp2(rng.uniform(-30, -14, 768)) * sign(768),      # FP16 subnormal range
+        rng.integers(-2048, 2048, 254) * 2.0 ** -12,          # exact FP16 values
+        [0.0, -0.0],
+    ]
+    return np.concatenate([np.asarray(p, dtype=np.float64) for p in parts]).astype(np.float32)
+
+
+def sr_words():
+    rng = np.random.default_rng(77)
+    top = np.arange(8192, dtype=np.uint64) << np.uint64(19)   # every top-13-bit value, low bits 0
+    rand = rng.integers(0, 1 << 32, 2048, dtype=np.uint64)
+    return np.concatenate([top, rand]).astype(np.uint32)
+
+
+def fp16_neighbours(x):
+    """(toward zero, away from zero) FP16 neighbours of the FP32 values x, as
+    float64; equal where x is an FP16 value."""
+    xd = x.astype(np.float64)
+    h = x.astype(np.float16)
+    hd = h.astype(np.float64)
+    down = np.where(hd <= xd, h, np.nextafter(h, np.float16(-np.inf))).astype(np.float64)
+    up = np.where(hd >= xd, h, np.nextafter(h, np.float16(np.inf))).astype(np.float64)
+    toward = np.where(xd >= 0, down, up)
+    away = np.where(xd >= 0, up, down)
+    return toward, away
+
+
+def test_sr_operands_distinguish_rounding_to_nearest():
+    """Rounding to nearest, the rounding the storage replaced, fails the mean
+    check on most of the values above (host arithmetic), so the check has
+    teeth."""
+    x = sr_values()
+    rn = x.astype(np.float16).astype(np.float64)
+    assert np.count_nonzero(rn != x.astype(np.float64)) > len(x) // 2
+
+
+@pytest.mark.gpu
+def test_stochastic_rounding_to_fp16():
+    lib = load()
+    lib.sr_check.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2
+    lib.sr_check.restype = ctypes.c_int
+    x = np.ascontiguousarray(sr_values())
+    r = np.ascontiguousarray(sr_words())
+    nx, nr = len(x), len(r)
+    out = np.empty((nr, nx), dtype=np.uint16)
+    rc = lib.sr_check(x.ctypes.data, r.ctypes.data, out.ctypes.data, nx, nr)
+    assert rc == 0, f"sr_check returned {rc}"
+    got = out.view(np.float16).astype(np.float64)
+    xd = x.astype(np.float64)
+    toward, away = fp16_neighbours(x)
+    # one of the two neighbours, for every word
+    ok = (got == toward[None, :]) | (got == away[None, :])
+    assert ok.all(), f"{np.count_nonzero(~ok)} results outside the neighbours, e.g. x " \
+        f"{xd[np.nonzero(~ok)[1][:4]]}"
+    # the carry rule, on normal results, for every word
+    normal = np.abs(toward) >= FP16_MIN_NORMAL
+    dropped = (x.view(np.uint32) & np.uint32(0x1FFF)).astype(np.int64)
+    carry = dropped[None, :] + (r.astype(np.int64) >> 19)[:, None] >= 8192
+    want = np.where(carry, away[None, :], toward[None, :])
+    bad = (got != want) & normal[None, :]
+    assert not bad.any(), f"{np.count_nonzero(bad)} normal results break the carry rule"
+    # the mean over every value of the top 13 bits
+    mean = got[:8192].sum(axis=0) / 8192.0
+    assert np.array_equal(mean[normal], xd[normal])
+    err = np.abs(mean[~normal] - xd[~normal])
+    assert err.max() <= 2.0 ** -37, err.max()

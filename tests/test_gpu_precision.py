@@ -3,8 +3,9 @@ reductions) against the FP64 path from the same initial state.  The gate is
 rel |dNu| <= 1e-4 for the FP32 form on the steady Ra=1e6 / Ra=1e7 cavities
 after 20 000 steps and on the 256x1024 Ra=1e10 configuration (utarget 0.005)
 after 10 000 steps, where the FP64 run itself is the oracle's state bit for
-bit (tests/test_golden.py).  The FP16 form's error is bounded and reported.
-Errors are printed (pytest -s)."""
+bit (tests/test_golden.py).  The FP16-offset form meets the same Nusselt
+gate (measured 3.1e-5, 6.6e-5, 2.0e-5) with a looser bound on w_max.  Errors
+are printed (pytest -s)."""
 import numpy as np
 import pytest
 
@@ -17,12 +18,12 @@ CASES = {
     "96x384_ra1e7": ((96, 384, 1.0e7, 0.71, 0.10, 0.9, 0.02, 42), 20_000),
     # 10k steps: the plumes have formed (Nu_hot 134) and the flow has not yet
     # turned chaotic (past ~15k steps the two runs decorrelate); measured
-    # FP32 errors 6.8e-6 (Nu), 2.6e-6 (w), mass 2.9e-7; FP16-offset 5.6e-4,
-    # 7.0e-5, 7.1e-4
+    # FP32 errors 6.8e-6 (Nu), 2.6e-6 (w), mass 2.9e-7; FP16-offset 2.0e-5,
+    # 1.1e-3, 1.9e-6
     "256x1024_ra1e10": ((256, 1024, 1.0e10, 0.71, 0.10, 0.9, 0.005, 42), 10_000),
 }
 # (rel dNu bound, rel d(w_max mid-height) bound, rel mass drift bound)
-BOUNDS = {"fp32": (1e-4, 1e-3, 2e-5), "fp16_offset": (5e-3, 5e-2, 1e-3)}
+BOUNDS = {"fp32": (1e-4, 1e-3, 2e-5), "fp16_offset": (1e-4, 5e-3, 2e-5)}
 
 
 def metrics(s):
