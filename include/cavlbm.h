@@ -46,9 +46,12 @@ enum {
 enum {
   CAVLBM_STORAGE_FP64 = 0,
   CAVLBM_STORAGE_FP32 = 1,
-  CAVLBM_STORAGE_FP16_OFFSET = 2 /* f stored as half(f - w_i), g as half(g - w_i T_j) with
-                                    T_j the writing node's temperature, kept in one FP32
-                                    plane after the 18 half planes */
+  CAVLBM_STORAGE_FP16_OFFSET = 2 /* f stored as half(f - o_i), o_i the weights w_i rounded
+                                    to multiples of 2^-22 that sum to 1; g as
+                                    half(g - w_i T_j) with T_j the writing node's
+                                    temperature, kept in one FP32 plane after the 18 half
+                                    planes; both rounded stochastically, with random words
+                                    hashed from the node index and the stored state */
 };
 
 enum { CAVLBM_FIELD_RHO = 0, CAVLBM_FIELD_UX = 1, CAVLBM_FIELD_UY = 2,
