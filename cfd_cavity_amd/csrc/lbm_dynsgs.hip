@@ -60,9 +60,23 @@ namespace {
 // ---------------------------------------------------------------------------
 // Per-node pieces of the model.
 
+// Square roots of the model: Arith::sqrtp, the generic sequence without its
+// 2^256 scaling of arguments below 2^-767, so IEEE sqrt for +0 and every
+// argument from 2^-767 up.  The arguments are +0 or far above that: sums of
+// squares of differences of velocities and temperatures (nonzero ones above
+// about 2^-110, the ulp of values that are themselves differences of O(1e-2)
+// populations divided by a density near 1), and products of such
+// differences, which stay above about 2^-450.  sqrt_pz maps -0 to +0, as
+// pow(-0, 0.5) does.
+__device__ __forceinline__ double sqrt_m(double x) { return Arith<double>::sqrtp(x); }
+__device__ __forceinline__ double sqrt_pz(double x) {
+  const double r = Arith<double>::sqrtp(x);
+  return x == 0.0 ? 0.0 : r;
+}
+
 // x^e for the mixed forms; every base here is >= +0.
 __device__ __forceinline__ double mixpow(double x, double e) {
-  if (e == 0.5) return sqrt(x == 0.0 ? 0.0 : x);
+  if (e == 0.5) return sqrt_pz(x);
   return pow(x, e);
 }
 
@@ -96,9 +110,10 @@ __device__ __forceinline__ void dyn_similarity(double ux, double uy, double t, d
                                                double uby, double tb, double& qsm, double& phi) {
   const double dux = ux - ubx, duy = uy - uby;
   const double du2 = dux * dux + duy * duy;
-  qsm = sqrt(0.5 * du2) / 2.0;
+  const double sq = sqrt_m(0.5 * du2);
+  qsm = sq / 2.0;
   const double dt_local = t - tb;
-  phi = 0.5 * fabs(dt_local) * sqrt(0.5 * du2);
+  phi = 0.5 * fabs(dt_local) * sq;
 }
 
 // S~ and |S~| from pi_neq of the post-stream populations with collide_node's
@@ -125,7 +140,7 @@ __device__ __forceinline__ DynStrain dyn_grid_strain(const double (&fp)[9], doub
   s.xx = -pi_xx / denom;
   s.xy = -pi_xy / denom;
   s.yy = -pi_yy / denom;
-  s.mag = sqrt(pi2) / denom;
+  s.mag = sqrt_m(pi2) / denom;
   return s;
 }
 
@@ -150,16 +165,16 @@ __device__ __forceinline__ DynTerms dyn_terms_node(const DynStrain& sh, const Dy
                                                    const DynSgsArgs& a) {
   const double shxx = sh.xx, shxy = sh.xy, shyy = sh.yy, smag_hat = sh.mag;
   const double sbxx = q.sbxx, sbyy = q.sbyy, sbxy = q.sbxy;
-  const double smag_bar = sqrt(sbxx * sbxx + 2.0 * sbxy * sbxy + sbyy * sbyy);
+  const double smag_bar = sqrt_m(sbxx * sbxx + 2.0 * sbxy * sbxy + sbyy * sbyy);
   const double gtx = q.gtx, gty = q.gty, gbx = q.gbx, gby = q.gby;
 
   // |T~| and |T-| (:702-707)
   const double th_x = gtx * shxx + gty * shxy;
   const double th_y = gtx * shxy + gty * shyy;
-  const double tmag_hat = sqrt(2.0 * (th_x * th_x + th_y * th_y));
+  const double tmag_hat = sqrt_m(2.0 * (th_x * th_x + th_y * th_y));
   const double tb_x = gbx * sbxx + gby * sbxy;
   const double tb_y = gbx * sbxy + gby * sbyy;
-  const double tmag_bar = sqrt(2.0 * (tb_x * tb_x + tb_y * tb_y));
+  const double tmag_bar = sqrt_m(2.0 * (tb_x * tb_x + tb_y * tb_y));
 
   // interpolated mixed forms at both levels (:733-740)
   const double dt = a.dtest, dx = 1.0, del = kDeltaT;
@@ -410,10 +425,13 @@ struct LdsPlane {
 // synchronised: the test filter on F (box_filter, solver.rs:1204-1224) and
 // the similarity scales on Q; then the contractions and mixed forms on T,
 // held in registers until R and F are no longer read and left in LDS planes
-// 0..5 over T.  Ends synchronised.
-template <int W, int TX, int TY>
+// 0..5 over T.  Ends synchronised.  `own` runs after the thread's
+// contractions and before the workgroup waits for the others': work that
+// reads no LDS (the one-launch step pulls its own node there, so that waves
+// without a second contraction node stream while the others compute).
+template <int W, int TX, int TY, class Own>
 __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs& a, double* sh,
-                                               int ox, int oy) {
+                                               int ox, int oy, Own&& own) {
   using DT = DynStepTile<W, TX, TY>;
   const int nx = geo.nx, ny = geo.ny;
   const int t = (int)threadIdx.x;
@@ -491,6 +509,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
     s.mag = sh[DT::kOffS + 3 * DT::kT + idx];
     keep[j] = dyn_terms_node(s, q, a);
   }
+  own();
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < DT::kTPer; ++j) {
@@ -569,17 +588,25 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
   }
   __syncthreads();
 
-  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy);
-
-  // 4. coefficient stage (box of the contractions) and the collision with
-  //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185); the
-  //    node's populations are pulled again (the tile's reads of step 1 are
-  //    still in L2) rather than held in registers through steps 2 and 3
+  // 4. the node's populations are pulled again (the tile's reads of step 1
+  //    are mostly still in L2) rather than held in registers through steps 2
+  //    and 3.  FP64: as soon as the thread's contractions are done, so that
+  //    waves without a second contraction node stream while waves 0-1
+  //    compute theirs (25.9 against 27.2 us per step at 256x1024); FP32 and
+  //    FP16-offset: after the contractions are in LDS (20.2 against 20.3 us).
+  constexpr bool kEarlyPull = S == ST_FP64;
   const int ix = ox + t % TX, iy = oy + t / TX;
-  if (ix >= nx || iy >= ny) return;
-  const int k = iy * pitch + ix;
+  const bool mine = ix < nx && iy < ny;
   R f[9], g[9], rho, ux, uy, temp;
-  pull(ix, iy, f, g, rho, ux, uy, temp);
+  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy, [&]() {
+    if (kEarlyPull && mine) pull(ix, iy, f, g, rho, ux, uy, temp);
+  });
+
+  // 5. coefficient stage (box of the contractions) and the collision with
+  //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185)
+  if (!mine) return;
+  if (!kEarlyPull) pull(ix, iy, f, g, rho, ux, uy, temp);
+  const int k = iy * pitch + ix;
   const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
   double num[4];
 #pragma unroll
@@ -665,7 +692,7 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_model(DynModelPair p) {
   }
   __syncthreads();
 
-  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy);
+  dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy, []() {});
 
   // 4. coefficient stage (box of the contractions) of the output nodes
   const int ix = ox + t % TX, iy = oy + t / TX;
