@@ -327,30 +327,53 @@ def copy_probe(mib: int, reps: int = 5):
     print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
 
 
-def dynamic_form(steps: int, warmup: int, cpu_s: float):
+def settle(s, ms: float) -> int:
+    """Run the solver itself for `ms` of sustained load, which brings the GPU
+    to the clocks a long run sees; returns the steps run."""
+    t0 = time.perf_counter()
+    n = 0
+    while (time.perf_counter() - t0) * 1e3 < ms:
+        s.run(200)
+        s.synchronize()
+        n += 200
+    return n
+
+
+def dynamic_form(steps: int, warmup: int, cpu_s: float, gpu_load_ms: float = 300.0):
     """The dynamic mixed-scale SGS path (solver.rs:562-828) on the same
     256x1024 workload, FP64: one launch per step (streaming, the model and
     the collision per tile), against the oracle's dynamic step on a bounded
-    CPU sample; and the FP32-storage step in its fused (3) and four-launch
-    forms."""
+    CPU sample; and the FP32-storage step in its one-launch and four-launch
+    forms.  Each form is timed right after `warmup` steps of a fresh solver
+    (*_after_idle, the GPU coming out of idle, the first steps of the
+    spin-up), then, for the reported figures, after `gpu_load_ms` of
+    sustained load on the same solver (settled clocks, past the spin-up:
+    graph replay)."""
     from oracle import pyoracle
 
     s = make_solver("fp64")
     nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
     s.set_dynamic_sgs(True)
     s.run(warmup)
+    ms_idle = s.time_run(steps)
+    load_steps = settle(s, gpu_load_ms)
     ms = s.time_run(steps)
     out = {"steps": steps, "ms_per_step": ms / steps, "mlups": nodes * steps / ms / 1e3,
+           "ms_per_step_after_idle": ms_idle / steps, "gpu_load_ms": gpu_load_ms,
+           "gpu_load_steps": load_steps,
            "launches_per_step": s.dynamic_launches, "cs_dyn_mean": s.dynamic_stats()[0]}
-    # FP32 storage: the model in one tiled launch between the post-stream
-    # fields and the collision (3 launches), and the four-launch form
+    # FP32 storage: the model, pull and collision in one launch per step, and
+    # the four-launch form
     for fuse in (True, False):
         f = make_solver("fp32")
         f.set_dynamic_fusion(fuse)
         f.set_dynamic_sgs(True)
         f.run(warmup)
+        ms_idle = f.time_run(steps)
+        settle(f, gpu_load_ms)
         ms = f.time_run(steps)
         out[f"fp32_{f.dynamic_launches}_launches_ms_per_step"] = ms / steps
+        out[f"fp32_{f.dynamic_launches}_launches_ms_per_step_after_idle"] = ms_idle / steps
         del f
     if cpu_s > 0:
         threads = min(os.cpu_count() or 1, 16)
@@ -670,7 +693,7 @@ def main():
         extra = {}
         if "dynamic_sgs" in paths:
             print("[bench] dynamic SGS 256x1024", file=sys.stderr, flush=True)
-            extra["dynamic_sgs"] = dynamic_form(500, 20, cs)
+            extra["dynamic_sgs"] = dynamic_form(500, 20, cs, a.gpu_load_ms)
         if "octree" in paths:
             oforms = set(a.octree_forms.split(","))
             for lv in (int(x) for x in a.octree_levels.split(",")):

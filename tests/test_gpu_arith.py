@@ -16,7 +16,8 @@ known failure unless its first approximation is already correctly rounded),
 quotients A/B within 1/(2B) ulp of a rounding midpoint (2^54 A = (2Q+1) B +- 1
 for significands A < B), the constant divisors, the window counts 1..64,
 exact square roots, roots within about 2^-96 relative of a rounding midpoint
-(X 2^55 = s^2 - t for small t of either sign), +0 and infinity.  The CPU test of the same sequences (test_division_algorithm.py)
+(X 2^55 = s^2 - t for small t of either sign), +0 and infinity, and random
+square roots from 2^-767 to the largest double.  The CPU test of the same sequences (test_division_algorithm.py)
 assumes a first approximation within 2^-20; the hardware's is checked here
 to be well inside that.
 """
@@ -127,6 +128,13 @@ def operand_groups():
             for sg in (1, -1) for x, _ in near_midpoint_roots(6000, sg)]
     xs = np.concatenate([ints * ints, near1, hard, [0.0, np.inf]])
     groups["square roots"] = (np.ones(len(xs)), np.ones(len(xs)), xs)
+    # the dynamic model's square roots rely on sqrtp from 2^-767 (where the
+    # generic sequence starts to scale) to the largest double
+    n = 1 << 17
+    xw = np.concatenate([np.exp2(rng.uniform(-767, 1023.99, n - 4)),
+                         [2.0 ** -767, np.nextafter(2.0 ** -767, 1.0),
+                          np.finfo(np.float64).max, np.nextafter(np.finfo(np.float64).max, 0.0)]])
+    groups["square roots, full range"] = (np.ones(n), np.ones(n), xw)
     n = 1 << 17
     groups["window counts"] = (rng.uniform(-64.0, 64.0, n) * np.exp2(rng.uniform(-50, 0, n)),
                                rng.integers(1, 65, n).astype(np.float64),
@@ -164,7 +172,7 @@ def test_device_helpers_equal_ieee(group):
     quot = a / b
     checks = {"Arith::rcp": (o[2], y), "Arith::sqrtp": (o[4], np.sqrt(x)),
               "DynWin::mean": (o[7], quot), "Arith::divr": (o[5], quot)}
-    if group != "square roots":
+    if not group.startswith("square roots"):
         checks["Arith::div"] = (o[3], quot)
     if group == "constant divisors":
         checks["Arith::divc"] = (o[6], quot)
