@@ -403,12 +403,38 @@ struct DynStepTile {
   static constexpr int kQW = TX + kQlo + kQhi, kQH = TY + kQlo + kQhi, kQ = kQW * kQH;
   static constexpr int kTW = TX + kTlo + kThi, kTH = TY + kTlo + kThi, kT = kTW * kTH;
   static constexpr int kTPer = (kT + kNT - 1) / kNT;  // contraction nodes per thread
+  static constexpr int kHalo = kR - TX * TY;           // nodes of R outside the tile
   // LDS planes (doubles): R at 0, F at kOffF, Q at kOffQ, strain at kOffS;
   // contractions and mixed forms at 0 once R and F are dead.
   static constexpr int kOffF = 3 * kR, kOffQ = kOffF + 3 * kF, kOffS = kOffQ + 2 * kQ;
   static constexpr int kLds = kOffS + 4 * kT;
   static_assert(6 * kT <= kOffQ, "contractions must fit over the R and F planes");
 };
+
+// Halo node j (0 <= j < kHalo) of the tile whose first output node is
+// (ox, oy): the kRlo rows of R below the tile, the kRhi rows above it (full
+// width), then, for each tile row, its kRlo nodes left and kRhi nodes right.
+template <int W, int TX, int TY>
+__device__ __forceinline__ void dyn_halo_node(int j, int ox, int oy, int& x, int& y) {
+  using DT = DynStepTile<W, TX, TY>;
+  const int rx0 = ox - DT::kRlo, ry0 = oy - DT::kRlo;
+  if (j < DT::kRlo * DT::kRW) {
+    y = ry0 + j / DT::kRW;
+    x = rx0 + j % DT::kRW;
+    return;
+  }
+  j -= DT::kRlo * DT::kRW;
+  if (j < DT::kRhi * DT::kRW) {
+    y = oy + TY + j / DT::kRW;
+    x = rx0 + j % DT::kRW;
+    return;
+  }
+  j -= DT::kRhi * DT::kRW;
+  constexpr int kE = DT::kRlo + DT::kRhi;
+  y = oy + j / kE;
+  const int c = j % kE;
+  x = c < DT::kRlo ? rx0 + c : ox + TX + (c - DT::kRlo);
+}
 
 // LDS plane accessor in global node coordinates: plane of width `w` whose
 // local (0, 0) is the global node (x0, y0).
@@ -576,36 +602,52 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
     else
       pull_node<S, GEQ, false>(geo, walls, src, fyp, x, y, pf, pg, r, u, v, tt);
   };
-  for (int idx = t; idx < DT::kR; idx += DT::kNT) {
-    const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
-    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
-    R pf[9], pg[9], r, u, v, tt;
+  auto pull_put = [&](int x, int y, R (&pf)[9], R (&pg)[9], R& r, R& u, R& v, R& tt) {
     pull(x, y, pf, pg, r, u, v, tt);
     double pd[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) pd[i] = (double)pf[i];
     put_node(x, y, pd, (double)r, (double)u, (double)v, (double)tt);
-  }
-  __syncthreads();
-
-  // 4. the node's populations are pulled again (the tile's reads of step 1
-  //    are mostly still in L2) rather than held in registers through steps 2
-  //    and 3.  FP64: as soon as the thread's contractions are done, so that
-  //    waves without a second contraction node stream while waves 0-1
-  //    compute theirs (25.9 against 27.2 us per step at 256x1024); FP32 and
-  //    FP16-offset: after the contractions are in LDS (20.2 against 20.3 us).
-  constexpr bool kEarlyPull = S == ST_FP64;
+  };
+  // FP32 and FP16-offset storage pull the thread's own output node first and
+  // keep its post-stream state in registers through steps 2 and 3 (22
+  // VGPRs), then the halo nodes (19.7 -> 19.0 us per FP32 step at 256x1024).
+  // FP64 (44 VGPRs) does not fit beside the model at 1024 threads: it pulls
+  // R in row order (the own-first order measured 25.8 against 25.2 us) and
+  // pulls its node again (mostly from L2) as soon as the thread's
+  // contractions are done, so that waves without a second contraction node
+  // stream while waves 0-1 compute theirs (25.9 against 27.2 us).
+  constexpr bool kRepull = S == ST_FP64;
   const int ix = ox + t % TX, iy = oy + t / TX;
   const bool mine = ix < nx && iy < ny;
   R f[9], g[9], rho, ux, uy, temp;
+  if constexpr (kRepull) {
+    for (int idx = t; idx < DT::kR; idx += DT::kNT) {
+      const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
+      if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+      R pf[9], pg[9], r, u, v, tt;
+      pull_put(x, y, pf, pg, r, u, v, tt);
+    }
+  } else {
+    if (mine) pull_put(ix, iy, f, g, rho, ux, uy, temp);
+    for (int j = t; j < DT::kHalo; j += DT::kNT) {
+      int x, y;
+      dyn_halo_node<W, TX, TY>(j, ox, oy, x, y);
+      if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+      R pf[9], pg[9], r, u, v, tt;
+      pull_put(x, y, pf, pg, r, u, v, tt);
+    }
+  }
+  __syncthreads();
+
+  // 2-4. filter, similarity, contractions; FP64 pulls its node again
   dyn_tile_terms<W, TX, TY>(geo, a, sh, ox, oy, [&]() {
-    if (kEarlyPull && mine) pull(ix, iy, f, g, rho, ux, uy, temp);
+    if (kRepull && mine) pull(ix, iy, f, g, rho, ux, uy, temp);
   });
 
   // 5. coefficient stage (box of the contractions) and the collision with
   //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185)
   if (!mine) return;
-  if (!kEarlyPull) pull(ix, iy, f, g, rho, ux, uy, temp);
   const int k = iy * pitch + ix;
   const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
   double num[4];
