@@ -21,7 +21,9 @@ solvers: right after set-up (reported as *_after_idle) and right after
 
 The CPU baseline is the oracle (a C restatement of the reference's row-band
 rayon path) timed on this host's cores on a bounded sample of the same
-workload.
+workload.  The other paths (batched cavities, the dynamic SGS step, the
+octree, the adjoint, the run loop's time averages and snapshots) are timed
+beside the headline under "batched" and "other_paths".
 """
 from __future__ import annotations
 
@@ -428,6 +430,78 @@ def octree_form(levels: int, steps: int, cpu_s: float, cfg=(64, 256, 8), dynamic
     return out
 
 
+def run_loop_form(samples: int = 50, every: int = 48):
+    """The run loop's helpers on the FP64 256x1024 workload: one sample of
+    the on-device time averages (post.rs:11-57) from the C-form state (the
+    M-form fields are materialised first, then k_avg_acc adds u, v, T, nu_t
+    and u^2 to five FP64 sum planes), at a cadence of `every` steps; the
+    accumulation alone (state already materialised; 112 algorithmic bytes per
+    node: 4 planes and 5 sums read, 5 sums written); and a CAVSNAP1 snapshot
+    written atomically from the device fields and read back into a restart
+    (snapshot.rs:10-131, main.rs:104-128, 284-300).  Host clocks around
+    synchronised regions, settled clocks (the solver runs for `gpu_load_ms`
+    first)."""
+    import tempfile
+
+    import numpy as np
+
+    from cfd_cavity_amd import snapshot
+
+    nodes = WORKLOAD["nx"] * WORKLOAD["ny"]
+    s = make_solver("fp64")
+    settle(s, 300.0)
+
+    def region(fn):
+        s.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        s.synchronize()
+        return time.perf_counter() - t0
+
+    def steps_only():
+        for _ in range(samples):
+            s.run(every)
+
+    def steps_and_samples():
+        for _ in range(samples):
+            s.run(every)
+            s.average_accumulate()
+
+    def accumulate_only():
+        for _ in range(4 * samples):
+            s.average_accumulate()
+
+    s.average_reset()
+    base = min(region(steps_only) for _ in range(3))
+    with_samples = min(region(steps_and_samples) for _ in range(3))
+    acc = min(region(accumulate_only) for _ in range(3)) / (4 * samples)
+    sample = (with_samples - base) / samples
+    out = {"config": "256x1024 Ra=1e10 FP64, a sample every %d steps" % every,
+           "ms_per_step_without_samples": base * 1e3 / (samples * every),
+           "sample_us": sample * 1e6,
+           "sample_over_step": sample / (base / (samples * every)),
+           "accumulate_us": acc * 1e6,
+           "accumulate_gbs": 112.0 * nodes / acc / 1e9,
+           "average_count": s.average_count()}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "snap.bin")
+        t0 = time.perf_counter()
+        snapshot.save(s, path)
+        out["snapshot_write_ms"] = (time.perf_counter() - t0) * 1e3
+        out["snapshot_bytes"] = os.path.getsize(path)
+        r = make_solver("fp64")
+        t0 = time.perf_counter()
+        snapshot.restart(r, path)
+        r.synchronize()
+        out["restart_ms"] = (time.perf_counter() - t0) * 1e3
+        same = all(np.array_equal(np.asarray(s.get_field(n)), np.asarray(r.get_field(n)))
+                   for n in ("rho", "ux", "uy", "temperature"))
+        out["restart_fields_equal"] = bool(same)
+        r.close()
+    s.close()
+    return out
+
+
 def adjoint_form():
     """The adjoint form's workload: 96x384, Ra=1e7, FP64, J = Nu_hot averaged
     over steps 8k-10k from the seeded spun-up state, theta = 0.5 +- 0.01
@@ -642,7 +716,7 @@ def main():
                     help="octree forms: plain (each level count) and dynamic (2 levels)")
     ap.add_argument("--precision-steps", type=int, default=200000,
                     help="steps of the FP64/FP32/FP16-offset comparison (0: skip)")
-    ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree,adjoint",
+    ap.add_argument("--paths", default="uniform,batched,dynamic_sgs,octree,adjoint,run_loop",
                     help="paths to measure (a subset prints a partial line, for profiling)")
     ap.add_argument("--clock-probe", action="store_true",
                     help="only run the launch-time-against-GPU-activity phases (for a trace)")
@@ -706,6 +780,9 @@ def main():
         if "adjoint" in paths:
             print("[bench] adjoint form 96x384", file=sys.stderr, flush=True)
             extra["adjoint"] = adjoint_form()
+        if "run_loop" in paths:
+            print("[bench] run loop: time averages and snapshots", file=sys.stderr, flush=True)
+            extra["run_loop"] = run_loop_form()
 
     precision = gate = None
     if "uniform" in paths and a.precision_steps > 0:
