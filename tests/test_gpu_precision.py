@@ -4,8 +4,11 @@ rel |dNu| <= 1e-4 for the FP32 form on the steady Ra=1e6 / Ra=1e7 cavities
 after 20 000 steps and on the 256x1024 Ra=1e10 configuration (utarget 0.005)
 after 10 000 steps, where the FP64 run itself is the oracle's state bit for
 bit (tests/test_golden.py).  The FP16-offset form meets the same Nusselt
-gate (measured 3.1e-5, 6.6e-5, 2.0e-5) with a looser bound on w_max.  Errors
-are printed (pytest -s)."""
+gate on the two steady cases (its stochastic rounding gave 1.6e-5 to 9.6e-5
+over five random streams; this build's stream 3.1e-5 and 6.6e-5).  On the
+256x1024 Ra=1e10 case, whose plumes amplify any perturbation, the five
+streams gave 2.0e-5 to 1.7e-4 (this build 2.0e-5), and the bound is 3e-4.
+Errors are printed (pytest -s)."""
 import numpy as np
 import pytest
 
@@ -24,6 +27,7 @@ CASES = {
 }
 # (rel dNu bound, rel d(w_max mid-height) bound, rel mass drift bound)
 BOUNDS = {"fp32": (1e-4, 1e-3, 2e-5), "fp16_offset": (1e-4, 5e-3, 2e-5)}
+CASE_BOUNDS = {("fp16_offset", "256x1024_ra1e10"): (3e-4, 5e-3, 2e-5)}
 
 
 def metrics(s):
@@ -48,7 +52,7 @@ def test_precision_form_against_fp64(case, storage):
     e_m = abs(lm - m0) / m0
     print(f"\n[precision] {case} {storage}: Nu_hot {lh:.8f} vs {nh:.8f}  rel dNu {e_nu:.2e}  "
           f"rel dw_max {e_w:.2e}  mass drift {e_m:.2e} (fp64 drift {abs(m - m0) / m0:.2e})")
-    b_nu, b_w, b_m = BOUNDS[storage]
+    b_nu, b_w, b_m = CASE_BOUNDS.get((storage, case), BOUNDS[storage])
     assert np.isfinite(lh) and low.diagnostics()["all_finite"]
     assert e_nu <= b_nu
     assert e_w <= b_w
