@@ -611,11 +611,11 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
   };
   // FP32 and FP16-offset storage pull the thread's own output node first and
   // keep its post-stream state in registers through steps 2 and 3 (22
-  // VGPRs), then the halo nodes (19.7 -> 19.0 us per FP32 step at 256x1024).
+  // VGPRs), then the halo nodes (19.6 -> 18.8 us per FP32 step at 256x1024).
   // FP64 (44 VGPRs) does not fit beside the model at 1024 threads: it pulls
-  // R in row order (halo first and own nodes last, which leaves more of the
-  // second pull's lines in L2, measured 26.2 against 25.2 us) and
-  // pulls its node again (mostly from L2) as soon as the thread's
+  // R in row order (pulling the halo first and the own nodes last, so that
+  // more of the second pull's lines stay in L2, measured 26.2 against 25.2
+  // us) and pulls its node again (mostly from L2) as soon as the thread's
   // contractions are done, so that waves without a second contraction node
   // stream while waves 0-1 compute theirs (25.9 against 27.2 us).
   constexpr bool kRepull = S == ST_FP64;
