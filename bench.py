@@ -250,6 +250,9 @@ def measured_traffic(steps_per_launch: int):
 
 
 CLOCK_GHZ = 2.4          # MI355X peak engine clock
+# shader clock measured under a dense FP64 FMA load on every CU (s_memtime
+# against the 100 MHz s_memrealtime in a probe kernel, two boxes)
+CLOCK_GHZ_FP64_LOAD = 2.0
 SIMDS = 256 * 4          # 256 CUs x 4 SIMDs
 FP64_CYCLES_PER_WAVE_INST = 4   # 16 FP64 lanes per SIMD: a wave64 instruction in 4 cycles
 
@@ -258,9 +261,10 @@ def fp64_issue_bound(steps_per_launch: int, launch_ms: float):
     """The second bound of the multi-step kernel: its FP64 VALU issue time
     (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per launch, from the committed
     counter summary, at 4 cycles per wave instruction over the 1024 SIMDs at
-    the peak clock) as a fraction of the launch time, and the same for all of
-    its VALU instructions (SQ_INSTS_VALU); None when the summary holds no
-    figure for this kernel."""
+    the peak clock and at the clock measured under FP64 load) as a fraction
+    of the launch time, and the same for all of its VALU instructions
+    (SQ_INSTS_VALU); None when the summary holds no figure for this
+    kernel."""
     keys = {"fp64": f"FP64_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH",
             "valu": f"VALU_WAVE_INSTS_STEP{steps_per_launch}_PER_LAUNCH"}
     found = {}
@@ -274,18 +278,23 @@ def fp64_issue_bound(steps_per_launch: int, launch_ms: float):
     if "fp64" not in found:
         return None
 
-    def busy(n):
-        return n * FP64_CYCLES_PER_WAVE_INST / SIMDS / (CLOCK_GHZ * 1e6)
+    def busy(n, ghz=CLOCK_GHZ):
+        return n * FP64_CYCLES_PER_WAVE_INST / SIMDS / (ghz * 1e6)
 
     n = found["fp64"]
     out = {"wave_instructions_per_launch": n, "issue_ms": round(busy(n), 6),
            "issue_frac": round(busy(n) / launch_ms, 4),
-           "clock_ghz": CLOCK_GHZ, "source": "profiles/k_stepn_256x1024.md"}
+           "clock_ghz": CLOCK_GHZ,
+           "issue_frac_at_fp64_load_clock": round(busy(n, CLOCK_GHZ_FP64_LOAD) / launch_ms, 4),
+           "fp64_load_clock_ghz": CLOCK_GHZ_FP64_LOAD,
+           "source": "profiles/k_stepn_256x1024.md"}
     if "valu" in found:
         v = found["valu"]
         out.update({"all_valu_wave_instructions_per_launch": v,
                     "all_valu_issue_ms": round(busy(v), 6),
-                    "all_valu_issue_frac": round(busy(v) / launch_ms, 4)})
+                    "all_valu_issue_frac": round(busy(v) / launch_ms, 4),
+                    "all_valu_issue_frac_at_fp64_load_clock":
+                        round(busy(v, CLOCK_GHZ_FP64_LOAD) / launch_ms, 4)})
     return out
 
 
