@@ -89,6 +89,22 @@ __device__ __forceinline__ double box_mean(const DynWin& w, const Acc& f) {
   return w.mean(s);
 }
 
+// Box sum over the unclamped W x W window whose first node is (x0, y0), for
+// accessors that read +0 at every position outside the lattice (the LDS
+// planes of a tile).  The unrolled loop adds the clamped window's terms in
+// the same (y outer, x inner) order with +0 terms between them; a sum
+// started at +0 is never -0, so s + (+0) = s and the result is the clamped
+// window's sum bit for bit.
+template <int W, class Acc>
+__device__ __forceinline__ double box_sum_w(const Acc& f, int x0, int y0) {
+  double s = 0.0;
+#pragma unroll
+  for (int dy = 0; dy < W; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < W; ++dx) s += f(x0 + dx, y0 + dy);
+  return s;
+}
+
 // Centred differences with one-sided clamping at the walls (solver.rs:1229-1242).
 template <class Acc>
 __device__ __forceinline__ double ddx(const Acc& f, int nx, int ix, int iy) {
@@ -217,8 +233,11 @@ __device__ __forceinline__ DynTerms dyn_terms_node(const DynStrain& sh, const Dy
 
 // The S-, gradient and box inputs of node (ix, iy) from accessors of the
 // resolved fields (u, v, T), the filtered fields (ub, vb, tb) and the
-// similarity scales (qs, ph); lb is filled by the caller.
-template <class AR, class AF, class AQ>
+// similarity scales (qs, ph); lb is filled by the caller.  WF = 0: the
+// similarity scales' box means over the clamped window w; WF > 0: over the
+// unclamped WF x WF window from (ix - HL, iy - HL), for accessors that read
+// +0 outside the lattice (box_sum_w).
+template <int WF, int HL, class AR, class AF, class AQ>
 __device__ __forceinline__ void dyn_local(const Geom& geo, const DynWin& w, int ix, int iy,
                                           const AR& rt, const AF& fu, const AF& fv,
                                           const AF& ft, const AQ& qs, const AQ& ph,
@@ -236,8 +255,13 @@ __device__ __forceinline__ void dyn_local(const Geom& geo, const DynWin& w, int 
   q.qsm_h = qs(ix, iy);
   q.phi_h = ph(ix, iy);
   // test-filtered similarity scales (:713-714)
-  q.qbar_h = box_mean(w, qs);
-  q.phibar_h = box_mean(w, ph);
+  if constexpr (WF > 0) {
+    q.qbar_h = w.mean(box_sum_w<WF>(qs, ix - HL, iy - HL));
+    q.phibar_h = w.mean(box_sum_w<WF>(ph, ix - HL, iy - HL));
+  } else {
+    q.qbar_h = box_mean(w, qs);
+    q.phibar_h = box_mean(w, ph);
+  }
   q.ubx = fu(ix, iy);
   q.uby = fv(ix, iy);
   q.tb = ft(ix, iy);
@@ -328,8 +352,9 @@ __global__ __launch_bounds__(kBlock) void k_dyn_terms(DynSgsPair p) {
   const int pitch = geo.pitch;
   const DynWin w = dyn_window(geo, a.width, a.half_left, ix, iy);
   DynLocal q;
-  dyn_local(geo, w, ix, iy, Plane{a.temp, pitch}, Plane{a.ubx, pitch}, Plane{a.uby, pitch},
-            Plane{a.tb, pitch}, Plane{a.qsm, pitch}, Plane{a.phi, pitch}, q);
+  dyn_local<0, 0>(geo, w, ix, iy, Plane{a.temp, pitch}, Plane{a.ubx, pitch},
+                  Plane{a.uby, pitch}, Plane{a.tb, pitch}, Plane{a.qsm, pitch},
+                  Plane{a.phi, pitch}, q);
 #pragma unroll
   for (int j = 0; j < 5; ++j) q.lb[j] = a.lb[j * P + k];
   const DynTerms r = dyn_terms_node(sh, q, a);
@@ -389,7 +414,10 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 //   R  the pulled nodes, boxed by F's windows.
 // LDS holds u, v, T on R, ub, vb, tb on F, qsm, Phi on Q and the grid-level
 // strain on T; after the contractions are computed, they and the mixed forms
-// (6 planes on T) overwrite the R and F planes.
+// (6 planes on T) overwrite the R and F planes.  The planes that are box-summed
+// (u, v, T on R, qsm, Phi on Q, the four contractions on T) hold +0 at their
+// positions outside the lattice, so every box sum runs over the unclamped
+// W x W window (box_sum_w) with no bounds in its loops.
 template <int W, int TX, int TY>
 struct DynStepTile {
   static constexpr int kNT = TX * TY;
@@ -474,14 +502,26 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
   const LdsPlane qs{sh + DT::kOffQ + 0 * DT::kQ, qx0, qy0, DT::kQW};
   const LdsPlane ph{sh + DT::kOffQ + 1 * DT::kQ, qx0, qy0, DT::kQW};
 
-  // 2. test filter on F and similarity scales on Q
+  // 2. test filter on F and similarity scales on Q (+0 at the positions of
+  //    Q outside the lattice, for the unclamped box sums)
   for (int idx = t; idx < DT::kF; idx += DT::kNT) {
     const int x = fx0 + idx % DT::kFW, y = fy0 + idx / DT::kFW;
-    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    const int lx = x - qx0, ly = y - qy0;
+    const bool in_q = lx >= 0 && lx < DT::kQW && ly >= 0 && ly < DT::kQH;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) {
+      if (in_q) {
+        sh[DT::kOffQ + 0 * DT::kQ + ly * DT::kQW + lx] = 0.0;
+        sh[DT::kOffQ + 1 * DT::kQ + ly * DT::kQW + lx] = 0.0;
+      }
+      continue;
+    }
     const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
     double s_u = 0.0, s_v = 0.0, s_t = 0.0;
-    for (int yy = w.y0; yy < w.y1; ++yy)
-      for (int xx = w.x0; xx < w.x1; ++xx) {
+#pragma unroll
+    for (int dy = 0; dy < W; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < W; ++dx) {
+        const int xx = x - DT::kHL + dx, yy = y - DT::kHL + dy;
         s_u += ru(xx, yy);
         s_v += rv(xx, yy);
         s_t += rt(xx, yy);
@@ -490,8 +530,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
     sh[DT::kOffF + 0 * DT::kF + idx] = ubx;
     sh[DT::kOffF + 1 * DT::kF + idx] = uby;
     sh[DT::kOffF + 2 * DT::kF + idx] = tb;
-    const int lx = x - qx0, ly = y - qy0;
-    if (lx >= 0 && lx < DT::kQW && ly >= 0 && ly < DT::kQH) {
+    if (in_q) {
       double q, p;
       dyn_similarity(ru(x, y), rv(x, y), rt(x, y), ubx, uby, tb, q, p);
       sh[DT::kOffQ + 0 * DT::kQ + ly * DT::kQW + lx] = q;
@@ -512,10 +551,13 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
     if (!have[j]) continue;
     const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
     DynLocal q;
-    dyn_local(geo, w, x, y, rt, fu, fv, ft, qs, ph, q);
+    dyn_local<W, DT::kHL>(geo, w, x, y, rt, fu, fv, ft, qs, ph, q);
     double s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0, s_vt = 0.0;
-    for (int yy = w.y0; yy < w.y1; ++yy)
-      for (int xx = w.x0; xx < w.x1; ++xx) {
+#pragma unroll
+    for (int dy = 0; dy < W; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < W; ++dx) {
+        const int xx = x - DT::kHL + dx, yy = y - DT::kHL + dy;
         const double u = ru(xx, yy), v = rv(xx, yy), tt = rt(xx, yy);
         s_uu += u * u;
         s_uv += u * v;
@@ -539,8 +581,15 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < DT::kTPer; ++j) {
-    if (!have[j]) continue;
     const int idx = t + j * DT::kNT;
+    if (!have[j]) {
+      // +0 contractions outside the lattice, for the unclamped box sums
+      if (idx < DT::kT) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) sh[m * DT::kT + idx] = 0.0;
+      }
+      continue;
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) sh[m * DT::kT + idx] = keep[j].t[m];
     sh[4 * DT::kT + idx] = keep[j].nu_mix_g;
@@ -594,6 +643,14 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
     }
   };
 
+  // +0 fields at a position of R outside the lattice
+  auto put_zero = [&](int x, int y) {
+    const int ir = (y - ry0) * DT::kRW + (x - rx0);
+    sh[0 * DT::kR + ir] = 0.0;
+    sh[1 * DT::kR + ir] = 0.0;
+    sh[2 * DT::kR + ir] = 0.0;
+  };
+
   // 1. streaming of every node of R that lies in the lattice
   auto pull = [&](int x, int y, R (&pf)[9], R (&pg)[9], R& r, R& u, R& v, R& tt) {
     const R fyp = fy_in[y * pitch + x];
@@ -625,16 +682,25 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
   if constexpr (kRepull) {
     for (int idx = t; idx < DT::kR; idx += DT::kNT) {
       const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
-      if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+      if (x < 0 || x >= nx || y < 0 || y >= ny) {
+        put_zero(x, y);
+        continue;
+      }
       R pf[9], pg[9], r, u, v, tt;
       pull_put(x, y, pf, pg, r, u, v, tt);
     }
   } else {
-    if (mine) pull_put(ix, iy, f, g, rho, ux, uy, temp);
+    if (mine)
+      pull_put(ix, iy, f, g, rho, ux, uy, temp);
+    else
+      put_zero(ix, iy);
     for (int j = t; j < DT::kHalo; j += DT::kNT) {
       int x, y;
       dyn_halo_node<W, TX, TY>(j, ox, oy, x, y);
-      if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+      if (x < 0 || x >= nx || y < 0 || y >= ny) {
+        put_zero(x, y);
+        continue;
+      }
       R pf[9], pg[9], r, u, v, tt;
       pull_put(x, y, pf, pg, r, u, v, tt);
     }
@@ -654,7 +720,8 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
   double num[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
-    num[m] = box_mean(w, LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW});
+    num[m] = w.mean(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW}, ix - DT::kHL,
+                                 iy - DT::kHL));
   const int it = (iy - ty0) * DT::kTW + (ix - tx0);
   double cnu, ck, nt, at;
   dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
@@ -714,7 +781,13 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_model(DynModelPair p) {
   // 1. u, v, T of every lattice node of R; the grid-level strain on T
   for (int idx = t; idx < DT::kR; idx += DT::kNT) {
     const int x = rx0 + idx % DT::kRW, y = ry0 + idx / DT::kRW;
-    if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+    if (x < 0 || x >= nx || y < 0 || y >= ny) {
+      // +0 outside the lattice, for the unclamped box sums
+      sh[0 * DT::kR + idx] = 0.0;
+      sh[1 * DT::kR + idx] = 0.0;
+      sh[2 * DT::kR + idx] = 0.0;
+      continue;
+    }
     const int k = y * pitch + x;
     const double u = a.ux[k], v = a.uy[k];
     sh[0 * DT::kR + idx] = u;
@@ -746,7 +819,8 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_model(DynModelPair p) {
     double num[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
-      num[m] = box_mean(w, LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW});
+      num[m] = w.mean(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW},
+                                   ix - DT::kHL, iy - DT::kHL));
     const int it = (iy - ty0) * DT::kTW + (ix - tx0);
     double cnu, ck;
     dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
