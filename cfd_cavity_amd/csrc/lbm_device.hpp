@@ -125,6 +125,17 @@ struct DynWin {
   int x0, x1, y0, y1;
   double cnt, rcnt;
   __device__ __forceinline__ double mean(double s) const { return quot_f64(s, cnt, rcnt); }
+  // mean(s) of a window of width W: for a power-of-two count W*W, a window
+  // that is not clamped divides by multiplying with the exact reciprocal
+  // (both round s / (W*W) once), the others as mean().
+  template <int W>
+  __device__ __forceinline__ double mean_w(double s) const {
+    constexpr int kC = W * W;
+    if constexpr ((kC & (kC - 1)) == 0) {
+      if (cnt == (double)kC) return s * (1.0 / kC);
+    }
+    return quot_f64(s, cnt, rcnt);
+  }
 };
 
 __device__ __forceinline__ DynWin dyn_window(const Geom& geo, int width, int half_left, int ix,
@@ -136,6 +147,24 @@ __device__ __forceinline__ DynWin dyn_window(const Geom& geo, int width, int hal
   w.y1 = min(iy + width - half_left, geo.ny);
   w.cnt = (double)((w.y1 - w.y0) * (w.x1 - w.x0));
   w.rcnt = rcp_f64(w.cnt);
+  return w;
+}
+
+// dyn_window for a compile-time width W: the reciprocal of the count is
+// formed only for windows that the walls clamp (an unclamped one has the
+// count W*W and the constant reciprocal RN(1 / (W*W))).
+template <int W>
+__device__ __forceinline__ DynWin dyn_window_w(const Geom& geo, int half_left, int ix, int iy) {
+  DynWin w;
+  w.x0 = ix >= half_left ? ix - half_left : 0;
+  w.x1 = min(ix + W - half_left, geo.nx);
+  w.y0 = iy >= half_left ? iy - half_left : 0;
+  w.y1 = min(iy + W - half_left, geo.ny);
+  w.cnt = (double)((w.y1 - w.y0) * (w.x1 - w.x0));
+  if (w.cnt == (double)(W * W))
+    w.rcnt = 1.0 / (double)(W * W);
+  else
+    w.rcnt = rcp_f64(w.cnt);
   return w;
 }
 

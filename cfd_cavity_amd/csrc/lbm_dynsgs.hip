@@ -256,8 +256,8 @@ __device__ __forceinline__ void dyn_local(const Geom& geo, const DynWin& w, int 
   q.phi_h = ph(ix, iy);
   // test-filtered similarity scales (:713-714)
   if constexpr (WF > 0) {
-    q.qbar_h = w.mean(box_sum_w<WF>(qs, ix - HL, iy - HL));
-    q.phibar_h = w.mean(box_sum_w<WF>(ph, ix - HL, iy - HL));
+    q.qbar_h = w.mean_w<WF>(box_sum_w<WF>(qs, ix - HL, iy - HL));
+    q.phibar_h = w.mean_w<WF>(box_sum_w<WF>(ph, ix - HL, iy - HL));
   } else {
     q.qbar_h = box_mean(w, qs);
     q.phibar_h = box_mean(w, ph);
@@ -515,7 +515,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
       }
       continue;
     }
-    const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
+    const DynWin w = dyn_window_w<W>(geo, DT::kHL, x, y);
     double s_u = 0.0, s_v = 0.0, s_t = 0.0;
 #pragma unroll
     for (int dy = 0; dy < W; ++dy)
@@ -526,7 +526,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
         s_v += rv(xx, yy);
         s_t += rt(xx, yy);
       }
-    const double ubx = w.mean(s_u), uby = w.mean(s_v), tb = w.mean(s_t);
+    const double ubx = w.mean_w<W>(s_u), uby = w.mean_w<W>(s_v), tb = w.mean_w<W>(s_t);
     sh[DT::kOffF + 0 * DT::kF + idx] = ubx;
     sh[DT::kOffF + 1 * DT::kF + idx] = uby;
     sh[DT::kOffF + 2 * DT::kF + idx] = tb;
@@ -549,7 +549,7 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
     const int x = tx0 + idx % DT::kTW, y = ty0 + idx / DT::kTW;
     have[j] = idx < DT::kT && x >= 0 && x < nx && y >= 0 && y < ny;
     if (!have[j]) continue;
-    const DynWin w = dyn_window(geo, W, DT::kHL, x, y);
+    const DynWin w = dyn_window_w<W>(geo, DT::kHL, x, y);
     DynLocal q;
     dyn_local<W, DT::kHL>(geo, w, x, y, rt, fu, fv, ft, qs, ph, q);
     double s_uu = 0.0, s_uv = 0.0, s_vv = 0.0, s_ut = 0.0, s_vt = 0.0;
@@ -565,11 +565,11 @@ __device__ __forceinline__ void dyn_tile_terms(const Geom& geo, const DynSgsArgs
         s_ut += u * tt;
         s_vt += v * tt;
       }
-    q.lb[0] = w.mean(s_uu);
-    q.lb[1] = w.mean(s_uv);
-    q.lb[2] = w.mean(s_vv);
-    q.lb[3] = w.mean(s_ut);
-    q.lb[4] = w.mean(s_vt);
+    q.lb[0] = w.mean_w<W>(s_uu);
+    q.lb[1] = w.mean_w<W>(s_uv);
+    q.lb[2] = w.mean_w<W>(s_vv);
+    q.lb[3] = w.mean_w<W>(s_ut);
+    q.lb[4] = w.mean_w<W>(s_vt);
     DynStrain s;
     s.xx = sh[DT::kOffS + 0 * DT::kT + idx];
     s.xy = sh[DT::kOffS + 1 * DT::kT + idx];
@@ -716,12 +716,12 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_step(
   //    this node's nu_t / alpha_t (collide_node, solver.rs:1054-1185)
   if (!mine) return;
   const int k = iy * pitch + ix;
-  const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
+  const DynWin w = dyn_window_w<W>(geo, DT::kHL, ix, iy);
   double num[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
-    num[m] = w.mean(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW}, ix - DT::kHL,
-                                 iy - DT::kHL));
+    num[m] = w.mean_w<W>(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW},
+                                      ix - DT::kHL, iy - DT::kHL));
   const int it = (iy - ty0) * DT::kTW + (ix - tx0);
   double cnu, ck, nt, at;
   dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
@@ -815,12 +815,12 @@ __global__ __launch_bounds__(TX * TY) void k_dyn_model(DynModelPair p) {
   double nt = 0.0, at = 0.0;
   if (ix < nx && iy < ny) {
     const int k = iy * pitch + ix;
-    const DynWin w = dyn_window(geo, W, DT::kHL, ix, iy);
+    const DynWin w = dyn_window_w<W>(geo, DT::kHL, ix, iy);
     double num[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
-      num[m] = w.mean(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW},
-                                   ix - DT::kHL, iy - DT::kHL));
+      num[m] = w.mean_w<W>(box_sum_w<W>(LdsPlane{sh + m * DT::kT, tx0, ty0, DT::kTW},
+                                        ix - DT::kHL, iy - DT::kHL));
     const int it = (iy - ty0) * DT::kTW + (ix - tx0);
     double cnu, ck;
     dyn_coef_vals(num[0], num[1], num[2], num[3], sh[4 * DT::kT + it], sh[5 * DT::kT + it],
